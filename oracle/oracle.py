@@ -1,0 +1,134 @@
+"""ctypes front end of the CPU oracle (oracle/libffb_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py use this module as the checker.  The product
+path (fastfortranboltzmann_amd) never imports it.
+
+Arrays use the reference's "xyd" layout with the one-cell halo:
+``f[i-1, y, x]`` for x = 0..nx+1, y = 0..ny+1 (numpy shape ``(9, ny+2, nx+2)``,
+C order), i.e. ``f(0:NX+1, 0:NY+1, 1:9)`` of
+source/layouts/xyd/simulation.f90:27 viewed from C.  Macro fields are
+``(ny, nx)`` (x fastest), the Fortran ``rho(NX, NY)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffb_oracle.so")
+
+SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID = 1, 2, 3, 4
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, "libffb_oracle.so"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        for p in ("d", "s"):
+            real = ctypes.c_double if p == "d" else ctypes.c_float
+            ptr = ctypes.c_void_p
+            i = ctypes.c_int
+            f = getattr(_lib, f"ffbo_{p}_init")
+            f.argtypes = [i, i, i, i, i, real, real, real, ptr, ptr, ptr, ptr]
+            f.restype = None
+            f = getattr(_lib, f"ffbo_{p}_run")
+            f.argtypes = [i, i, i, i, real, real, real, real, real, real,
+                          ptr, ptr, ptr, i, ptr, ptr, ptr]
+            f.restype = None
+            f = getattr(_lib, f"ffbo_{p}_pf_seed")
+            f.argtypes = [i, real, real, ptr]
+            f.restype = None
+        _lib.ffbo_set_threads.argtypes = [ctypes.c_int]
+        _lib.ffbo_max_threads.restype = ctypes.c_int
+    return _lib
+
+
+@dataclass
+class Params:
+    """Physical parameters of one run (names of source/settings.f90:35-53)."""
+
+    omega: float = 1.1
+    rho_0: float = 1.0
+    u_max: float = 0.1
+    n_sin: float = 2.0
+    u_wall: float = 0.1
+    u_lid: float = 0.1
+    rho_in: float = 1.00005
+    rho_out: float = 0.99995
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class OracleLattice:
+    """One single-domain run of the reference loop on the CPU."""
+
+    def __init__(self, mode: int, nx: int, ny: int, params: Params | None = None,
+                 dtype=np.float64):
+        self.mode, self.nx, self.ny = int(mode), int(nx), int(ny)
+        self.params = params or Params()
+        self.dtype = np.dtype(dtype)
+        self.p = "d" if self.dtype == np.float64 else "s"
+        self.f = np.zeros((9, ny + 2, nx + 2), self.dtype)
+        self.f_b = np.zeros_like(self.f)
+        self.rho = np.zeros((ny, nx), self.dtype)
+        self.u_x = np.zeros_like(self.rho)
+        self.u_y = np.zeros_like(self.rho)
+        self.pf_state = np.zeros((4, 3, ny), self.dtype)
+        self.steps = 0
+
+    def init(self) -> "OracleLattice":
+        P = self.params
+        getattr(lib(), f"ffbo_{self.p}_init")(
+            self.mode, self.nx, self.ny, 1, self.ny, P.rho_0, P.u_max, P.n_sin,
+            _p(self.f), _p(self.rho), _p(self.u_x), _p(self.u_y))
+        getattr(lib(), f"ffbo_{self.p}_pf_seed")(self.ny, P.rho_in, P.rho_out, _p(self.pf_state))
+        self.steps = 0
+        return self
+
+    def run(self, n_steps: int, write_macro_last: bool = True) -> "OracleLattice":
+        P = self.params
+        getattr(lib(), f"ffbo_{self.p}_run")(
+            self.mode, self.nx, self.ny, int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid,
+            P.rho_in, P.rho_out, _p(self.f), _p(self.f_b), _p(self.pf_state),
+            1 if write_macro_last else 0, _p(self.rho), _p(self.u_x), _p(self.u_y))
+        self.steps += int(n_steps)
+        return self
+
+    @property
+    def interior(self) -> np.ndarray:
+        """Populations of the owned cells, shape (9, ny, nx)."""
+        return self.f[:, 1:-1, 1:-1]
+
+    @property
+    def recv_macro_left(self) -> np.ndarray:
+        return self.pf_state[0]
+
+    @property
+    def recv_macro_right(self) -> np.ndarray:
+        return self.pf_state[1]
+
+
+def set_threads(n: int) -> None:
+    lib().ffbo_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().ffbo_max_threads())
