@@ -1,0 +1,194 @@
+"""Benchmark of the fused D2Q9 pull-stream-collide step on one MI355X.
+
+Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
+bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
+Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
+FP32 populations.  One "step" is one fused pull-stream-collide pass over the
+whole domain (one kernel launch); inputs are resident in HBM before timing.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
+
+Prints ONE JSON line: MLUPS (= N_X*N_Y*steps/s/1e6, source/layouts/xyd/main.f90:326),
+the HBM roofline of the step kernel (algorithmic 18*s bytes per cell update,
+HIP events on the library's stream), an FP64 run of the same workload, a
+copy-kernel ceiling, and the CPU baseline: the reference's own kernel
+modules compiled by flang -O3 (oracle/_ref) on one host core, or, where that
+binary is absent, the CPU oracle on all host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2}
+
+
+def cavity_omega(re: float = 1000.0, u: float = 0.1, n: int = 4096) -> float:
+    return 1.0 / (3.0 * u * n / re + 0.5)
+
+
+def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0) -> dict:
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    s = Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=cavity_omega(n=n), U_LID=0.1, RHO_0=1.0,
+                 precision=precision, storage_scale_log2=scale_log2)
+    with Lattice(s) as lat:
+        lat.init()
+        lat.step(warmup)
+        lat.sync()
+        lat.reset_timer()
+        t0 = time.perf_counter()
+        lat.step(steps)
+        lat.sync()
+        wall = time.perf_counter() - t0
+        ev_ms = lat.elapsed_ms()
+        mass = lat.moments()[0]
+    cells = n * n
+    kernel_ms = ev_ms / steps  # one launch per step for this workload
+    alg_bytes = 18 * BYTES_PER_POP[precision] * cells
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    return {
+        "mlups": cells * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_ms": kernel_ms,
+        "alg_bytes_per_launch": alg_bytes,
+        "achieved_gbs": achieved,
+        "mass_per_cell": mass / cells,
+    }
+
+
+def copy_ceiling() -> float | None:
+    from fastfortranboltzmann_amd import _lib
+    import ctypes
+
+    L = _lib.lib()
+    if not hasattr(L, "ffb_bench_copy"):
+        return None
+    gbs = ctypes.c_double()
+    if L.ffb_bench_copy(0, 1 << 30, 20, ctypes.byref(gbs)) != 0:
+        return None
+    return gbs.value
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    as recorded in profiles/traffic.txt ("<workload> <fetch_bytes> <write_bytes>")."""
+    path = os.path.join(REPO, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == workload:
+            try:
+                return float(parts[1]) + float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(precision: str) -> dict | None:
+    exe = os.path.join(REPO, "oracle", "_ref", f"bench_sl4096_{precision}", "ref_driver")
+    if os.path.exists(exe):
+        try:
+            out = subprocess.run([exe, "-"], capture_output=True, text=True, timeout=300).stdout
+            m = re.search(r"steps=(\d+).*MLUPS=\s*([0-9.eE+-]+)", out)
+            if m:
+                return {"value": float(m.group(2)), "unit": "MLUPS", "cores": 1, "kind": "reference",
+                        "sample": f"{m.group(1)} steps of the same 4096^2 cavity ({precision}); the "
+                                  "reference's kernel modules (simulation.f90, modes/*.f90) "
+                                  "compiled by flang -O3, one thread = one coarray image"}
+        except (subprocess.TimeoutExpired, OSError):
+            pass
+    # the CPU oracle (the reference restated in C) on all host cores
+    import numpy as np
+
+    from oracle.oracle import OracleLattice, Params, max_threads, set_threads
+
+    threads = min(16, os.cpu_count() or 1)
+    set_threads(threads)
+    o = OracleLattice(4, 4096, 4096, Params(omega=cavity_omega(), u_lid=0.1),
+                      np.float64 if precision == "fp64" else np.float32).init()
+    o.run(1)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        o.run(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": 4096 * 4096 * steps / dt / 1e6, "unit": "MLUPS", "cores": max_threads(),
+            "kind": "port", "sample": f"{steps} steps of the same 4096^2 cavity ({precision}), "
+                                      "CPU oracle, OpenMP"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--precision", default="fp32", choices=list(BYTES_PER_POP))
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run of the workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the domain is one device-resident "
+              "lattice and no multi-GPU path is built (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    n = args.size
+    print(f"bench: cavity {n}^2 {args.precision}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    main_run = run_gpu(n, args.precision, args.steps, args.warmup)
+    extra = {}
+    if not args.no_fp64 and args.precision != "fp64":
+        print("bench: fp64 run", file=sys.stderr, flush=True)
+        r64 = run_gpu(n, "fp64", max(1, args.steps // 2), args.warmup)
+        extra["fp64"] = {"mlups": round(r64["mlups"], 1), "achieved_gbs": round(r64["achieved_gbs"], 1),
+                         "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                         "kernel_ms": round(r64["kernel_ms"], 5)}
+    cc = copy_ceiling()
+    workload = f"sl{n}_{args.precision}"
+    traffic = traffic_from_profiles(workload)
+    cpu = None
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.precision)
+    line = {
+        "metric": "MLUPS + % HBM roofline (plain FP64/FP32 vs reduced/batched/adjoint) at stated error",
+        "value": round(main_run["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp16dev": "f16-storage/f32", "bf16dev": "bf16-storage/f32"}[args.precision],
+        "data": "synthetic (fluid at rest, deterministic initial state)",
+        "config": {"workload": f"lid-driven cavity {n}x{n}, bounce-back walls + moving lid u=0.1, "
+                               f"Re=1000 (omega={cavity_omega(n=n):.6f}), {args.precision} populations",
+                   "n_x": n, "n_y": n, "sim_mode": "sliding_lid", "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+        "kernel_ms": round(main_run["kernel_ms"], 5),
+        "copy_ceiling_gbs": round(cc, 1) if cc else None,
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,399 @@
+// d2q9_kernels.h -- device code of the fused D2Q9 pull-stream-collide BGK step
+// for gfx950 (MI355X), shared by the single-domain library (ffb_api.hip) and
+// the batched launcher.
+//
+// Data layout in HBM (one domain):
+//   populations: 9 planes ("SoA"), plane d holds f_d(x, y) at d*plane + y*pitch + x,
+//   x fastest; pitch is n_x rounded up to 64 elements so every row starts on a
+//   256-byte boundary; no halo.  Two such sets ping-pong between steps.
+// Direction index d = 0..8 is the reference's i = 1..9 minus one
+// (source/settings.f90:96-113): 0 rest, 1 (1,0), 2 (0,1), 3 (-1,0), 4 (0,-1),
+// 5 (1,1), 6 (-1,1), 7 (-1,-1), 8 (1,-1).
+//
+// The reference writes boundary values into a one-cell halo before every
+// step (source/layouts/xyd/modes/*.f90, exchange.f90:213-291) and then pulls
+// f_i(x - c_i).  Each halo entry is read by exactly one (cell, direction)
+// pair, so here the value that halo entry would hold is computed where it is
+// read (pull_edge below); the arithmetic is the reference's, in its order.
+//
+// Floating-point: every expression keeps the reference's left-to-right order
+// and the file is compiled without contraction, so FP64/FP32 results are
+// bitwise those of the reference kernels compiled without FMA contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace ffb {
+
+enum Mode { SW = 1, CF = 2, PF = 3, SL = 4 };
+
+__host__ __device__ constexpr int cx_of(int d) {
+    return (d == 1 || d == 5 || d == 8) ? 1 : ((d == 3 || d == 6 || d == 7) ? -1 : 0);
+}
+__host__ __device__ constexpr int cy_of(int d) {
+    return (d == 2 || d == 5 || d == 6) ? 1 : ((d == 4 || d == 7 || d == 8) ? -1 : 0);
+}
+__host__ __device__ constexpr int opp_of(int d) {
+    return d == 0 ? 0 : (d <= 4 ? ((d + 1) % 4) + 1 : ((d - 3) % 4) + 5);
+}
+static_assert(opp_of(1) == 3 && opp_of(2) == 4 && opp_of(3) == 1 && opp_of(4) == 2, "opp axis");
+static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) == 6, "opp diag");
+
+// ---------------------------------------------------------------------------
+// storage of one population value.  FP64 / FP32 store f itself.  The 16-bit
+// forms store d = (f - w) * 2^s and convert to FP32 for the collision.
+// ---------------------------------------------------------------------------
+template <typename S> struct Store;
+
+template <> struct Store<double> {
+    using C = double;
+    static __device__ __forceinline__ C ld(const double *p, size_t k, C, C) { return p[k]; }
+    static __device__ __forceinline__ void st(double *p, size_t k, C v, C, C) {
+        __builtin_nontemporal_store(v, p + k);
+    }
+};
+template <> struct Store<float> {
+    using C = float;
+    static __device__ __forceinline__ C ld(const float *p, size_t k, C, C) { return p[k]; }
+    static __device__ __forceinline__ void st(float *p, size_t k, C v, C, C) {
+        __builtin_nontemporal_store(v, p + k);
+    }
+};
+template <> struct Store<__half> {
+    using C = float;
+    static __device__ __forceinline__ C ld(const __half *p, size_t k, C w, C inv_scale) {
+        return w + __half2float(p[k]) * inv_scale;
+    }
+    static __device__ __forceinline__ void st(__half *p, size_t k, C v, C w, C scale) {
+        p[k] = __float2half_rn((v - w) * scale);
+    }
+};
+template <> struct Store<__hip_bfloat16> {
+    using C = float;
+    static __device__ __forceinline__ C ld(const __hip_bfloat16 *p, size_t k, C w, C inv_scale) {
+        return w + __bfloat162float(p[k]) * inv_scale;
+    }
+    static __device__ __forceinline__ void st(__hip_bfloat16 *p, size_t k, C v, C w, C scale) {
+        p[k] = __float2bfloat16((v - w) * scale);
+    }
+};
+
+// Parameters of one step over one domain.
+template <typename C> struct StepArgs {
+    const void *src;       // populations read (pitch/plane layout above)
+    void *dst;             // populations written
+    int nx, ny, pitch;
+    long long plane;
+    C omega;
+    C corr8, corr9;        // moving-wall terms ((6*w_6)*rho_0)*u_wall, couette_flow.f90:118-119
+    C rho_in, rho_out;     // Poiseuille boundary densities
+    const C *m_old;        // Poiseuille strips of the previous step [side][rho,ux,uy][ny]
+    C *m_new;              // strips of this step (written by the strip pass, read by the step)
+    C *rho_o, *ux_o, *uy_o;// macro fields (nx*ny, x fastest), written when write_macro
+    int write_macro;
+    C w[9];                // lattice weights in C precision
+    C scale, inv_scale;    // 2^s and 2^-s of the 16-bit forms
+    int n_tiles_x;         // tiles per row
+    int n_blocks;          // grid size (for the XCD-aware tile order)
+    int xcd_order;         // 1: give each XCD a contiguous band of tiles
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  With xcd_order, XCD k gets the contiguous tile range
+// [k*q + min(k, r), ...) (q = n/8, r = n%8), so tiles that share cache lines
+// (x-neighbours, the rows either side) are fetched by one L2.
+__device__ __forceinline__ int tile_of_block(int p, int n, int xcd_order) {
+    if (!xcd_order) return p;
+    const int q = n >> 3, r = n & 7, k = p & 7, j = p >> 3;
+    return k * q + (k < r ? k : r) + j;
+}
+
+template <typename S, typename C>
+__device__ __forceinline__ C LD(const StepArgs<C> &a, const S *f, int d, int x, int y) {
+    return Store<S>::ld(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x,
+                        a.w[d], a.inv_scale);
+}
+
+// Pressure-periodic increment of pressure_periodic_distribution
+// (modes/poiseuille_flow.f90:189-194): (w_d * (rho_b - rho_src)) * factor.
+template <typename C>
+__device__ __forceinline__ C pf_delta(const StepArgs<C> &a, const C *m, int d, int side, int row) {
+    const int ny = a.ny;
+    const C rho_s = m[(side * 3 + 0) * ny + row];
+    const C ux = m[(side * 3 + 1) * ny + row];
+    const C uy = m[(side * 3 + 2) * ny + row];
+    const C rho_b = side == 0 ? a.rho_in : a.rho_out;
+    const C usq = ux * ux + uy * uy;
+    const C cu = (C)cx_of(d) * ux + (C)cy_of(d) * uy;
+    const C fac = (((C)1 + (C)3 * cu) + (C)4.5 * cu * cu) - (C)1.5 * usq;
+    return a.w[d] * (rho_b - rho_s) * fac;
+}
+
+// Value of f_d pulled into cell (x, y) when the source (x - c_d) may lie
+// outside the domain: the halo entry the reference would have written.
+//   SW: periodic in x and y (shear_wave.f90:31-55)
+//   CF: bounce-back + moving top wall in y, periodic in x; the wall rule wins
+//       at the corners (couette_flow.f90:95-126)
+//   SL: bounce-back on all sides, moving lid on top; the lid wins at the top
+//       corners (sliding_lid.f90:167-203)
+//   PF: bounce-back in y; in x the periodic value plus the pressure
+//       increments of the previous strips (first application, main.f90:216)
+//       and of this step's strips (second application, simulation.f90:47-52);
+//       the four corner entries are bounce-back plus the increment of the
+//       application that wrote them last.
+// PHASE 1 gives the halo after the first application only (the state the
+// strip pass reads, poiseuille_flow.f90:140-165), PHASE 2 after both.
+template <typename S, typename C, int MODE, int PHASE>
+__device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const S *f, int d, int x, int y) {
+    const int nx = a.nx, ny = a.ny;
+    int xs = x - cx_of(d), ys = y - cy_of(d);
+    const bool xo = (xs < 0) | (xs >= nx);
+    const bool yo = (ys < 0) | (ys >= ny);
+    if (MODE == SW) {
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+        return LD<S, C>(a, f, d, xs, ys);
+    }
+    if ((MODE == SL) ? (xo | yo) : yo) {
+        C v = LD<S, C>(a, f, opp_of(d), x, y);
+        if (MODE == CF || MODE == SL) {
+            if (ys >= ny) {
+                if (d == 7) v = v - a.corr8;
+                else if (d == 8) v = v + a.corr9;
+            }
+        }
+        if (MODE == PF && xo) {
+            const int side = xs < 0 ? 0 : 1;
+            v = v + pf_delta<C>(a, PHASE == 1 ? a.m_old : a.m_new, d, side, y);
+        }
+        return v;
+    }
+    if (xo) {
+        const int xw = xs < 0 ? xs + nx : xs - nx;
+        C v = LD<S, C>(a, f, d, xw, ys);
+        if (MODE == PF) {
+            const int side = xs < 0 ? 0 : 1;
+            v = v + pf_delta<C>(a, a.m_old, d, side, ys);
+            if (PHASE == 2) v = v + pf_delta<C>(a, a.m_new, d, side, ys);
+        }
+        return v;
+    }
+    return LD<S, C>(a, f, d, xs, ys);
+}
+
+// BGK collision of fuzed_pull_streaming_collision_local_unrolled_universal
+// (source/layouts/xyd/simulation.f90:224-293), same operation order.
+template <typename S, typename C>
+__device__ __forceinline__ void collide_store(const StepArgs<C> &a, S *g, size_t c, int x, int y,
+                                              C f1, C f2, C f3, C f4, C f5, C f6, C f7, C f8, C f9) {
+    C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
+    C ux = f2 - f4 + f6 - f7 - f8 + f9;
+    C uy = f3 - f5 + f6 + f7 - f8 - f9;
+    ux = ux / rho;
+    uy = uy / rho;
+    const C usq = ux * ux + uy * uy;
+    if (a.write_macro) {
+        const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
+        a.rho_o[m] = rho;
+        a.ux_o[m] = ux;
+        a.uy_o[m] = uy;
+    }
+    const C om = a.omega;
+    const C one = 1, three = 3, c45 = (C)4.5, c15 = (C)1.5;
+    const C w0 = a.w[0], w1 = a.w[1], w2 = a.w[5];
+    const size_t P = (size_t)a.plane;
+    C s;
+    Store<S>::st(g, c, f1 + om * (w0 * rho * (one - c15 * usq) - f1), a.w[0], a.scale);
+    Store<S>::st(g, P + c, f2 + om * (w1 * rho * (one + three * ux + c45 * ux * ux - c15 * usq) - f2), a.w[1], a.scale);
+    Store<S>::st(g, 2 * P + c, f3 + om * (w1 * rho * (one + three * uy + c45 * uy * uy - c15 * usq) - f3), a.w[2], a.scale);
+    Store<S>::st(g, 3 * P + c, f4 + om * (w1 * rho * (one - three * ux + c45 * ux * ux - c15 * usq) - f4), a.w[3], a.scale);
+    Store<S>::st(g, 4 * P + c, f5 + om * (w1 * rho * (one - three * uy + c45 * uy * uy - c15 * usq) - f5), a.w[4], a.scale);
+    s = ux + uy;
+    Store<S>::st(g, 5 * P + c, f6 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f6), a.w[5], a.scale);
+    s = -ux + uy;
+    Store<S>::st(g, 6 * P + c, f7 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f7), a.w[6], a.scale);
+    s = ux + uy;
+    Store<S>::st(g, 7 * P + c, f8 + om * (w2 * rho * (one - three * s + c45 * s * s - c15 * usq) - f8), a.w[7], a.scale);
+    s = ux - uy;
+    Store<S>::st(g, 8 * P + c, f9 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f9), a.w[8], a.scale);
+}
+
+// One fused pull-stream-collide step over one domain.  One thread per cell;
+// a workgroup is a tile of TW consecutive cells of one row.  Cells on the
+// domain edge take the boundary rule, all others pull plainly.
+template <typename S, typename C, int MODE, int TW>
+__global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_order);
+    const int ty = t / a.n_tiles_x;
+    const int x = (t - ty * a.n_tiles_x) * TW + (int)threadIdx.x;
+    const int y = ty;
+    if (x >= a.nx || y >= a.ny) return;
+    const S *__restrict__ f = static_cast<const S *>(a.src);
+    S *__restrict__ g = static_cast<S *>(a.dst);
+    const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
+    const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
+    C f1, f2, f3, f4, f5, f6, f7, f8, f9;
+    if (!edge) {
+        const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
+        f1 = Store<S>::ld(f, c, a.w[0], a.inv_scale);
+        f2 = Store<S>::ld(f, P + c - 1, a.w[1], a.inv_scale);
+        f3 = Store<S>::ld(f, 2 * P + c - W, a.w[2], a.inv_scale);
+        f4 = Store<S>::ld(f, 3 * P + c + 1, a.w[3], a.inv_scale);
+        f5 = Store<S>::ld(f, 4 * P + c + W, a.w[4], a.inv_scale);
+        f6 = Store<S>::ld(f, 5 * P + c - W - 1, a.w[5], a.inv_scale);
+        f7 = Store<S>::ld(f, 6 * P + c - W + 1, a.w[6], a.inv_scale);
+        f8 = Store<S>::ld(f, 7 * P + c + W + 1, a.w[7], a.inv_scale);
+        f9 = Store<S>::ld(f, 8 * P + c + W - 1, a.w[8], a.inv_scale);
+    } else {
+        f1 = LD<S, C>(a, f, 0, x, y);
+        f2 = pull_edge<S, C, MODE, 2>(a, f, 1, x, y);
+        f3 = pull_edge<S, C, MODE, 2>(a, f, 2, x, y);
+        f4 = pull_edge<S, C, MODE, 2>(a, f, 3, x, y);
+        f5 = pull_edge<S, C, MODE, 2>(a, f, 4, x, y);
+        f6 = pull_edge<S, C, MODE, 2>(a, f, 5, x, y);
+        f7 = pull_edge<S, C, MODE, 2>(a, f, 6, x, y);
+        f8 = pull_edge<S, C, MODE, 2>(a, f, 7, x, y);
+        f9 = pull_edge<S, C, MODE, 2>(a, f, 8, x, y);
+    }
+    collide_store<S, C>(a, g, c, x, y, f1, f2, f3, f4, f5, f6, f7, f8, f9);
+}
+
+// Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,
+// modes/poiseuille_flow.f90:92-167, then the one-domain macro exchange of
+// exchange.f90:416-431): side 0 (inlet strip, recv_macro_left) is the pulled
+// (rho, u_x, u_y) of column nx-1, side 1 (outlet, recv_macro_right) that of
+// column 0, both from the halo after the first application.
+template <typename S, typename C>
+__global__ void __launch_bounds__(256) pf_strip_kernel(StepArgs<C> a) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= 2 * a.ny) return;
+    const int side = k / a.ny, y = k - side * a.ny;
+    const int x = side == 0 ? a.nx - 1 : 0;
+    const S *f = static_cast<const S *>(a.src);
+    const C f1 = pull_edge<S, C, PF, 1>(a, f, 0, x, y);
+    const C f2 = pull_edge<S, C, PF, 1>(a, f, 1, x, y);
+    const C f3 = pull_edge<S, C, PF, 1>(a, f, 2, x, y);
+    const C f4 = pull_edge<S, C, PF, 1>(a, f, 3, x, y);
+    const C f5 = pull_edge<S, C, PF, 1>(a, f, 4, x, y);
+    const C f6 = pull_edge<S, C, PF, 1>(a, f, 5, x, y);
+    const C f7 = pull_edge<S, C, PF, 1>(a, f, 6, x, y);
+    const C f8 = pull_edge<S, C, PF, 1>(a, f, 7, x, y);
+    const C f9 = pull_edge<S, C, PF, 1>(a, f, 8, x, y);
+    const C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
+    const C ux = f2 - f4 + f6 - f7 - f8 + f9;
+    const C uy = f3 - f5 + f6 + f7 - f8 - f9;
+    C *m = a.m_new;
+    m[(side * 3 + 0) * a.ny + y] = rho;
+    m[(side * 3 + 1) * a.ny + y] = ux / rho;
+    m[(side * 3 + 2) * a.ny + y] = uy / rho;
+}
+
+// Equilibrium initial state (initialization.f90:9-198): f_d = w_d * rho_0 *
+// (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) with u = (u_x(y), 0) from a per-row
+// profile (shear wave) or u = 0.
+template <typename S, typename C>
+__global__ void __launch_bounds__(256) init_kernel(StepArgs<C> a, C rho_0, const C *ux_row) {
+    const int tiles_x = (a.nx + 255) / 256;
+    const int y = (int)blockIdx.x / tiles_x;
+    const int x = ((int)blockIdx.x - y * tiles_x) * 256 + (int)threadIdx.x;
+    if (x >= a.nx || y >= a.ny) return;
+    S *g = static_cast<S *>(a.dst);
+    const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
+    const C ux = ux_row ? ux_row[y] : (C)0;
+    const C uy = 0;
+    const C usq = ux * ux;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        C feq;
+        if (ux_row) {
+            const C cu = (C)cx_of(d) * ux + (C)cy_of(d) * uy;
+            feq = a.w[d] * rho_0 * ((((C)1 + (C)3 * cu) + (C)4.5 * cu * cu) - (C)1.5 * usq);
+        } else {
+            feq = a.w[d] * rho_0;
+        }
+        Store<S>::st(g, (size_t)d * (size_t)a.plane + c, feq, a.w[d], a.scale);
+    }
+    if (a.rho_o) {
+        const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
+        a.rho_o[m] = rho_0;
+        a.ux_o[m] = ux;
+        a.uy_o[m] = uy;
+    }
+}
+
+// FP64 sums of mass and momentum over the owned cells: one partial per block
+// (deterministic order), reduced by moments_final_kernel.
+template <typename S, typename C>
+__global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, double *partial) {
+    __shared__ double sh[3][256];
+    double m = 0, mx = 0, my = 0;
+    const long long n = (long long)a.nx * a.ny;
+    const S *f = static_cast<const S *>(a.src);
+    for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const int y = (int)(k / a.nx), x = (int)(k - (long long)y * a.nx);
+        const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            double v;
+            if (sizeof(S) == 2)
+                v = (double)a.w[d] + (double)Store<S>::ld(f, (size_t)d * (size_t)a.plane + c, 0, 1) *
+                                         (double)a.inv_scale;
+            else
+                v = (double)Store<S>::ld(f, (size_t)d * (size_t)a.plane + c, a.w[d], a.inv_scale);
+            m += v;
+            mx += (double)cx_of(d) * v;
+            my += (double)cy_of(d) * v;
+        }
+    }
+    sh[0][threadIdx.x] = m;
+    sh[1][threadIdx.x] = mx;
+    sh[2][threadIdx.x] = my;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = sh[0][0];
+        partial[3 * blockIdx.x + 1] = sh[1][0];
+        partial[3 * blockIdx.x + 2] = sh[2][0];
+    }
+}
+
+__global__ void __launch_bounds__(256) moments_final_kernel(const double *partial, int n, double *out) {
+    __shared__ double sh[3][256];
+    double m = 0, mx = 0, my = 0;
+    for (int k = (int)threadIdx.x; k < n; k += 256) {
+        m += partial[3 * k];
+        mx += partial[3 * k + 1];
+        my += partial[3 * k + 2];
+    }
+    sh[0][threadIdx.x] = m;
+    sh[1][threadIdx.x] = mx;
+    sh[2][threadIdx.x] = my;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+}  // namespace ffb

@@ -1,0 +1,558 @@
+// ffb_api.hip -- C ABI (include/ffb.h) of the single-domain D2Q9 solver:
+// handle, device buffers, stream, launches and host conversions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "d2q9_kernels.h"
+#include "ffb.h"
+#include "ffb_internal.h"
+
+using namespace ffb;
+
+namespace ffb {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+}  // namespace ffb
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ffb::fail(FFB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+struct ffb_lattice {
+    ffb_config_t cfg;
+    int pitch = 0;           // elements per row (n_x rounded up to 64)
+    long long plane = 0;     // elements per population plane
+    size_t elem = 0;         // bytes per stored population
+    size_t celem = 0;        // bytes per compute value (macro fields, strips)
+    void *buf[2] = {nullptr, nullptr};
+    int cur = 0;             // buffer the next step reads
+    void *macro[3] = {nullptr, nullptr, nullptr};
+    void *strips[2] = {nullptr, nullptr};  // Poiseuille strips, [side][3][ny] each
+    int strip_old = 0;
+    void *ux_row = nullptr;  // shear-wave initial profile
+    double *mom_partial = nullptr;
+    double *mom_out = nullptr;
+    int mom_blocks = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timing = false;
+    long long steps = 0;
+    int tile_w = 256;
+    int xcd_order = 1;
+};
+
+namespace {
+
+template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macro) {
+    StepArgs<C> a{};
+    const ffb_config_t &c = L->cfg;
+    a.src = L->buf[L->cur];
+    a.dst = L->buf[1 - L->cur];
+    a.nx = c.n_x;
+    a.ny = c.n_y;
+    a.pitch = L->pitch;
+    a.plane = L->plane;
+    a.omega = (C)c.omega;
+    const C rho0 = (C)c.rho_0;
+    const C w6 = (C)1 / (C)36;
+    // ((6*w_6)*rho_0)*u, couette_flow.f90:118-119, sliding_lid.f90:195-196
+    const C u = (C)(c.sim_mode == FFB_COUETTE_FLOW ? c.u_wall : c.u_lid);
+    a.corr8 = (C)6 * w6 * rho0 * u;
+    a.corr9 = (C)6 * w6 * rho0 * u;
+    a.rho_in = (C)c.rho_in;
+    a.rho_out = (C)c.rho_out;
+    a.m_old = static_cast<const C *>(L->strips[L->strip_old]);
+    a.m_new = static_cast<C *>(L->strips[1 - L->strip_old]);
+    a.rho_o = static_cast<C *>(L->macro[0]);
+    a.ux_o = static_cast<C *>(L->macro[1]);
+    a.uy_o = static_cast<C *>(L->macro[2]);
+    a.write_macro = write_macro;
+    a.w[0] = (C)4 / (C)9;
+    for (int d = 1; d <= 4; ++d) a.w[d] = (C)1 / (C)9;
+    for (int d = 5; d <= 8; ++d) a.w[d] = (C)1 / (C)36;
+    const int s = (c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV) ? c.storage_scale_log2 : 0;
+    a.scale = (C)std::ldexp(1.0, s);
+    a.inv_scale = (C)std::ldexp(1.0, -s);
+    a.n_tiles_x = (c.n_x + L->tile_w - 1) / L->tile_w;
+    a.n_blocks = a.n_tiles_x * c.n_y;
+    a.xcd_order = L->xcd_order;
+    return a;
+}
+
+template <typename S, typename C, int MODE>
+int launch_step_mode(ffb_lattice *L, const StepArgs<C> &a) {
+    const dim3 grid((unsigned)a.n_blocks);
+    switch (L->tile_w) {
+    case 64: hipLaunchKernelGGL((step_kernel<S, C, MODE, 64>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((step_kernel<S, C, MODE, 128>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((step_kernel<S, C, MODE, 512>), grid, dim3(512), 0, L->stream, a); break;
+    case 1024: hipLaunchKernelGGL((step_kernel<S, C, MODE, 1024>), grid, dim3(1024), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((step_kernel<S, C, MODE, 256>), grid, dim3(256), 0, L->stream, a); break;
+    }
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+template <typename S, typename C> int launch_step(ffb_lattice *L, int write_macro) {
+    StepArgs<C> a = make_args<C>(L, write_macro);
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: return launch_step_mode<S, C, SW>(L, a);
+    case FFB_COUETTE_FLOW: return launch_step_mode<S, C, CF>(L, a);
+    case FFB_POISEUILLE_FLOW: {
+        const unsigned nb = (unsigned)((2 * a.ny + 255) / 256);
+        hipLaunchKernelGGL((pf_strip_kernel<S, C>), dim3(nb), dim3(256), 0, L->stream, a);
+        HIPCHK(hipGetLastError());
+        return launch_step_mode<S, C, PF>(L, a);
+    }
+    default: return launch_step_mode<S, C, SL>(L, a);
+    }
+}
+
+int dispatch_step(ffb_lattice *L, int write_macro) {
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: return launch_step<double, double>(L, write_macro);
+    case FFB_PREC_FP16_DEV: return launch_step<__half, float>(L, write_macro);
+    case FFB_PREC_BF16_DEV: return launch_step<__hip_bfloat16, float>(L, write_macro);
+    default: return launch_step<float, float>(L, write_macro);
+    }
+}
+
+template <typename S, typename C> int launch_init(ffb_lattice *L) {
+    StepArgs<C> a = make_args<C>(L, 0);
+    a.dst = L->buf[0];
+    const ffb_config_t &c = L->cfg;
+    const C *ux_row = nullptr;
+    if (c.sim_mode == FFB_SHEAR_WAVE) {
+        // u_x(y) = U_MAX * sin((2*PI*N_SIN*(y-1)) / N_Y), initialization.f90:42,
+        // evaluated on the host with the C library's sin in C precision.
+        std::vector<C> row((size_t)c.n_y);
+        const C pi = (C)3.141592653589793238462643383279502884197L;
+        for (int y = 0; y < c.n_y; ++y) {
+            const C arg = (((C)2 * pi) * (C)c.n_sin * (C)y) / (C)c.n_y;
+            row[(size_t)y] = (C)c.u_max * std::sin(arg);
+        }
+        HIPCHK(hipMemcpyAsync(L->ux_row, row.data(), row.size() * sizeof(C), hipMemcpyHostToDevice, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+        ux_row = static_cast<const C *>(L->ux_row);
+    }
+    const dim3 grid((unsigned)(((c.n_x + 255) / 256) * c.n_y));
+    hipLaunchKernelGGL((init_kernel<S, C>), grid, dim3(256), 0, L->stream, a, (C)c.rho_0, ux_row);
+    HIPCHK(hipGetLastError());
+    // Poiseuille strip seeds (main.f90:139-155): rho_in / rho_out, zero velocity.
+    if (c.sim_mode == FFB_POISEUILLE_FLOW) {
+        std::vector<C> seed((size_t)6 * (size_t)c.n_y, (C)0);
+        for (int y = 0; y < c.n_y; ++y) {
+            seed[(size_t)y] = (C)c.rho_in;
+            seed[(size_t)3 * c.n_y + (size_t)y] = (C)c.rho_out;
+        }
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(hipMemcpyAsync(L->strips[k], seed.data(), seed.size() * sizeof(C), hipMemcpyHostToDevice, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+    }
+    return FFB_OK;
+}
+
+int dispatch_init(ffb_lattice *L) {
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: return launch_init<double, double>(L);
+    case FFB_PREC_FP16_DEV: return launch_init<__half, float>(L);
+    case FFB_PREC_BF16_DEV: return launch_init<__hip_bfloat16, float>(L);
+    default: return launch_init<float, float>(L);
+    }
+}
+
+template <typename S, typename C> int launch_moments(ffb_lattice *L) {
+    StepArgs<C> a = make_args<C>(L, 0);
+    hipLaunchKernelGGL((moments_partial_kernel<S, C>), dim3((unsigned)L->mom_blocks), dim3(256), 0, L->stream, a,
+                       L->mom_partial);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, L->stream, L->mom_partial, L->mom_blocks,
+                       L->mom_out);
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+// host-side conversions between the stored form and the reference's values
+float half_to_float(uint16_t h) {
+    __half v;
+    std::memcpy(&v, &h, 2);
+    return __half2float(v);
+}
+uint16_t float_to_half(float f) {
+    __half v = __float2half_rn(f);
+    uint16_t h;
+    std::memcpy(&h, &v, 2);
+    return h;
+}
+float bf16_to_float(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+uint16_t float_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;  // round to nearest, ties to even
+    return (uint16_t)(u >> 16);
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) copy_kernel(const v4f *__restrict__ src, v4f *__restrict__ dst,
+                                                   long long n) {
+    for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256)
+        __builtin_nontemporal_store(src[k], dst + k);
+}
+
+double weight(int d) { return d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+
+}  // namespace
+
+extern "C" {
+
+void ffb_config_default(ffb_config_t *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    // source/settings.f90:27-63
+    cfg->n_x = 2520;
+    cfg->n_y = 2520;
+    cfg->sim_mode = FFB_SLIDING_LID;
+    cfg->precision = FFB_PREC_FP32;
+    cfg->omega = 1.1;
+    cfg->rho_0 = 1.0;
+    cfg->u_max = 0.1;
+    cfg->n_sin = 2.0;
+    cfg->u_wall = 0.1;
+    cfg->u_lid = 0.1;
+    cfg->rho_in = 1.00005;
+    cfg->rho_out = 0.99995;
+    cfg->storage_scale_log2 = 0;
+    cfg->device = 0;
+}
+
+int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
+    if (!cfg || !out) return fail(FFB_E_INVALID, "ffb_create: null argument");
+    *out = nullptr;
+    if (cfg->n_x < 1 || cfg->n_y < 1) return fail(FFB_E_INVALID, "ffb_create: n_x and n_y must be >= 1");
+    if ((long long)cfg->n_x * cfg->n_y > (1LL << 34))
+        return fail(FFB_E_INVALID, "ffb_create: domain larger than 2^34 cells");
+    if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_SLIDING_LID)
+        return fail(FFB_E_INVALID, "ffb_create: unknown sim mode " + std::to_string(cfg->sim_mode));
+    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_BF16_DEV)
+        return fail(FFB_E_INVALID, "ffb_create: unknown precision " + std::to_string(cfg->precision));
+    if ((cfg->precision == FFB_PREC_FP16_DEV || cfg->precision == FFB_PREC_BF16_DEV) &&
+        (cfg->storage_scale_log2 < -30 || cfg->storage_scale_log2 > 30))
+        return fail(FFB_E_INVALID, "ffb_create: storage_scale_log2 outside [-30, 30]");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return fail(FFB_E_INVALID, "ffb_create: device " + std::to_string(cfg->device) + " not present");
+    HIPCHK(hipSetDevice(cfg->device));
+
+    ffb_lattice *L = new ffb_lattice();
+    L->cfg = *cfg;
+    L->pitch = ((cfg->n_x + 63) / 64) * 64;
+    L->plane = (long long)L->pitch * cfg->n_y;
+    L->elem = cfg->precision == FFB_PREC_FP64 ? 8 : (cfg->precision == FFB_PREC_FP32 ? 4 : 2);
+    L->celem = cfg->precision == FFB_PREC_FP64 ? 8 : 4;
+    if (const char *tw = std::getenv("FFB_TILE_W")) L->tile_w = std::atoi(tw);
+    if (const char *xo = std::getenv("FFB_XCD_ORDER")) L->xcd_order = std::atoi(xo);
+    if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512 && L->tile_w != 1024)
+        L->tile_w = 256;
+
+    auto cleanup_fail = [&](hipError_t e, const char *what) {
+        ffb_destroy(L);
+        return fail(FFB_E_HIP, std::string("ffb_create: ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    const size_t pop_bytes = (size_t)9 * (size_t)L->plane * L->elem;
+    for (int k = 0; k < 2; ++k)
+        if ((e = hipMalloc(&L->buf[k], pop_bytes)) != hipSuccess) return cleanup_fail(e, "population buffers");
+    const size_t mbytes = (size_t)cfg->n_x * (size_t)cfg->n_y * L->celem;
+    for (int k = 0; k < 3; ++k)
+        if ((e = hipMalloc(&L->macro[k], mbytes)) != hipSuccess) return cleanup_fail(e, "macro fields");
+    const size_t sbytes = (size_t)6 * (size_t)cfg->n_y * L->celem;
+    for (int k = 0; k < 2; ++k) {
+        if ((e = hipMalloc(&L->strips[k], sbytes)) != hipSuccess) return cleanup_fail(e, "strips");
+        if ((e = hipMemset(L->strips[k], 0, sbytes)) != hipSuccess) return cleanup_fail(e, "strips");
+    }
+    if ((e = hipMalloc(&L->ux_row, (size_t)cfg->n_y * L->celem)) != hipSuccess) return cleanup_fail(e, "profile");
+    long long cells = (long long)cfg->n_x * cfg->n_y;
+    L->mom_blocks = (int)std::min<long long>(2048, (cells + 255) / 256);
+    if ((e = hipMalloc(&L->mom_partial, (size_t)L->mom_blocks * 3 * sizeof(double))) != hipSuccess)
+        return cleanup_fail(e, "moment partials");
+    if ((e = hipMalloc(&L->mom_out, 3 * sizeof(double))) != hipSuccess) return cleanup_fail(e, "moments");
+    if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) return cleanup_fail(e, "stream");
+    if ((e = hipEventCreate(&L->ev_start)) != hipSuccess) return cleanup_fail(e, "event");
+    if ((e = hipEventCreate(&L->ev_stop)) != hipSuccess) return cleanup_fail(e, "event");
+    *out = L;
+    return FFB_OK;
+}
+
+int ffb_destroy(ffb_lattice_t *L) {
+    if (!L) return FFB_OK;
+    // best effort: a failing release must not stop the others
+    (void)hipSetDevice(L->cfg.device);
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    for (void *p : L->buf) if (p) (void)hipFree(p);
+    for (void *p : L->macro) if (p) (void)hipFree(p);
+    for (void *p : L->strips) if (p) (void)hipFree(p);
+    if (L->ux_row) (void)hipFree(L->ux_row);
+    if (L->mom_partial) (void)hipFree(L->mom_partial);
+    if (L->mom_out) (void)hipFree(L->mom_out);
+    if (L->ev_start) (void)hipEventDestroy(L->ev_start);
+    if (L->ev_stop) (void)hipEventDestroy(L->ev_stop);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    delete L;
+    return FFB_OK;
+}
+
+int ffb_init(ffb_lattice_t *L) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_init: null handle");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    L->cur = 0;
+    L->strip_old = 0;
+    L->steps = 0;
+    L->timing = false;
+    int rc = dispatch_init(L);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return FFB_OK;
+}
+
+int ffb_step(ffb_lattice_t *L, int32_t n_steps, int32_t write_macro_last) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_step: null handle");
+    if (n_steps < 0) return fail(FFB_E_INVALID, "ffb_step: n_steps < 0");
+    if (n_steps == 0) return FFB_OK;
+    HIPCHK(hipSetDevice(L->cfg.device));
+    if (!L->timing) {
+        HIPCHK(hipEventRecord(L->ev_start, L->stream));
+        L->timing = true;
+    }
+    for (int32_t k = 0; k < n_steps; ++k) {
+        const int wm = (write_macro_last && k == n_steps - 1) ? 1 : 0;
+        int rc = dispatch_step(L, wm);
+        if (rc) return rc;
+        L->cur = 1 - L->cur;
+        if (L->cfg.sim_mode == FFB_POISEUILLE_FLOW) L->strip_old = 1 - L->strip_old;
+        ++L->steps;
+    }
+    HIPCHK(hipEventRecord(L->ev_stop, L->stream));
+    return FFB_OK;
+}
+
+int ffb_sync(ffb_lattice_t *L) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_sync: null handle");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return FFB_OK;
+}
+
+int ffb_read_macros(ffb_lattice_t *L, void *rho, void *u_x, void *u_y) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_read_macros: null handle");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    const size_t bytes = (size_t)L->cfg.n_x * (size_t)L->cfg.n_y * L->celem;
+    void *dst[3] = {rho, u_x, u_y};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], L->macro[k], bytes, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return FFB_OK;
+}
+
+int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
+    if (!L || !f_halo) return fail(FFB_E_INVALID, "ffb_read_populations: null argument");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    const int nx = L->cfg.n_x, ny = L->cfg.n_y;
+    std::vector<unsigned char> stage((size_t)9 * (size_t)L->plane * L->elem);
+    HIPCHK(hipMemcpyAsync(stage.data(), L->buf[L->cur], stage.size(), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
+    const int s = L->cfg.storage_scale_log2;
+    for (int d = 0; d < 9; ++d) {
+        for (int y = 0; y < ny; ++y) {
+            for (int x = 0; x < nx; ++x) {
+                const size_t src = (size_t)d * (size_t)L->plane + (size_t)y * (size_t)L->pitch + (size_t)x;
+                const size_t dst = (size_t)d * hplane + (size_t)(y + 1) * hp + (size_t)(x + 1);
+                switch (L->cfg.precision) {
+                case FFB_PREC_FP64: ((double *)f_halo)[dst] = ((const double *)stage.data())[src]; break;
+                case FFB_PREC_FP32: ((float *)f_halo)[dst] = ((const float *)stage.data())[src]; break;
+                case FFB_PREC_FP16_DEV:
+                    ((float *)f_halo)[dst] = (float)weight(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
+                                                                    (float)std::ldexp(1.0, -s);
+                    break;
+                default:
+                    ((float *)f_halo)[dst] = (float)weight(d) + bf16_to_float(((const uint16_t *)stage.data())[src]) *
+                                                                    (float)std::ldexp(1.0, -s);
+                    break;
+                }
+            }
+        }
+    }
+    // halo entries: zero
+    const size_t ce = L->celem;
+    for (int d = 0; d < 9; ++d) {
+        unsigned char *base = (unsigned char *)f_halo + (size_t)d * hplane * ce;
+        std::memset(base, 0, hp * ce);
+        std::memset(base + (size_t)(ny + 1) * hp * ce, 0, hp * ce);
+        for (int y = 1; y <= ny; ++y) {
+            std::memset(base + (size_t)y * hp * ce, 0, ce);
+            std::memset(base + ((size_t)y * hp + (size_t)(nx + 1)) * ce, 0, ce);
+        }
+    }
+    return FFB_OK;
+}
+
+int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
+    if (!L || !f_halo) return fail(FFB_E_INVALID, "ffb_write_populations: null argument");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    const int nx = L->cfg.n_x, ny = L->cfg.n_y;
+    std::vector<unsigned char> stage((size_t)9 * (size_t)L->plane * L->elem, 0);
+    const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
+    const int s = L->cfg.storage_scale_log2;
+    for (int d = 0; d < 9; ++d) {
+        for (int y = 0; y < ny; ++y) {
+            for (int x = 0; x < nx; ++x) {
+                const size_t dst = (size_t)d * (size_t)L->plane + (size_t)y * (size_t)L->pitch + (size_t)x;
+                const size_t src = (size_t)d * hplane + (size_t)(y + 1) * hp + (size_t)(x + 1);
+                switch (L->cfg.precision) {
+                case FFB_PREC_FP64: ((double *)stage.data())[dst] = ((const double *)f_halo)[src]; break;
+                case FFB_PREC_FP32: ((float *)stage.data())[dst] = ((const float *)f_halo)[src]; break;
+                case FFB_PREC_FP16_DEV:
+                    ((uint16_t *)stage.data())[dst] =
+                        float_to_half((((const float *)f_halo)[src] - (float)weight(d)) * (float)std::ldexp(1.0, s));
+                    break;
+                default:
+                    ((uint16_t *)stage.data())[dst] =
+                        float_to_bf16((((const float *)f_halo)[src] - (float)weight(d)) * (float)std::ldexp(1.0, s));
+                    break;
+                }
+            }
+        }
+    }
+    HIPCHK(hipMemcpyAsync(L->buf[L->cur], stage.data(), stage.size(), hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return FFB_OK;
+}
+
+int ffb_read_pf_strips(ffb_lattice_t *L, void *macro_left, void *macro_right) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_read_pf_strips: null handle");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    const size_t bytes = (size_t)3 * (size_t)L->cfg.n_y * L->celem;
+    unsigned char *base = (unsigned char *)L->strips[L->strip_old];
+    if (macro_left) HIPCHK(hipMemcpyAsync(macro_left, base, bytes, hipMemcpyDeviceToHost, L->stream));
+    if (macro_right) HIPCHK(hipMemcpyAsync(macro_right, base + bytes, bytes, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return FFB_OK;
+}
+
+int ffb_elapsed_ms(ffb_lattice_t *L, double *ms) {
+    if (!L || !ms) return fail(FFB_E_INVALID, "ffb_elapsed_ms: null argument");
+    *ms = 0.0;
+    if (!L->timing) return FFB_OK;
+    HIPCHK(hipEventSynchronize(L->ev_stop));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, L->ev_start, L->ev_stop));
+    *ms = (double)t;
+    return FFB_OK;
+}
+
+int ffb_reset_timer(ffb_lattice_t *L) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_reset_timer: null handle");
+    L->timing = false;
+    return FFB_OK;
+}
+
+int ffb_step_count(ffb_lattice_t *L, int64_t *steps) {
+    if (!L || !steps) return fail(FFB_E_INVALID, "ffb_step_count: null argument");
+    *steps = L->steps;
+    return FFB_OK;
+}
+
+int ffb_moments(ffb_lattice_t *L, double *mass, double *mom_x, double *mom_y) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_moments: null handle");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    int rc;
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: rc = launch_moments<double, double>(L); break;
+    case FFB_PREC_FP16_DEV: rc = launch_moments<__half, float>(L); break;
+    case FFB_PREC_BF16_DEV: rc = launch_moments<__hip_bfloat16, float>(L); break;
+    default: rc = launch_moments<float, float>(L); break;
+    }
+    if (rc) return rc;
+    double h[3];
+    HIPCHK(hipMemcpyAsync(h, L->mom_out, sizeof(h), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (mass) *mass = h[0];
+    if (mom_x) *mom_x = h[1];
+    if (mom_y) *mom_y = h[2];
+    return FFB_OK;
+}
+
+int ffb_device_info(ffb_lattice_t *L, void **cur, void **stream, int64_t *pitch, int64_t *plane) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_device_info: null handle");
+    if (cur) *cur = L->buf[L->cur];
+    if (stream) *stream = (void *)L->stream;
+    if (pitch) *pitch = L->pitch;
+    if (plane) *plane = L->plane;
+    return FFB_OK;
+}
+
+int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps) {
+    if (!gbps || bytes < 16 || iters < 1) return fail(FFB_E_INVALID, "ffb_bench_copy: bad argument");
+    HIPCHK(hipSetDevice(device));
+    const long long n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FFB_OK;
+    float ms = 0.f;
+    auto run = [&]() -> hipError_t {
+        hipError_t e;
+        if ((e = hipMalloc(&a, (size_t)n * 16)) != hipSuccess) return e;
+        if ((e = hipMalloc(&b, (size_t)n * 16)) != hipSuccess) return e;
+        if ((e = hipMemset(a, 0, (size_t)n * 16)) != hipSuccess) return e;
+        if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&e0)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&e1)) != hipSuccess) return e;
+        const unsigned grid = 256u * 32u;
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n);
+        if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
+        for (int k = 0; k < iters; ++k)
+            hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+        return hipEventElapsedTime(&ms, e0, e1);
+    };
+    hipError_t e = run();
+    if (e != hipSuccess) rc = fail(FFB_E_HIP, std::string("ffb_bench_copy: ") + hipGetErrorString(e));
+    else *gbps = 2.0 * (double)n * 16.0 * iters / ((double)ms * 1e-3) / 1e9;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
+const char *ffb_last_error(void) { return g_last_error.c_str(); }
+
+const char *ffb_version(void) { return "ffb-hip 0.1.0 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+/*
+ * ffb.h -- C ABI of the MI355X (gfx950) D2Q9 pull-stream-collide BGK solver
+ * (libffb_hip.so).
+ *
+ * The reference advances one domain per time step with
+ *   exchange_halos(...)                                   source/layouts/xyd/exchange.f90:161-378
+ *   [Poiseuille] prepare_poiseuille_flow_halos_PF + update_poiseuille_flow_macro_strips_PF
+ *                + exchange_poiseuille_macro_halos        source/layouts/xyd/main.f90:212-246
+ *   execute_local_sim_step(domain_info, halo_buffers, n_x_local, n_y_local,
+ *                          write_macro_fields, f, f_next, rho, u_x, u_y)
+ *                                                         source/layouts/xyd/simulation.f90:18-73
+ * on host arrays f(0:NX+1, 0:NY+1, 9) it owns.  Here one handle owns a
+ * device-resident domain (two ping-pong population sets, no halo: the halo
+ * writes of the reference become index rules inside the kernel) and
+ * ffb_step() performs that whole sequence, n_steps times, on the GPU.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero FFB_E_* code on
+ *     failure; ffb_last_error() then holds a message (thread-local).  The
+ *     reference stops all images with `error stop "..."` instead
+ *     (simulation.f90:59-60, domain.f90:79-81); a Fortran host maps a
+ *     non-zero status to `error stop ffb_last_error()`.
+ *   - Calls on one handle are not thread-safe; different handles are
+ *     independent.  Work is enqueued on the handle's own HIP stream and
+ *     ffb_step() returns without waiting; the ffb_read_* calls and
+ *     ffb_sync() wait.
+ *   - Host arrays use the reference's Fortran layouts: macro fields
+ *     rho(NX, NY) with x fastest (simulation.f90:32-34), populations
+ *     f(0:NX+1, 0:NY+1, 1:9) with x fastest, then y, then direction
+ *     (simulation.f90:27).  Element type is double for FFB_PREC_FP64 and
+ *     float for every other precision.
+ */
+#ifndef FFB_H
+#define FFB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* sim modes, source/settings.f90:56-59 */
+#define FFB_SHEAR_WAVE 1
+#define FFB_COUETTE_FLOW 2
+#define FFB_POISEUILLE_FLOW 3
+#define FFB_SLIDING_LID 4
+
+/* storage precision of the populations.  FP64 / FP32 are the reference's
+ * FFB_FP64 on / off builds (source/settings.f90:7-13).  The 16-bit forms
+ * store d_i = (f_i - w_i) * 2^storage_scale_log2 and collide in FP32; they
+ * have no reference counterpart. */
+#define FFB_PREC_FP32 0
+#define FFB_PREC_FP64 1
+#define FFB_PREC_FP16_DEV 2
+#define FFB_PREC_BF16_DEV 3
+
+/* status codes */
+#define FFB_OK 0
+#define FFB_E_INVALID 1   /* bad argument or configuration */
+#define FFB_E_HIP 2       /* HIP runtime error (allocation, launch, copy) */
+#define FFB_E_STATE 3     /* call not valid in the handle's current state */
+
+/* Run configuration: the compile-time parameters of source/settings.f90:21-53
+ * as run-time values. */
+typedef struct ffb_config {
+    int32_t n_x;          /* N_X, cells in x (settings.f90:23) */
+    int32_t n_y;          /* N_Y (settings.f90:24) */
+    int32_t sim_mode;     /* SIM_MODE, FFB_SHEAR_WAVE .. FFB_SLIDING_LID (settings.f90:61) */
+    int32_t precision;    /* FFB_PREC_* */
+    double omega;         /* OMEGA, BGK relaxation rate (settings.f90:37) */
+    double rho_0;         /* RHO_0 (settings.f90:36) */
+    double u_max;         /* U_MAX, shear-wave amplitude (settings.f90:38) */
+    double n_sin;         /* N_SIN, shear-wave periods over N_Y (settings.f90:39) */
+    double u_wall;        /* U_WALL, Couette top wall (settings.f90:40) */
+    double u_lid;         /* U_LID, sliding lid (settings.f90:41) */
+    double rho_in;        /* RHO_IN, Poiseuille inlet density (settings.f90:42) */
+    double rho_out;       /* RHO_OUT (settings.f90:43) */
+    int32_t storage_scale_log2; /* s of the 16-bit deviation forms; ignored otherwise */
+    int32_t device;       /* HIP device ordinal */
+} ffb_config_t;
+
+typedef struct ffb_lattice ffb_lattice_t;
+
+/* Fills *cfg with the reference defaults of source/settings.f90:27-63
+ * (2520 x 2520, sliding lid, omega 1.1, ...), FP32 storage, device 0. */
+void ffb_config_default(ffb_config_t *cfg);
+
+/* Allocates the device domain.  Replaces the reference's allocate(f_a, f_b,
+ * rho, u_x, u_y) and halo buffers (main.f90:96-105, exchange.f90:117-135). */
+int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out);
+int ffb_destroy(ffb_lattice_t *lat);
+
+/* Initial condition of cfg->sim_mode on the device, plus the Poiseuille strip
+ * seeds.  Replaces apply_condition_*_local (initialization.f90:9-198) and
+ * main.f90:139-155.  Resets the step counter. */
+int ffb_init(ffb_lattice_t *lat);
+
+/* Enqueues n_steps time steps (main.f90:194-262 loop body, incl.
+ * exchange_halos and the Poiseuille pre-pass, then execute_local_sim_step,
+ * simulation.f90:18-73).  When write_macro_last != 0 the last step also
+ * writes rho, u_x, u_y (the reference's write_macro_fields). */
+int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
+
+/* Waits for all work of the handle. */
+int ffb_sync(ffb_lattice_t *lat);
+
+/* Copies the macro fields written by the last write_macro step to host
+ * arrays of n_x*n_y elements each, x fastest (any pointer may be NULL). */
+int ffb_read_macros(ffb_lattice_t *lat, void *rho, void *u_x, void *u_y);
+
+/* Current populations as the reference array f(0:NX+1, 0:NY+1, 1:9) of the
+ * buffer the next step reads.  Halo entries are returned as 0 (on the device
+ * they do not exist). */
+int ffb_read_populations(ffb_lattice_t *lat, void *f_halo);
+
+/* Sets the current populations from f(0:NX+1, 0:NY+1, 1:9); halo entries are
+ * ignored.  16-bit forms convert to the stored deviation with rounding. */
+int ffb_write_populations(ffb_lattice_t *lat, const void *f_halo);
+
+/* Poiseuille strips that the next step will use as "old" (recv_macro_left,
+ * recv_macro_right of exchange.f90:33-40), each (NY, 3) column-major: rho,
+ * u_x, u_y.  Either pointer may be NULL. */
+int ffb_read_pf_strips(ffb_lattice_t *lat, void *macro_left, void *macro_right);
+
+/* Device time of the steps enqueued since ffb_init() or the last
+ * ffb_reset_timer(), in milliseconds (HIP events around each ffb_step()). */
+int ffb_elapsed_ms(ffb_lattice_t *lat, double *ms);
+int ffb_reset_timer(ffb_lattice_t *lat);
+
+/* Number of steps taken since ffb_init(). */
+int ffb_step_count(ffb_lattice_t *lat, int64_t *steps);
+
+/* FP64 sums over the owned cells of the current state: mass = sum rho,
+ * mom_x = sum rho u_x, mom_y = sum rho u_y (conservation drift). */
+int ffb_moments(ffb_lattice_t *lat, double *mass, double *mom_x, double *mom_y);
+
+/* Device pointers for zero-copy interop (torch, hipGraph capture):
+ * *cur = populations the next step reads, *stream = the handle's hipStream_t,
+ * *pitch = row pitch in elements, *plane = plane stride in elements. */
+int ffb_device_info(ffb_lattice_t *lat, void **cur, void **stream, int64_t *pitch, int64_t *plane);
+
+/* Measurement helper with no reference counterpart: device-to-device copy of
+ * `bytes` bytes by a plain HIP copy kernel (16-byte loads and stores),
+ * repeated `iters` times on `device`; *gbps = (read + written bytes) / time.
+ * Gives the measured HBM ceiling the step kernel is compared with. */
+int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *ffb_last_error(void);
+
+/* Version string of the library. */
+const char *ffb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFB_H */

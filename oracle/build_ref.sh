@@ -10,6 +10,8 @@
 #
 # usage: oracle/build_ref.sh <name> <fp64|fp32> FFB_SIM_MODE=<m> FFB_N_X=<nx> ...
 #   Any FFB_* definition not given takes the default below.
+#   REF_OPT (default "-O0") sets the optimisation flags; REF_RUN=0 builds
+#   without running; REF_KEEP=0 removes everything but state.bin after the run.
 # output: oracle/_ref/<name>/ref_driver and oracle/_ref/<name>/state.bin
 #
 # Test infrastructure only; needs the reference tree, so it runs in the build
@@ -34,7 +36,8 @@ declare -A defs=(
 )
 for kv in "$@"; do defs[${kv%%=*}]=${kv#*=}; done
 
-flags=(-cpp -O0 -ffp-contract=off -DFFB_USE_CMAKE_SETTINGS '-DFFB_EXPORT_NUM="ref"')
+read -r -a opt <<< "${REF_OPT:--O0}"
+flags=(-cpp "${opt[@]}" -ffp-contract=off -DFFB_USE_CMAKE_SETTINGS '-DFFB_EXPORT_NUM="ref"')
 [ "$prec" = fp64 ] && flags+=(-DFFB_FP64)
 for k in "${!defs[@]}"; do flags+=("-D$k=${defs[$k]}"); done
 
@@ -69,5 +72,11 @@ for s in "${src[@]}"; do
 done
 "$FLANG" "${flags[@]}" -c "$HERE/ref_driver.f90" -o ref_driver.o
 "$FLANG" -o ref_driver ref_driver.o "${objs[@]}"
-./ref_driver "$out/state.bin"
-echo "build_ref: $name -> $out/state.bin"
+rm -f ./*.o ./*.mod ./*.log
+if [ "${REF_RUN:-1}" = 1 ]; then
+  ./ref_driver "$out/state.bin"
+  echo "build_ref: $name -> $out/state.bin"
+  [ "${REF_KEEP:-1}" = 1 ] || rm -f ref_driver
+else
+  echo "build_ref: $name -> $out/ref_driver"
+fi

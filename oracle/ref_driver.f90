@@ -18,12 +18,14 @@
 ! The domain_t and halo_buffers_t types come from the reference's own module
 ! files (domain.f90, exchange.f90 are parsed by flang for their .mod files).
 !
-! Usage: ref_driver <output.bin>
+! Usage: ref_driver <output.bin | ->
+! Prints the wall time of the step loop and MLUPS = N_X*N_Y*N_STEPS/s/1e6
+! (main.f90:287-326).  With "-" no state is written.
 ! Output (stream, native endian): f(0:N_X+1,0:N_Y+1,9) of the current buffer
 ! after N_STEPS steps, then rho, u_x, u_y (N_X,N_Y) of the last step, then
 ! recv_macro_left, recv_macro_right (N_Y,3) (zeros unless Poiseuille).
 program ref_driver
-    use iso_fortran_env, only: int32
+    use iso_fortran_env, only: int32, int64, real64
     use domain, only: domain_t
     use exchange, only: halo_buffers_t
     use settings, only: N_X, N_Y, N_STEPS, N_DIRS, FP, SIM_MODE, &
@@ -43,6 +45,8 @@ program ref_driver
     real(FP), allocatable :: win_left(:,:), win_right(:,:)
     logical :: read_from_a, write_macro
     integer(int32) :: step, unit_out
+    integer(int64) :: clock_start, clock_end, clock_rate
+    real(real64) :: elapsed_seconds
     character(len=4096) :: out_path
 
     call get_command_argument(1, out_path)
@@ -90,6 +94,7 @@ program ref_driver
     end if
 
     read_from_a = .true.
+    call system_clock(clock_start, clock_rate)
     do step = 1, N_STEPS
         write_macro = (step == N_STEPS)
         if (read_from_a) then
@@ -99,6 +104,13 @@ program ref_driver
         end if
         read_from_a = .not. read_from_a
     end do
+    call system_clock(clock_end)
+    elapsed_seconds = real(clock_end - clock_start, real64) / real(clock_rate, real64)
+    write(*, "(a, i0, a, i0, a, es14.6, a, f12.3)") "ref_driver: steps=", N_STEPS, " cells=", &
+        int(N_X, int64) * int(N_Y, int64), " seconds=", elapsed_seconds, " MLUPS=", &
+        real(N_X, real64) * real(N_Y, real64) * real(N_STEPS, real64) / elapsed_seconds / 1.0e6_real64
+
+    if (trim(out_path) == "-") stop
 
     open(newunit=unit_out, file=trim(out_path), access="stream", form="unformatted", status="replace")
     if (read_from_a) then

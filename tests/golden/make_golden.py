@@ -75,8 +75,9 @@ def run_case(name: str) -> dict:
     lits.update(extra)
     defs = [f"FFB_SIM_MODE={mode}", f"FFB_N_X={nx}", f"FFB_N_Y={ny}", f"FFB_N_STEPS={steps}"]
     defs += [f"{k}={v}" for k, v in lits.items()]
+    env = dict(os.environ, REF_OPT="-O0", REF_RUN="1", REF_KEEP="0")
     subprocess.run(["bash", BUILD_REF, name, prec, *defs], check=True,
-                   stdout=subprocess.DEVNULL)
+                   stdout=subprocess.DEVNULL, env=env)
     dt = np.float64 if prec == "fp64" else np.float32
     raw = np.fromfile(os.path.join(REPO, "oracle", "_ref", name, "state.bin"), dtype=dt)
     n = 9 * (nx + 2) * (ny + 2)

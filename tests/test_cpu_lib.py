@@ -1,0 +1,72 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol that
+include/ffb.h declares, fills the reference defaults, and reports errors
+cleanly when no GPU is present (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(REPO, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(REPO, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(ffb_\w+)\s*\(", src, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from fastfortranboltzmann_amd import _lib
+
+    L = _lib.lib()
+    declared = _declared_symbols()
+    assert declared >= set(_lib.EXPORTS)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
+def test_config_defaults_are_reference_settings():
+    from fastfortranboltzmann_amd import _lib
+
+    c = _lib.FfbConfig()
+    _lib.lib().ffb_config_default(ctypes.byref(c))
+    # source/settings.f90:27-63
+    assert (c.n_x, c.n_y, c.sim_mode) == (2520, 2520, 4)
+    assert (c.omega, c.rho_0, c.u_max, c.n_sin) == (1.1, 1.0, 0.1, 2.0)
+    assert (c.u_wall, c.u_lid, c.rho_in, c.rho_out) == (0.1, 0.1, 1.00005, 0.99995)
+
+
+def test_invalid_config_is_rejected_with_message():
+    from fastfortranboltzmann_amd import _lib
+
+    L = _lib.lib()
+    c = _lib.FfbConfig()
+    L.ffb_config_default(ctypes.byref(c))
+    h = ctypes.c_void_p()
+    c.sim_mode = 9
+    assert L.ffb_create(ctypes.byref(c), ctypes.byref(h)) == 1
+    assert b"sim mode" in L.ffb_last_error()
+    c.sim_mode = 4
+    c.n_x = 0
+    assert L.ffb_create(ctypes.byref(c), ctypes.byref(h)) == 1
+    assert h.value is None
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU may be present")
+def test_create_without_gpu_fails_loudly():
+    from fastfortranboltzmann_amd import FfbError, Lattice
+
+    with pytest.raises(FfbError):
+        Lattice(N_X=8, N_Y=8)
+
+
+def test_python_mirror_settings():
+    from fastfortranboltzmann_amd import Settings, omega_for_reynolds, reynolds
+
+    s = Settings(N_Y=4096, SIM_MODE=4, U_LID=0.1, OMEGA=omega_for_reynolds(1000, 0.1, 4096))
+    assert abs(s.OMEGA - 0.578436) < 1e-6
+    assert abs(reynolds(s) - 1000) < 1e-9
