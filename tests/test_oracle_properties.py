@@ -1,0 +1,185 @@
+"""Pins of the CPU oracle that do not come from the reference's code: closed
+forms and invariants of D2Q9 BGK with halfway bounce-back, written here
+independently of the oracle's formulation (numpy rolls and masks instead of
+halo writes, the textbook equilibrium instead of the unrolled expressions).
+
+Each check is chosen to catch a plausible mistake: a wrong opposite
+direction or shift (pure-streaming brute force), a dropped or mis-signed
+equilibrium term (omega = 1 and invariants), a wrong moving-wall term (Ladd
+correction), a wrong viscosity (shear-wave decay, Couette and Poiseuille
+profiles)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleLattice, Params
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def random_state(nx, ny, seed, dt=np.float64):
+    rng = np.random.default_rng(seed)
+    return (W[:, None, None] * (1 + 0.3 * rng.random((9, ny + 2, nx + 2)))).astype(dt)
+
+
+def stream_brute_force(f, mode, u_top=0.0, rho_0=1.0):
+    """Pure streaming (omega = 0) of the interior f[:, 1:-1, 1:-1] of a halo array.
+    Periodic directions use np.roll; walls reflect the cell's own opposite
+    population (halfway bounce-back), plus the Ladd term 6 w_i rho_0 (c_i . u_w)
+    on the opposite direction for a moving top wall."""
+    g = f[:, 1:-1, 1:-1]
+    ny, nx = g.shape[1:]
+    out = np.empty_like(g)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    for i in range(9):
+        rolled = np.roll(g[i], shift=(CY[i], CX[i]), axis=(0, 1))
+        if mode == 1:
+            out[i] = rolled
+            continue
+        ys, xs = yy - CY[i], xx - CX[i]
+        out_y = (ys < 0) | (ys >= ny)
+        out_x = (xs < 0) | (xs >= nx)
+        wall = out_y | out_x if mode == 4 else out_y
+        bb = g[OPP[i]].copy()
+        top = ys >= ny
+        # moving wall: f_i = f_opp(i) + 6 w rho_0 c_i.u_w  (u_w = (u_top, 0))
+        bb = np.where(top, bb + 6 * W[i] * rho_0 * CX[i] * u_top, bb)
+        out[i] = np.where(wall, bb, rolled)
+    return out
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+@pytest.mark.parametrize("shape", [(7, 5), (5, 7), (1, 4), (4, 1), (2, 2)])
+def test_pure_streaming_matches_brute_force(mode, shape):
+    nx, ny = shape
+    f0 = random_state(nx, ny, 7)
+    o = OracleLattice(mode, nx, ny, Params(omega=0.0, u_wall=0.0, u_lid=0.0)).init()
+    o.f[:] = f0
+    o.run(1)
+    np.testing.assert_array_equal(o.interior, stream_brute_force(f0, mode))
+
+
+@pytest.mark.parametrize("mode", [2, 4])
+def test_moving_wall_is_ladd_correction(mode):
+    """omega = 0: the lid/wall row receives f_opp + 2 w rho_0 c.u_w / c_s^2 (c_s^2 = 1/3)."""
+    nx, ny, u = 9, 6, 0.1
+    f0 = random_state(nx, ny, 3)
+    o = OracleLattice(mode, nx, ny, Params(omega=0.0, u_wall=u, u_lid=u, rho_0=1.0)).init()
+    o.f[:] = f0
+    o.run(1)
+    np.testing.assert_allclose(o.interior, stream_brute_force(f0, mode, u_top=u), rtol=0, atol=1e-16)
+    # sign check on the lid row: f8 (c = (-1,-1)) loses, f9 (c = (1,-1)) gains rho_0 u / 6
+    assert o.interior[7, -1, 4] == pytest.approx(f0[5, ny, 5] - u / 6, abs=1e-16)
+    assert o.interior[8, -1, 4] == pytest.approx(f0[6, ny, 5] + u / 6, abs=1e-16)
+
+
+def equilibrium(rho, ux, uy):
+    """Textbook D2Q9 equilibrium w_i rho (1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2)."""
+    cu = CX[:, None, None] * ux + CY[:, None, None] * uy
+    return W[:, None, None] * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux ** 2 + uy ** 2))
+
+
+def test_omega_one_gives_equilibrium_of_pulled_moments():
+    nx, ny = 11, 8
+    f0 = random_state(nx, ny, 11)
+    pulled = stream_brute_force(f0, 1)
+    rho = pulled.sum(0)
+    ux = (CX[:, None, None] * pulled).sum(0) / rho
+    uy = (CY[:, None, None] * pulled).sum(0) / rho
+    o = OracleLattice(1, nx, ny, Params(omega=1.0)).init()
+    o.f[:] = f0
+    o.run(1)
+    np.testing.assert_allclose(o.interior, equilibrium(rho, ux, uy), rtol=1e-14, atol=0)
+    np.testing.assert_allclose(o.rho, rho, rtol=1e-15)
+    np.testing.assert_allclose(o.u_x, ux, rtol=1e-13, atol=1e-17)
+
+
+@pytest.mark.parametrize("omega", [0.6, 1.3, 1.9])
+def test_collision_conserves_mass_and_momentum_per_cell(omega):
+    nx, ny = 10, 9
+    f0 = random_state(nx, ny, 5)
+    pulled = stream_brute_force(f0, 1)
+    o = OracleLattice(1, nx, ny, Params(omega=omega)).init()
+    o.f[:] = f0
+    o.run(1)
+    g = o.interior
+    np.testing.assert_allclose(g.sum(0), pulled.sum(0), rtol=1e-14)
+    for c in (CX, CY):
+        np.testing.assert_allclose((c[:, None, None] * g).sum(0),
+                                   (c[:, None, None] * pulled).sum(0), atol=1e-15)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+def test_rest_state_is_a_fixed_point(mode):
+    """Uniform rest with resting walls stays at w_i rho_0 (to rounding: sum w_i is
+    not exactly 1 in binary, so w_i * rho differs from w_i in the last bit)."""
+    o = OracleLattice(mode, 12, 9, Params(omega=1.7, u_max=0.0, u_wall=0.0, u_lid=0.0)).init()
+    f0 = o.interior.copy()
+    o.run(40)
+    np.testing.assert_allclose(o.interior, f0, rtol=0, atol=2e-16)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+def test_total_mass_is_conserved(mode):
+    o = OracleLattice(mode, 24, 18, Params(omega=1.45, u_wall=0.08, u_lid=0.08)).init()
+    m0 = o.interior.sum()
+    o.run(300)
+    assert abs(o.interior.sum() - m0) / m0 < 1e-13
+
+
+def test_shear_wave_decays_with_bgk_viscosity():
+    from fastfortranboltzmann_amd.analysis import decay_amplitude, fit_viscosity
+
+    n, omega = 64, 1.0
+    o = OracleLattice(1, n, n, Params(omega=omega, u_max=0.01, n_sin=1.0)).init()
+    ts, amps = [], []
+    for _ in range(10):
+        o.run(100)
+        ts.append(o.steps)
+        amps.append(decay_amplitude(o.u_x, 1.0))
+    nu = (1 / omega - 0.5) / 3
+    assert abs(fit_viscosity(ts, amps, n, 1.0) - nu) / nu < 1e-3
+
+
+def test_couette_steady_profile_is_linear():
+    """Halfway walls at y = 1/2 and N_Y + 1/2: u_x(y) = U (y - 1/2) / N_Y."""
+    nx, ny, u, omega = 4, 12, 0.01, 1.2
+    o = OracleLattice(2, nx, ny, Params(omega=omega, u_wall=u)).init().run(4000)
+    y = np.arange(1, ny + 1)
+    np.testing.assert_allclose(o.u_x.mean(1), u * (y - 0.5) / ny, rtol=0, atol=1e-8)
+    assert np.abs(o.u_y).max() < 1e-12
+
+
+def test_poiseuille_steady_profile_is_parabolic():
+    """Pressure-driven channel: u_x(y) = G / (2 rho nu) (y - 1/2)(N_Y + 1/2 - y) with
+    G = -c_s^2 d rho/dx measured in the bulk.  (The reference applies the
+    pressure-periodic increment twice per step, poiseuille_flow.f90:55-88 from
+    main.f90:216 and simulation.f90:47, so the bulk gradient is not the textbook
+    (rho_in - rho_out)/N_X; the profile is checked against the gradient the run
+    actually has.)"""
+    nx, ny, omega = 16, 10, 1.0
+    rho_in, rho_out = 1.001, 0.999
+    o = OracleLattice(3, nx, ny, Params(omega=omega, rho_in=rho_in, rho_out=rho_out)).init()
+    o.run(3000)
+    y = np.arange(1, ny + 1)
+    shape = (y - 0.5) * (ny + 0.5 - y)
+    prof = o.u_x[:, nx // 2]
+    fit = prof @ shape / (shape @ shape)
+    assert np.abs(prof - fit * shape).max() / prof.max() < 5e-3
+    nu = (1 / omega - 0.5) / 3
+    xs = np.arange(3, nx - 3)
+    slope = np.polyfit(xs, o.rho.mean(0)[xs], 1)[0]
+    assert slope < 0
+    expect = -slope / 3 / (2 * nu * o.rho.mean())
+    assert fit == pytest.approx(expect, rel=0.03)
+    # the gradient exceeds the single-application value (rho_in - rho_out)/(N_X + 1)
+    assert -slope > 1.5 * (rho_in - rho_out) / (nx + 1)
+
+
+def test_fp32_oracle_tracks_fp64():
+    p = Params(omega=1.3)
+    a = OracleLattice(4, 20, 16, p, np.float64).init().run(200)
+    b = OracleLattice(4, 20, 16, p, np.float32).init().run(200)
+    assert np.abs(a.interior - b.interior).max() < 1e-5
