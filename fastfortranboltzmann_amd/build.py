@@ -35,6 +35,50 @@ def hipcc() -> str:
     return "hipcc"
 
 
+FORTRAN = ["ffb_c.f90", "ffb_settings.f90", "ffb_host.f90"]
+HOST_BIN = os.path.join(PKG, "bin", "ffb_host")
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/llvm/bin/flang", "flang", "amdflang"):
+        if not cand:
+            continue
+        if os.path.sep in cand and os.path.exists(cand):
+            return cand
+        if os.path.sep not in cand:
+            from shutil import which
+            if which(cand):
+                return cand
+    return None
+
+
+def build_fortran_host(verbose: bool = False, defines: list[str] | None = None) -> str | None:
+    """Compile the Fortran host (fortran/*.f90) against libffb_hip.so with flang.
+    `defines` are FFB_* preprocessor settings, as in the reference's
+    FFB_SETTINGS_DEFINITIONS.  Returns the binary path, or None without flang."""
+    fc = flang()
+    if fc is None:
+        return None
+    import tempfile
+
+    os.makedirs(os.path.dirname(HOST_BIN), exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        objs = []
+        for src in FORTRAN:
+            obj = os.path.join(tmp, src.replace(".f90", ".o"))
+            cmd = [fc, "-cpp", "-O2", "-module-dir", tmp, "-I", tmp,
+                   *[f"-D{d}" for d in (defines or [])],
+                   "-c", os.path.join(PKG, "fortran", src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True, capture_output=not verbose)
+            objs.append(obj)
+        cmd = [fc, "-o", HOST_BIN + ".tmp", *objs, "-L", PKG, "-lffb_hip", "-Wl,-rpath,$ORIGIN/.."]
+        subprocess.run(cmd, check=True, capture_output=not verbose)
+    os.replace(HOST_BIN + ".tmp", HOST_BIN)
+    return HOST_BIN
+
+
 def build(verbose: bool = False) -> str:
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-I", CSRC,
            "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
@@ -45,8 +89,10 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libffb_hip.so")
     os.replace(LIB + ".tmp", LIB)
+    build_fortran_host(verbose)
     return LIB
 
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(HOST_BIN if os.path.exists(HOST_BIN) else "(no flang: Fortran host not built)")

@@ -51,37 +51,51 @@ static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) ==
 template <typename S> struct Store;
 
 template <> struct Store<double> {
-    using C = double;
-    static __device__ __forceinline__ C ld(const double *p, size_t k, C, C) { return p[k]; }
-    static __device__ __forceinline__ void st(double *p, size_t k, C v, C, C) {
-        __builtin_nontemporal_store(v, p + k);
-    }
+    using C = double;  // compute type
+    using R = double;  // raw stored type
+    static __device__ __forceinline__ C in(R r, C, C) { return r; }
+    static __device__ __forceinline__ R out(C v, C, C) { return v; }
 };
 template <> struct Store<float> {
     using C = float;
-    static __device__ __forceinline__ C ld(const float *p, size_t k, C, C) { return p[k]; }
-    static __device__ __forceinline__ void st(float *p, size_t k, C v, C, C) {
-        __builtin_nontemporal_store(v, p + k);
-    }
+    using R = float;
+    static __device__ __forceinline__ C in(R r, C, C) { return r; }
+    static __device__ __forceinline__ R out(C v, C, C) { return v; }
 };
 template <> struct Store<__half> {
     using C = float;
-    static __device__ __forceinline__ C ld(const __half *p, size_t k, C w, C inv_scale) {
-        return w + __half2float(p[k]) * inv_scale;
+    using R = unsigned short;
+    static __device__ __forceinline__ C in(R r, C w, C inv_scale) {
+        return w + __half2float(__ushort_as_half(r)) * inv_scale;
     }
-    static __device__ __forceinline__ void st(__half *p, size_t k, C v, C w, C scale) {
-        p[k] = __float2half_rn((v - w) * scale);
+    static __device__ __forceinline__ R out(C v, C w, C scale) {
+        return __half_as_ushort(__float2half_rn((v - w) * scale));  // round to nearest even
     }
 };
 template <> struct Store<__hip_bfloat16> {
     using C = float;
-    static __device__ __forceinline__ C ld(const __hip_bfloat16 *p, size_t k, C w, C inv_scale) {
-        return w + __bfloat162float(p[k]) * inv_scale;
+    using R = unsigned short;
+    static __device__ __forceinline__ C in(R r, C w, C inv_scale) {
+        return w + __uint_as_float((unsigned)r << 16) * inv_scale;
     }
-    static __device__ __forceinline__ void st(__hip_bfloat16 *p, size_t k, C v, C w, C scale) {
-        p[k] = __float2bfloat16((v - w) * scale);
+    static __device__ __forceinline__ R out(C v, C w, C scale) {
+        unsigned u = __float_as_uint((v - w) * scale);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (R)((u >> 16) | 0x40u);  // quiet NaN
+        u += 0x7fffu + ((u >> 16) & 1u);                                       // nearest even
+        return (R)(u >> 16);
     }
 };
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C ld_pop(const typename Store<S>::R *p, size_t k,
+                                                      typename Store<S>::C w, typename Store<S>::C inv) {
+    return Store<S>::in(p[k], w, inv);
+}
+template <typename S>
+__device__ __forceinline__ void st_pop(typename Store<S>::R *p, size_t k, typename Store<S>::C v,
+                                       typename Store<S>::C w, typename Store<S>::C scale) {
+    __builtin_nontemporal_store(Store<S>::out(v, w, scale), p + k);
+}
 
 // Parameters of one step over one domain.
 template <typename C> struct StepArgs {
@@ -114,9 +128,9 @@ __device__ __forceinline__ int tile_of_block(int p, int n, int xcd_order) {
 }
 
 template <typename S, typename C>
-__device__ __forceinline__ C LD(const StepArgs<C> &a, const S *f, int d, int x, int y) {
-    return Store<S>::ld(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x,
-                        a.w[d], a.inv_scale);
+__device__ __forceinline__ C LD(const StepArgs<C> &a, const typename Store<S>::R *f, int d, int x, int y) {
+    return ld_pop<S>(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x, a.w[d],
+                     a.inv_scale);
 }
 
 // Pressure-periodic increment of pressure_periodic_distribution
@@ -149,7 +163,8 @@ __device__ __forceinline__ C pf_delta(const StepArgs<C> &a, const C *m, int d, i
 // PHASE 1 gives the halo after the first application only (the state the
 // strip pass reads, poiseuille_flow.f90:140-165), PHASE 2 after both.
 template <typename S, typename C, int MODE, int PHASE>
-__device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const S *f, int d, int x, int y) {
+__device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const typename Store<S>::R *f, int d, int x,
+                                      int y) {
     const int nx = a.nx, ny = a.ny;
     int xs = x - cx_of(d), ys = y - cy_of(d);
     const bool xo = (xs < 0) | (xs >= nx);
@@ -187,10 +202,11 @@ __device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const S *f, int d, 
 }
 
 // BGK collision of fuzed_pull_streaming_collision_local_unrolled_universal
-// (source/layouts/xyd/simulation.f90:224-293), same operation order.
-template <typename S, typename C>
-__device__ __forceinline__ void collide_store(const StepArgs<C> &a, S *g, size_t c, int x, int y,
-                                              C f1, C f2, C f3, C f4, C f5, C f6, C f7, C f8, C f9) {
+// (source/layouts/xyd/simulation.f90:224-293), same operation order.  f[]
+// holds the pulled populations on entry and the post-collision ones on exit.
+template <typename C>
+__device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, int y) {
+    const C f1 = f[0], f2 = f[1], f3 = f[2], f4 = f[3], f5 = f[4], f6 = f[5], f7 = f[6], f8 = f[7], f9 = f[8];
     C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
@@ -206,61 +222,121 @@ __device__ __forceinline__ void collide_store(const StepArgs<C> &a, S *g, size_t
     const C om = a.omega;
     const C one = 1, three = 3, c45 = (C)4.5, c15 = (C)1.5;
     const C w0 = a.w[0], w1 = a.w[1], w2 = a.w[5];
-    const size_t P = (size_t)a.plane;
     C s;
-    Store<S>::st(g, c, f1 + om * (w0 * rho * (one - c15 * usq) - f1), a.w[0], a.scale);
-    Store<S>::st(g, P + c, f2 + om * (w1 * rho * (one + three * ux + c45 * ux * ux - c15 * usq) - f2), a.w[1], a.scale);
-    Store<S>::st(g, 2 * P + c, f3 + om * (w1 * rho * (one + three * uy + c45 * uy * uy - c15 * usq) - f3), a.w[2], a.scale);
-    Store<S>::st(g, 3 * P + c, f4 + om * (w1 * rho * (one - three * ux + c45 * ux * ux - c15 * usq) - f4), a.w[3], a.scale);
-    Store<S>::st(g, 4 * P + c, f5 + om * (w1 * rho * (one - three * uy + c45 * uy * uy - c15 * usq) - f5), a.w[4], a.scale);
+    f[0] = f1 + om * (w0 * rho * (one - c15 * usq) - f1);
+    f[1] = f2 + om * (w1 * rho * (one + three * ux + c45 * ux * ux - c15 * usq) - f2);
+    f[2] = f3 + om * (w1 * rho * (one + three * uy + c45 * uy * uy - c15 * usq) - f3);
+    f[3] = f4 + om * (w1 * rho * (one - three * ux + c45 * ux * ux - c15 * usq) - f4);
+    f[4] = f5 + om * (w1 * rho * (one - three * uy + c45 * uy * uy - c15 * usq) - f5);
     s = ux + uy;
-    Store<S>::st(g, 5 * P + c, f6 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f6), a.w[5], a.scale);
+    f[5] = f6 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f6);
     s = -ux + uy;
-    Store<S>::st(g, 6 * P + c, f7 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f7), a.w[6], a.scale);
+    f[6] = f7 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f7);
     s = ux + uy;
-    Store<S>::st(g, 7 * P + c, f8 + om * (w2 * rho * (one - three * s + c45 * s * s - c15 * usq) - f8), a.w[7], a.scale);
+    f[7] = f8 + om * (w2 * rho * (one - three * s + c45 * s * s - c15 * usq) - f8);
     s = ux - uy;
-    Store<S>::st(g, 8 * P + c, f9 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f9), a.w[8], a.scale);
+    f[8] = f9 + om * (w2 * rho * (one + three * s + c45 * s * s - c15 * usq) - f9);
 }
 
-// One fused pull-stream-collide step over one domain.  One thread per cell;
-// a workgroup is a tile of TW consecutive cells of one row.  Cells on the
-// domain edge take the boundary rule, all others pull plainly.
-template <typename S, typename C, int MODE, int TW>
-__global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
-    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_order);
-    const int ty = t / a.n_tiles_x;
-    const int x = (t - ty * a.n_tiles_x) * TW + (int)threadIdx.x;
-    const int y = ty;
-    if (x >= a.nx || y >= a.ny) return;
-    const S *__restrict__ f = static_cast<const S *>(a.src);
-    S *__restrict__ g = static_cast<S *>(a.dst);
+// x offset and y offset of the pull source of direction d: f_d(x - c_d).
+__host__ __device__ constexpr int pull_dx(int d) { return -cx_of(d); }
+__host__ __device__ constexpr int pull_dy(int d) { return -cy_of(d); }
+
+// One cell, any position: plain pull in the interior, boundary rule on the edge.
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
+                                          typename Store<S>::R *__restrict__ g, int x, int y) {
     const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
+    const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
     const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
-    C f1, f2, f3, f4, f5, f6, f7, f8, f9;
+    C v[9];
     if (!edge) {
-        const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
-        f1 = Store<S>::ld(f, c, a.w[0], a.inv_scale);
-        f2 = Store<S>::ld(f, P + c - 1, a.w[1], a.inv_scale);
-        f3 = Store<S>::ld(f, 2 * P + c - W, a.w[2], a.inv_scale);
-        f4 = Store<S>::ld(f, 3 * P + c + 1, a.w[3], a.inv_scale);
-        f5 = Store<S>::ld(f, 4 * P + c + W, a.w[4], a.inv_scale);
-        f6 = Store<S>::ld(f, 5 * P + c - W - 1, a.w[5], a.inv_scale);
-        f7 = Store<S>::ld(f, 6 * P + c - W + 1, a.w[6], a.inv_scale);
-        f8 = Store<S>::ld(f, 7 * P + c + W + 1, a.w[7], a.inv_scale);
-        f9 = Store<S>::ld(f, 8 * P + c + W - 1, a.w[8], a.inv_scale);
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            v[d] = ld_pop<S>(f, (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W + pull_dx(d), a.w[d],
+                             a.inv_scale);
     } else {
-        f1 = LD<S, C>(a, f, 0, x, y);
-        f2 = pull_edge<S, C, MODE, 2>(a, f, 1, x, y);
-        f3 = pull_edge<S, C, MODE, 2>(a, f, 2, x, y);
-        f4 = pull_edge<S, C, MODE, 2>(a, f, 3, x, y);
-        f5 = pull_edge<S, C, MODE, 2>(a, f, 4, x, y);
-        f6 = pull_edge<S, C, MODE, 2>(a, f, 5, x, y);
-        f7 = pull_edge<S, C, MODE, 2>(a, f, 6, x, y);
-        f8 = pull_edge<S, C, MODE, 2>(a, f, 7, x, y);
-        f9 = pull_edge<S, C, MODE, 2>(a, f, 8, x, y);
+        v[0] = LD<S, C>(a, f, 0, x, y);
+#pragma unroll
+        for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
     }
-    collide_store<S, C>(a, g, c, x, y, f1, f2, f3, f4, f5, f6, f7, f8, f9);
+    collide<C>(a, v, x, y);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st_pop<S>(g, (size_t)d * P + c, v[d], a.w[d], a.scale);
+}
+
+// One fused pull-stream-collide step over one domain.  Each thread updates V
+// consecutive cells of one row (V * element size = 16 bytes); a workgroup is a
+// tile of TW*V cells.  Where all V cells are interior, every plane is read
+// with one aligned 16-byte vector load at the thread's own x0; the x-1 and x+1
+// planes add one scalar load of the element just outside the vector (a cache
+// hit: the neighbouring lane's vector holds it) and shift it in, so every
+// vector load is naturally aligned.  Each plane is written with one aligned nontemporal 16-byte
+// store.  Cells on the domain edge take the boundary rule cell by cell.
+template <typename S, typename C, int MODE, int TW, int V>
+__global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
+    using R = typename Store<S>::R;
+    typedef R RV __attribute__((ext_vector_type(V)));
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_order);
+    const int y = t / a.n_tiles_x;
+    const int x0 = ((t - y * a.n_tiles_x) * TW + (int)threadIdx.x) * V;
+    if (x0 >= a.nx || y >= a.ny) return;
+    const R *__restrict__ f = static_cast<const R *>(a.src);
+    R *__restrict__ g = static_cast<R *>(a.dst);
+    const bool interior = (V > 1) & (y > 0) & (y < a.ny - 1) & (x0 > 0) & (x0 + V < a.nx);
+    if (!interior) {
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (x0 + k < a.nx) step_cell<S, C, MODE>(a, f, g, x0 + k, y);
+        return;
+    }
+    const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x0;
+    const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
+    // neighbour elements first, then a compiler-only memory barrier (emits no
+    // instruction): it keeps these element loads separate from the aligned
+    // vector loads below, which the load combiner would otherwise fuse into
+    // 4-byte-aligned 16-byte loads.
+    R side[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        const R *row = f + (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W;
+        if (pull_dx(d) < 0) side[d] = row[-1];
+        else if (pull_dx(d) > 0) side[d] = row[V];
+    }
+    asm volatile("" ::: "memory");
+    RV in[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        const R *row = f + (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W;  // 16-byte aligned
+        const RV own = *reinterpret_cast<const RV *>(row);
+        if (pull_dx(d) == 0) {
+            in[d] = own;
+        } else if (pull_dx(d) < 0) {  // f(x0-1 .. x0+V-2)
+            RV t;
+            t[0] = side[d];
+#pragma unroll
+            for (int k = 1; k < V; ++k) t[k] = own[k - 1];
+            in[d] = t;
+        } else {                       // f(x0+1 .. x0+V)
+            RV t;
+#pragma unroll
+            for (int k = 0; k < V - 1; ++k) t[k] = own[k + 1];
+            t[V - 1] = side[d];
+            in[d] = t;
+        }
+    }
+    RV out[9];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        C v[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) v[d] = Store<S>::in(in[d][k], a.w[d], a.inv_scale);
+        collide<C>(a, v, x0 + k, y);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d], a.w[d], a.scale);
+    }
+#pragma unroll
+    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(g + (size_t)d * P + c));
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,
@@ -274,7 +350,7 @@ __global__ void __launch_bounds__(256) pf_strip_kernel(StepArgs<C> a) {
     if (k >= 2 * a.ny) return;
     const int side = k / a.ny, y = k - side * a.ny;
     const int x = side == 0 ? a.nx - 1 : 0;
-    const S *f = static_cast<const S *>(a.src);
+    const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(a.src);
     const C f1 = pull_edge<S, C, PF, 1>(a, f, 0, x, y);
     const C f2 = pull_edge<S, C, PF, 1>(a, f, 1, x, y);
     const C f3 = pull_edge<S, C, PF, 1>(a, f, 2, x, y);
@@ -302,7 +378,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepArgs<C> a, C rho_0, const
     const int y = (int)blockIdx.x / tiles_x;
     const int x = ((int)blockIdx.x - y * tiles_x) * 256 + (int)threadIdx.x;
     if (x >= a.nx || y >= a.ny) return;
-    S *g = static_cast<S *>(a.dst);
+    typename Store<S>::R *g = static_cast<typename Store<S>::R *>(a.dst);
     const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
     const C ux = ux_row ? ux_row[y] : (C)0;
     const C uy = 0;
@@ -316,7 +392,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepArgs<C> a, C rho_0, const
         } else {
             feq = a.w[d] * rho_0;
         }
-        Store<S>::st(g, (size_t)d * (size_t)a.plane + c, feq, a.w[d], a.scale);
+        st_pop<S>(g, (size_t)d * (size_t)a.plane + c, feq, a.w[d], a.scale);
     }
     if (a.rho_o) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
@@ -333,7 +409,7 @@ __global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, dou
     __shared__ double sh[3][256];
     double m = 0, mx = 0, my = 0;
     const long long n = (long long)a.nx * a.ny;
-    const S *f = static_cast<const S *>(a.src);
+    const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(a.src);
     for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
         const int y = (int)(k / a.nx), x = (int)(k - (long long)y * a.nx);
         const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
@@ -341,10 +417,10 @@ __global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, dou
         for (int d = 0; d < 9; ++d) {
             double v;
             if (sizeof(S) == 2)
-                v = (double)a.w[d] + (double)Store<S>::ld(f, (size_t)d * (size_t)a.plane + c, 0, 1) *
+                v = (double)a.w[d] + (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, 0, 1) *
                                          (double)a.inv_scale;
             else
-                v = (double)Store<S>::ld(f, (size_t)d * (size_t)a.plane + c, a.w[d], a.inv_scale);
+                v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, a.w[d], a.inv_scale);
             m += v;
             mx += (double)cx_of(d) * v;
             my += (double)cy_of(d) * v;

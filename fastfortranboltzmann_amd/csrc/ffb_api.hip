@@ -38,6 +38,7 @@ struct ffb_lattice {
     long long plane = 0;     // elements per population plane
     size_t elem = 0;         // bytes per stored population
     size_t celem = 0;        // bytes per compute value (macro fields, strips)
+    void *pop_alloc = nullptr;  // one allocation holding both population sets
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;             // buffer the next step reads
     void *macro[3] = {nullptr, nullptr, nullptr};
@@ -51,9 +52,15 @@ struct ffb_lattice {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timing = false;
     long long steps = 0;
-    int tile_w = 256;
+    int tile_w = 512;        // threads per workgroup
+    int vec = 0;             // 1: V = 16 bytes / element cells per thread; 0: one cell per thread
     int xcd_order = 1;
+    long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
+    long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };
+
+// cells per thread of the step kernel
+static int cells_per_thread(const ffb_lattice *L) { return L->vec ? (int)(16 / L->elem) : 1; }
 
 namespace {
 
@@ -87,24 +94,33 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     const int s = (c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV) ? c.storage_scale_log2 : 0;
     a.scale = (C)std::ldexp(1.0, s);
     a.inv_scale = (C)std::ldexp(1.0, -s);
-    a.n_tiles_x = (c.n_x + L->tile_w - 1) / L->tile_w;
+    const int cells_per_tile = L->tile_w * cells_per_thread(L);
+    a.n_tiles_x = (c.n_x + cells_per_tile - 1) / cells_per_tile;
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_order = L->xcd_order;
     return a;
 }
 
-template <typename S, typename C, int MODE>
-int launch_step_mode(ffb_lattice *L, const StepArgs<C> &a) {
+template <typename S, typename C, int MODE, int TW>
+int launch_step_tw(ffb_lattice *L, const StepArgs<C> &a) {
+    constexpr int VEC = 16 / (int)sizeof(typename Store<S>::R);  // cells per thread on the vector path
     const dim3 grid((unsigned)a.n_blocks);
-    switch (L->tile_w) {
-    case 64: hipLaunchKernelGGL((step_kernel<S, C, MODE, 64>), grid, dim3(64), 0, L->stream, a); break;
-    case 128: hipLaunchKernelGGL((step_kernel<S, C, MODE, 128>), grid, dim3(128), 0, L->stream, a); break;
-    case 512: hipLaunchKernelGGL((step_kernel<S, C, MODE, 512>), grid, dim3(512), 0, L->stream, a); break;
-    case 1024: hipLaunchKernelGGL((step_kernel<S, C, MODE, 1024>), grid, dim3(1024), 0, L->stream, a); break;
-    default: hipLaunchKernelGGL((step_kernel<S, C, MODE, 256>), grid, dim3(256), 0, L->stream, a); break;
-    }
+    if (cells_per_thread(L) > 1)
+        hipLaunchKernelGGL((step_kernel<S, C, MODE, TW, VEC>), grid, dim3(TW), 0, L->stream, a);
+    else
+        hipLaunchKernelGGL((step_kernel<S, C, MODE, TW, 1>), grid, dim3(TW), 0, L->stream, a);
     HIPCHK(hipGetLastError());
     return FFB_OK;
+}
+
+template <typename S, typename C, int MODE>
+int launch_step_mode(ffb_lattice *L, const StepArgs<C> &a) {
+    switch (L->tile_w) {
+    case 64: return launch_step_tw<S, C, MODE, 64>(L, a);
+    case 128: return launch_step_tw<S, C, MODE, 128>(L, a);
+    case 512: return launch_step_tw<S, C, MODE, 512>(L, a);
+    default: return launch_step_tw<S, C, MODE, 256>(L, a);
+    }
 }
 
 template <typename S, typename C> int launch_step(ffb_lattice *L, int write_macro) {
@@ -215,13 +231,34 @@ uint16_t float_to_bf16(float f) {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) copy_kernel(const v4f *__restrict__ src, v4f *__restrict__ dst,
-                                                   long long n) {
-    for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256)
-        __builtin_nontemporal_store(src[k], dst + k);
+// Copy probes for the measured HBM ceiling.  VARIANT 0: grid-stride loop;
+// 1: one 16-byte element per thread, plain store; 2: same with a nontemporal
+// store; 3: four 16-byte elements per thread (strided by the block), nontemporal.
+template <int VARIANT>
+__global__ void __launch_bounds__(256) copy_kernel(const v4f *__restrict__ src, v4f *__restrict__ dst, long long n) {
+    if (VARIANT == 0) {
+        for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256)
+            __builtin_nontemporal_store(src[k], dst + k);
+    } else if (VARIANT == 3) {
+        const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+        v4f v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (base + j * 256 < n) v[j] = src[base + j * 256];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (base + j * 256 < n) __builtin_nontemporal_store(v[j], dst + base + j * 256);
+    } else {
+        const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+        if (k < n) {
+            if (VARIANT == 2) __builtin_nontemporal_store(src[k], dst + k);
+            else dst[k] = src[k];
+        }
+    }
 }
 
-double weight(int d) { return d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+// lattice weight in FP32, folded like the device's (C)4 / (C)9 etc.
+float weightf(int d) { return d == 0 ? 4.0f / 9.0f : (d <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f); }
 
 }  // namespace
 
@@ -269,22 +306,34 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     ffb_lattice *L = new ffb_lattice();
     L->cfg = *cfg;
     L->pitch = ((cfg->n_x + 63) / 64) * 64;
-    L->plane = (long long)L->pitch * cfg->n_y;
+    // Tuning parameters of the step kernel.  Defaults measured on MI355X
+    // (4096^2 cavity): one cell per thread, 512-thread tiles, XCD-banded tile
+    // order; the 16-byte vector path (FFB_VEC=1) measured slower.
+    // FFB_TILE_W, FFB_VEC, FFB_XCD_ORDER, FFB_PLANE_PAD, FFB_BUF_PAD override.
+    auto env_ll = [](const char *name, long long dflt) {
+        const char *v = std::getenv(name);
+        return v ? std::atoll(v) : dflt;
+    };
+    L->plane_pad = ((env_ll("FFB_PLANE_PAD", 0) + 63) / 64) * 64;
+    L->buf_pad = ((env_ll("FFB_BUF_PAD", 0) + 63) / 64) * 64;
+    L->plane = (long long)L->pitch * cfg->n_y + L->plane_pad;
+    L->vec = (int)env_ll("FFB_VEC", 0);
     L->elem = cfg->precision == FFB_PREC_FP64 ? 8 : (cfg->precision == FFB_PREC_FP32 ? 4 : 2);
     L->celem = cfg->precision == FFB_PREC_FP64 ? 8 : 4;
-    if (const char *tw = std::getenv("FFB_TILE_W")) L->tile_w = std::atoi(tw);
-    if (const char *xo = std::getenv("FFB_XCD_ORDER")) L->xcd_order = std::atoi(xo);
-    if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512 && L->tile_w != 1024)
-        L->tile_w = 256;
+    L->tile_w = (int)env_ll("FFB_TILE_W", 512);
+    L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 1);
+    if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512) L->tile_w = 256;
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);
         return fail(FFB_E_HIP, std::string("ffb_create: ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    const size_t pop_bytes = (size_t)9 * (size_t)L->plane * L->elem;
-    for (int k = 0; k < 2; ++k)
-        if ((e = hipMalloc(&L->buf[k], pop_bytes)) != hipSuccess) return cleanup_fail(e, "population buffers");
+    const size_t set_elems = (size_t)9 * (size_t)L->plane + (size_t)L->buf_pad;
+    if ((e = hipMalloc(&L->pop_alloc, 2 * set_elems * L->elem)) != hipSuccess)
+        return cleanup_fail(e, "population buffers");
+    L->buf[0] = L->pop_alloc;
+    L->buf[1] = (unsigned char *)L->pop_alloc + set_elems * L->elem;
     const size_t mbytes = (size_t)cfg->n_x * (size_t)cfg->n_y * L->celem;
     for (int k = 0; k < 3; ++k)
         if ((e = hipMalloc(&L->macro[k], mbytes)) != hipSuccess) return cleanup_fail(e, "macro fields");
@@ -311,7 +360,7 @@ int ffb_destroy(ffb_lattice_t *L) {
     // best effort: a failing release must not stop the others
     (void)hipSetDevice(L->cfg.device);
     if (L->stream) (void)hipStreamSynchronize(L->stream);
-    for (void *p : L->buf) if (p) (void)hipFree(p);
+    if (L->pop_alloc) (void)hipFree(L->pop_alloc);
     for (void *p : L->macro) if (p) (void)hipFree(p);
     for (void *p : L->strips) if (p) (void)hipFree(p);
     if (L->ux_row) (void)hipFree(L->ux_row);
@@ -393,11 +442,11 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
                 case FFB_PREC_FP64: ((double *)f_halo)[dst] = ((const double *)stage.data())[src]; break;
                 case FFB_PREC_FP32: ((float *)f_halo)[dst] = ((const float *)stage.data())[src]; break;
                 case FFB_PREC_FP16_DEV:
-                    ((float *)f_halo)[dst] = (float)weight(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
+                    ((float *)f_halo)[dst] = weightf(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
                                                                     (float)std::ldexp(1.0, -s);
                     break;
                 default:
-                    ((float *)f_halo)[dst] = (float)weight(d) + bf16_to_float(((const uint16_t *)stage.data())[src]) *
+                    ((float *)f_halo)[dst] = weightf(d) + bf16_to_float(((const uint16_t *)stage.data())[src]) *
                                                                     (float)std::ldexp(1.0, -s);
                     break;
                 }
@@ -435,11 +484,11 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
                 case FFB_PREC_FP32: ((float *)stage.data())[dst] = ((const float *)f_halo)[src]; break;
                 case FFB_PREC_FP16_DEV:
                     ((uint16_t *)stage.data())[dst] =
-                        float_to_half((((const float *)f_halo)[src] - (float)weight(d)) * (float)std::ldexp(1.0, s));
+                        float_to_half((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
                     break;
                 default:
                     ((uint16_t *)stage.data())[dst] =
-                        float_to_bf16((((const float *)f_halo)[src] - (float)weight(d)) * (float)std::ldexp(1.0, s));
+                        float_to_bf16((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
                     break;
                 }
             }
@@ -530,15 +579,31 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps) {
         if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return e;
         if ((e = hipEventCreate(&e0)) != hipSuccess) return e;
         if ((e = hipEventCreate(&e1)) != hipSuccess) return e;
-        const unsigned grid = 256u * 32u;
-        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n);
-        if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
-        for (int k = 0; k < iters; ++k)
-            hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
-        if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
-        return hipEventElapsedTime(&ms, e0, e1);
+        float best = 0.f;
+        for (int variant = 0; variant < 4; ++variant) {
+            const unsigned grid = variant == 0 ? 256u * 32u
+                                  : variant == 3 ? (unsigned)((n + 1023) / 1024)
+                                                 : (unsigned)((n + 255) / 256);
+            auto launch = [&]() {
+                switch (variant) {
+                case 0: hipLaunchKernelGGL(copy_kernel<0>, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n); break;
+                case 1: hipLaunchKernelGGL(copy_kernel<1>, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n); break;
+                case 2: hipLaunchKernelGGL(copy_kernel<2>, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n); break;
+                default: hipLaunchKernelGGL(copy_kernel<3>, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n); break;
+                }
+            };
+            launch();
+            if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
+            for (int k = 0; k < iters; ++k) launch();
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
+            if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+            float t = 0.f;
+            if ((e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) return e;
+            if (best == 0.f || t < best) best = t;
+        }
+        ms = best;
+        return hipSuccess;
     };
     hipError_t e = run();
     if (e != hipSuccess) rc = fail(FFB_E_HIP, std::string("ffb_bench_copy: ") + hipGetErrorString(e));

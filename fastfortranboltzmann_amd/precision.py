@@ -1,0 +1,131 @@
+"""Reduced-storage (16-bit) form of the solver: error against FP64 and speed.
+
+The populations are stored as d_i = (f_i - w_i) * 2^s in FP16 or BF16, the
+collision runs in FP32, and the conservation sums (mass, momentum) are taken in
+FP64 (``Lattice.moments``).  This form has no counterpart in the reference;
+its results are stated against an FP64 run of the same initial state:
+
+  * relative L2 error of the velocity field, ||u - u64|| / ||u64||
+  * maximum absolute velocity error
+  * total-mass drift (m(t) - m(0)) / m(0), FP64 sums
+
+    python -m fastfortranboltzmann_amd.precision --n 8192 --re 5000 --steps 50000
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from .lattice import Lattice, Settings, omega_for_reynolds
+
+FP16_MAX = 65504.0
+
+
+def default_scale_log2(precision: str, u_ref: float = 0.1, rho_dev: float = 0.05) -> int:
+    """Scale 2^s for the deviation store: FP16 puts the largest expected
+    deviation near 2^14 (a factor 4 below the FP16 maximum), so the small
+    deviations of the resting bulk stay in the normal range; BF16 has FP32's
+    exponent range and needs no scale (s = 0).
+
+    The bound is |f_i - w_i| <= w_0 (rho_dev + 3 u + 6 u^2) with w_0 = 4/9 the
+    largest weight, u the largest expected speed, rho_dev the largest density
+    deviation."""
+    if precision == "bf16dev":
+        return 0
+    if precision != "fp16dev":
+        raise ValueError(f"not a 16-bit form: {precision}")
+    d_max = (4.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref)
+    return int(math.floor(math.log2(2.0 ** 14 / d_max)))
+
+
+def cavity_settings(n: int, re: float, u_lid: float, precision: str, scale_log2: int = 0) -> Settings:
+    return Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=omega_for_reynolds(re, u_lid, n), U_LID=u_lid,
+                    RHO_0=1.0, precision=precision, storage_scale_log2=scale_log2)
+
+
+def run_form(settings: Settings, steps: int, chunk: int = 5000):
+    """Runs one form; returns (u_x, u_y) as float64 host arrays, timing and the
+    mass history (FP64 sums)."""
+    with Lattice(settings) as lat:
+        lat.init()
+        m0 = lat.moments()[0]
+        masses = [m0]
+        done, kernel_ms = 0, 0.0
+        t0 = time.perf_counter()
+        while done < steps:
+            k = min(chunk, steps - done)
+            lat.reset_timer()
+            lat.step(k, write_macro_last=(done + k == steps))
+            lat.sync()
+            kernel_ms += lat.elapsed_ms()
+            done += k
+            masses.append(lat.moments()[0])
+        wall = time.perf_counter() - t0
+        _, ux, uy = lat.macros()
+    cells = settings.N_X * settings.N_Y
+    return {
+        "u_x": ux.astype(np.float64), "u_y": uy.astype(np.float64),
+        "mlups": cells * steps / (kernel_ms * 1e-3) / 1e6,
+        "wall_s": wall,
+        "mass_drift": (masses[-1] - m0) / m0,
+        "mass_drift_max": max(abs(m - m0) for m in masses) / m0,
+    }
+
+
+def compare(ref: dict, run: dict) -> dict:
+    du = np.sqrt((run["u_x"] - ref["u_x"]) ** 2 + (run["u_y"] - ref["u_y"]) ** 2)
+    norm = math.sqrt(float(np.sum(ref["u_x"] ** 2 + ref["u_y"] ** 2)))
+    return {"rel_l2_u": float(math.sqrt(float(np.sum(du ** 2))) / norm),
+            "max_abs_u": float(du.max()),
+            "u_max_ref": float(np.sqrt(ref["u_x"] ** 2 + ref["u_y"] ** 2).max())}
+
+
+def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
+                    forms=("fp16dev", "bf16dev", "fp32"), scales: dict | None = None,
+                    log=print) -> dict:
+    out = {"n": n, "re": re, "u_lid": u_lid, "steps": steps,
+           "omega": omega_for_reynolds(re, u_lid, n), "forms": {}}
+    log(f"precision: fp64 reference {n}^2, {steps} steps")
+    ref = run_form(cavity_settings(n, re, u_lid, "fp64"), steps)
+    out["forms"]["fp64"] = {"mlups": ref["mlups"], "mass_drift": ref["mass_drift"],
+                            "mass_drift_max": ref["mass_drift_max"]}
+    for form in forms:
+        s = (scales or {}).get(form)
+        if s is None:
+            s = default_scale_log2(form, u_lid) if form.endswith("dev") else 0
+        log(f"precision: {form} (s = {s})")
+        r = run_form(cavity_settings(n, re, u_lid, form, s), steps)
+        rec = {"scale_log2": s, "mlups": r["mlups"], "mass_drift": r["mass_drift"],
+               "mass_drift_max": r["mass_drift_max"], **compare(ref, r)}
+        out["forms"][form] = rec
+        log(f"precision: {form}: " + json.dumps({k: v for k, v in rec.items()}))
+    if "fp32" in out["forms"]:
+        for form in forms:
+            if form.endswith("dev"):
+                out["forms"][form]["speedup_vs_fp32"] = out["forms"][form]["mlups"] / out["forms"]["fp32"]["mlups"]
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="16-bit storage error and speed against FP64")
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--re", type=float, default=5000.0)
+    ap.add_argument("--u-lid", type=float, default=0.1)
+    ap.add_argument("--steps", type=int, default=50000)
+    ap.add_argument("--forms", default="fp16dev,bf16dev,fp32")
+    ap.add_argument("--scale", action="append", default=[], help="form=s, e.g. fp16dev=16")
+    args = ap.parse_args(argv)
+    scales = {k: int(v) for k, v in (x.split("=") for x in args.scale)}
+    res = precision_study(args.n, args.re, args.steps, args.u_lid, tuple(args.forms.split(",")),
+                          scales, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -141,9 +141,10 @@ int ffb_moments(ffb_lattice_t *lat, double *mass, double *mom_x, double *mom_y);
 int ffb_device_info(ffb_lattice_t *lat, void **cur, void **stream, int64_t *pitch, int64_t *plane);
 
 /* Measurement helper with no reference counterpart: device-to-device copy of
- * `bytes` bytes by a plain HIP copy kernel (16-byte loads and stores),
- * repeated `iters` times on `device`; *gbps = (read + written bytes) / time.
- * Gives the measured HBM ceiling the step kernel is compared with. */
+ * `bytes` bytes by plain HIP copy kernels (16-byte loads and stores; four
+ * launch shapes, the fastest counts), repeated `iters` times on `device`;
+ * *gbps = (read + written bytes) / time.  Gives the measured HBM ceiling the
+ * step kernel is compared with. */
 int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -22,7 +22,7 @@
 
 /* D2Q9 weights, folded exactly like the reference's 4.0_FP/9.0_FP etc.
  * (source/settings.f90:114-123). */
-static REAL NAME(w)(int i)
+REAL NAME(w)(int i)
 {
     switch (i) {
     case 1: return (REAL)4.0 / (REAL)9.0;

@@ -53,6 +53,20 @@ def lib() -> ctypes.CDLL:
             f = getattr(_lib, f"ffbo_{p}_pf_seed")
             f.argtypes = [i, real, real, ptr]
             f.restype = None
+        f = _lib.ffbo_s_run_quantized
+        f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_int] * 4 + [ctypes.c_float] * 6 + \
+            [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        f.restype = None
+        _lib.ffbo_s_quantize.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p]
+        _lib.ffbo_s_quantize.restype = None
+        _lib.ffbo_float_to_half.argtypes = [ctypes.c_float]
+        _lib.ffbo_float_to_half.restype = ctypes.c_ushort
+        _lib.ffbo_half_to_float.argtypes = [ctypes.c_ushort]
+        _lib.ffbo_half_to_float.restype = ctypes.c_float
+        _lib.ffbo_float_to_bf16.argtypes = [ctypes.c_float]
+        _lib.ffbo_float_to_bf16.restype = ctypes.c_ushort
+        _lib.ffbo_bf16_to_float.argtypes = [ctypes.c_ushort]
+        _lib.ffbo_bf16_to_float.restype = ctypes.c_float
         _lib.ffbo_set_threads.argtypes = [ctypes.c_int]
         _lib.ffbo_max_threads.restype = ctypes.c_int
     return _lib
@@ -77,13 +91,25 @@ def _p(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
+STORAGE_KINDS = {"fp16dev": 1, "bf16dev": 2}
+
+
 class OracleLattice:
-    """One single-domain run of the reference loop on the CPU."""
+    """One single-domain run of the reference loop on the CPU.
+
+    ``storage`` = "fp16dev" / "bf16dev" emulates the GPU's 16-bit scaled
+    deviation storage (FP32 arithmetic, populations rounded through
+    (f - w) * 2^scale_log2 after every step); it has no reference counterpart."""
 
     def __init__(self, mode: int, nx: int, ny: int, params: Params | None = None,
-                 dtype=np.float64):
+                 dtype=np.float64, storage: str | None = None, scale_log2: int = 0):
         self.mode, self.nx, self.ny = int(mode), int(nx), int(ny)
         self.params = params or Params()
+        if storage is not None:
+            if storage not in STORAGE_KINDS:
+                raise ValueError(f"unknown storage {storage!r}")
+            dtype = np.float32
+        self.storage, self.scale_log2 = storage, int(scale_log2)
         self.dtype = np.dtype(dtype)
         self.p = "d" if self.dtype == np.float64 else "s"
         self.f = np.zeros((9, ny + 2, nx + 2), self.dtype)
@@ -100,11 +126,22 @@ class OracleLattice:
             self.mode, self.nx, self.ny, 1, self.ny, P.rho_0, P.u_max, P.n_sin,
             _p(self.f), _p(self.rho), _p(self.u_x), _p(self.u_y))
         getattr(lib(), f"ffbo_{self.p}_pf_seed")(self.ny, P.rho_in, P.rho_out, _p(self.pf_state))
+        if self.storage is not None:
+            lib().ffbo_s_quantize(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
+                                  _p(self.f))
         self.steps = 0
         return self
 
     def run(self, n_steps: int, write_macro_last: bool = True) -> "OracleLattice":
         P = self.params
+        if self.storage is not None:
+            lib().ffbo_s_run_quantized(
+                STORAGE_KINDS[self.storage], self.scale_log2, self.mode, self.nx, self.ny,
+                int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
+                _p(self.f), _p(self.f_b), _p(self.pf_state), 1 if write_macro_last else 0,
+                _p(self.rho), _p(self.u_x), _p(self.u_y))
+            self.steps += int(n_steps)
+            return self
         getattr(lib(), f"ffbo_{self.p}_run")(
             self.mode, self.nx, self.ny, int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid,
             P.rho_in, P.rho_out, _p(self.f), _p(self.f_b), _p(self.pf_state),
@@ -124,6 +161,22 @@ class OracleLattice:
     @property
     def recv_macro_right(self) -> np.ndarray:
         return self.pf_state[1]
+
+
+def half_bits(x: float) -> int:
+    return int(lib().ffbo_float_to_half(float(x)))
+
+
+def half_value(bits: int) -> float:
+    return float(lib().ffbo_half_to_float(int(bits)))
+
+
+def bf16_bits(x: float) -> int:
+    return int(lib().ffbo_float_to_bf16(float(x)))
+
+
+def bf16_value(bits: int) -> float:
+    return float(lib().ffbo_bf16_to_float(int(bits)))
 
 
 def set_threads(n: int) -> None:

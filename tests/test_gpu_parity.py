@@ -54,14 +54,18 @@ def test_gpu_matches_reference_golden_bitwise(case):
             np.testing.assert_array_equal(right, g["recv_macro"][1].reshape(3, -1))
 
 
-SHAPES = [(1, 1), (1, 5), (5, 1), (2, 2), (2, 7), (7, 2), (3, 3), (64, 3), (65, 17), (300, 7),
-          (257, 130)]
+# includes row pitches that do not divide 4 KiB (300, 1000, 1030), so rows and
+# planes start at every alignment within a page
+SHAPES = [(1, 1), (1, 5), (5, 1), (2, 2), (2, 7), (7, 2), (3, 3), (8, 3), (9, 4), (64, 3),
+          (65, 17), (300, 7), (257, 130), (1000, 9), (1030, 5)]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
-def test_gpu_matches_oracle_small_shapes(shape, mode, precision):
+@pytest.mark.parametrize("vec", [1, 0], ids=["vec", "scalar"])
+def test_gpu_matches_oracle_small_shapes(shape, mode, precision, vec, monkeypatch):
+    monkeypatch.setenv("FFB_VEC", str(vec))
     nx, ny = shape
     print(f"[gpu] oracle shapes {nx}x{ny} mode {mode} {precision}", flush=True)
     p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002,

@@ -1,0 +1,55 @@
+"""The 16-bit scaled-deviation storage form on the GPU.
+
+  * bitwise against the oracle's emulation of the same storage (FP32
+    arithmetic, every population rounded through (f - w) * 2^s to FP16/BF16
+    after each step; the conversions are pinned to numpy's float16 in
+    tests/test_oracle_storage16.py), every mode, odd shapes;
+  * error of the velocity field against an FP64 run of the same cavity,
+    with the bound written in the test.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleLattice, Params
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(13, 9), (65, 17), (300, 7), (1030, 5)]
+
+
+@pytest.mark.parametrize("storage,scale", [("fp16dev", 16), ("fp16dev", 11), ("bf16dev", 0)])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
+def test_16bit_storage_matches_oracle_emulation(shape, mode, storage, scale):
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    nx, ny = shape
+    print(f"[gpu] storage {storage} s={scale} {nx}x{ny} mode {mode}", flush=True)
+    p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002, rho_out=0.998)
+    steps = 17
+    o = OracleLattice(mode, nx, ny, p, storage=storage, scale_log2=scale).init().run(steps)
+    s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=p.omega, U_MAX=p.u_max, N_SIN=p.n_sin,
+                 U_WALL=p.u_wall, U_LID=p.u_lid, RHO_IN=p.rho_in, RHO_OUT=p.rho_out,
+                 precision=storage, storage_scale_log2=scale)
+    with Lattice(s) as lat:
+        lat.init().step(steps, write_macro_last=True)
+        np.testing.assert_array_equal(lat.interior(), o.interior)
+        rho, ux, uy = lat.macros()
+        np.testing.assert_array_equal(rho, o.rho)
+        np.testing.assert_array_equal(ux, o.u_x)
+
+
+def test_16bit_storage_error_against_fp64_cavity():
+    """Cavity 512^2, Re 1000, 6000 steps: stated error bounds of the 16-bit
+    forms against FP64 (relative L2 of |u|), and FP64-summed mass drift."""
+    from fastfortranboltzmann_amd.precision import precision_study
+
+    print("[gpu] precision study 512^2", flush=True)
+    res = precision_study(512, 1000.0, 6000, forms=("fp16dev", "bf16dev", "fp32"), log=lambda m: None)
+    f = res["forms"]
+    print({k: {kk: v[kk] for kk in ("rel_l2_u", "mass_drift") if kk in v} for k, v in f.items()}, flush=True)
+    assert f["fp32"]["rel_l2_u"] < 1e-5
+    assert f["fp16dev"]["rel_l2_u"] < 2e-3
+    assert f["bf16dev"]["rel_l2_u"] < 5e-2
+    assert abs(f["fp64"]["mass_drift"]) < 1e-12
+    assert abs(f["fp16dev"]["mass_drift"]) < 1e-3
