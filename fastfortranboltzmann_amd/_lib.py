@@ -11,7 +11,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libffb_hip.so")
 
-SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID = 1, 2, 3, 4
+SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID, BODY_FORCE_CHANNEL = 1, 2, 3, 4, 5
 PREC_FP32, PREC_FP64, PREC_FP16_DEV, PREC_BF16_DEV = 0, 1, 2, 3
 
 # every entry point include/ffb.h declares
@@ -34,6 +34,7 @@ class FfbConfig(ctypes.Structure):
         ("u_wall", ctypes.c_double), ("u_lid", ctypes.c_double),
         ("rho_in", ctypes.c_double), ("rho_out", ctypes.c_double),
         ("storage_scale_log2", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("body_force", ctypes.c_double),
     ]
 
 

@@ -54,7 +54,7 @@ struct ffb_lattice {
     long long steps = 0;
     int tile_w = 512;        // threads per workgroup
     int vec = 0;             // 1: V = 16 bytes / element cells per thread; 0: one cell per thread
-    int xcd_order = 1;
+    int xcd_order = 0;       // 1: XCD-banded tile order (measured 10 % slower, kept as an option)
     long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
     long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };
@@ -96,6 +96,7 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     a.inv_scale = (C)std::ldexp(1.0, -s);
     const int cells_per_tile = L->tile_w * cells_per_thread(L);
     a.n_tiles_x = (c.n_x + cells_per_tile - 1) / cells_per_tile;
+    a.force = (C)c.body_force;
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_order = L->xcd_order;
     return a;
@@ -128,6 +129,7 @@ template <typename S, typename C> int launch_step(ffb_lattice *L, int write_macr
     switch (L->cfg.sim_mode) {
     case FFB_SHEAR_WAVE: return launch_step_mode<S, C, SW>(L, a);
     case FFB_COUETTE_FLOW: return launch_step_mode<S, C, CF>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_step_mode<S, C, BF>(L, a);
     case FFB_POISEUILLE_FLOW: {
         const unsigned nb = (unsigned)((2 * a.ny + 255) / 256);
         hipLaunchKernelGGL((pf_strip_kernel<S, C>), dim3(nb), dim3(256), 0, L->stream, a);
@@ -282,6 +284,7 @@ void ffb_config_default(ffb_config_t *cfg) {
     cfg->rho_out = 0.99995;
     cfg->storage_scale_log2 = 0;
     cfg->device = 0;
+    cfg->body_force = 0.0;
 }
 
 int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
@@ -290,7 +293,7 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     if (cfg->n_x < 1 || cfg->n_y < 1) return fail(FFB_E_INVALID, "ffb_create: n_x and n_y must be >= 1");
     if ((long long)cfg->n_x * cfg->n_y > (1LL << 34))
         return fail(FFB_E_INVALID, "ffb_create: domain larger than 2^34 cells");
-    if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_SLIDING_LID)
+    if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_BODY_FORCE_CHANNEL)
         return fail(FFB_E_INVALID, "ffb_create: unknown sim mode " + std::to_string(cfg->sim_mode));
     if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_BF16_DEV)
         return fail(FFB_E_INVALID, "ffb_create: unknown precision " + std::to_string(cfg->precision));
@@ -307,8 +310,10 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->cfg = *cfg;
     L->pitch = ((cfg->n_x + 63) / 64) * 64;
     // Tuning parameters of the step kernel.  Defaults measured on MI355X
-    // (4096^2 cavity): one cell per thread, 512-thread tiles, XCD-banded tile
-    // order; the 16-byte vector path (FFB_VEC=1) measured slower.
+    // (4096^2 cavity): one cell per thread, 512-thread tiles, dispatch-order
+    // tiles (round-robin over the XCDs keeps all of them on neighbouring rows:
+    // 202 us per step against 222 us with XCD-banded tiles); the 16-byte vector
+    // path (FFB_VEC=1) measured slower.
     // FFB_TILE_W, FFB_VEC, FFB_XCD_ORDER, FFB_PLANE_PAD, FFB_BUF_PAD override.
     auto env_ll = [](const char *name, long long dflt) {
         const char *v = std::getenv(name);
@@ -321,7 +326,7 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->elem = cfg->precision == FFB_PREC_FP64 ? 8 : (cfg->precision == FFB_PREC_FP32 ? 4 : 2);
     L->celem = cfg->precision == FFB_PREC_FP64 ? 8 : 4;
     L->tile_w = (int)env_ll("FFB_TILE_W", 512);
-    L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 1);
+    L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 0);
     if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512) L->tile_w = 256;
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {

@@ -36,6 +36,7 @@ module ffb_c
         real(c_double) :: rho_out
         integer(c_int32_t) :: storage_scale_log2
         integer(c_int32_t) :: device
+        real(c_double) :: body_force
     end type ffb_config_t
 
     interface

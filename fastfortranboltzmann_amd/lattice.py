@@ -25,13 +25,15 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (COUETTE_FLOW, POISEUILLE_FLOW, PREC_BF16_DEV, PREC_FP16_DEV, PREC_FP32,
-                   PREC_FP64, SHEAR_WAVE, SLIDING_LID, FfbConfig, check, lib)
+from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, POISEUILLE_FLOW, PREC_BF16_DEV,
+                   PREC_FP16_DEV, PREC_FP32, PREC_FP64, SHEAR_WAVE, SLIDING_LID, FfbConfig, check,
+                   lib)
 
 PRECISIONS = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp16dev": PREC_FP16_DEV,
               "bf16dev": PREC_BF16_DEV}
 MODES = {"shear_wave": SHEAR_WAVE, "couette_flow": COUETTE_FLOW,
-         "poiseuille_flow": POISEUILLE_FLOW, "sliding_lid": SLIDING_LID}
+         "poiseuille_flow": POISEUILLE_FLOW, "sliding_lid": SLIDING_LID,
+         "body_force_channel": BODY_FORCE_CHANNEL}
 
 
 @dataclass
@@ -54,6 +56,7 @@ class Settings:
     precision: str = "fp32"          # FFB_FP64 off -> "fp32", on -> "fp64"
     storage_scale_log2: int = 0      # s of the 16-bit deviation forms
     device: int = 0
+    BODY_FORCE: float = 0.0          # body-force channel (mode 5) only
     extra: dict = field(default_factory=dict)
 
     def config(self) -> FfbConfig:
@@ -65,7 +68,7 @@ class Settings:
             rho_0=float(self.RHO_0), u_max=float(self.U_MAX), n_sin=float(self.N_SIN),
             u_wall=float(self.U_WALL), u_lid=float(self.U_LID), rho_in=float(self.RHO_IN),
             rho_out=float(self.RHO_OUT), storage_scale_log2=int(self.storage_scale_log2),
-            device=int(self.device))
+            device=int(self.device), body_force=float(self.BODY_FORCE))
 
 
 def reynolds(settings: Settings) -> float:

@@ -45,6 +45,10 @@ extern "C" {
 #define FFB_COUETTE_FLOW 2
 #define FFB_POISEUILLE_FLOW 3
 #define FFB_SLIDING_LID 4
+/* body-force channel: periodic in x, halfway bounce-back walls in y, uniform
+ * body force along x (Guo forcing).  No reference counterpart: the batched
+ * and differentiation forms use it for their Poiseuille lattices. */
+#define FFB_BODY_FORCE_CHANNEL 5
 
 /* storage precision of the populations.  FP64 / FP32 are the reference's
  * FFB_FP64 on / off builds (source/settings.f90:7-13).  The 16-bit forms
@@ -78,6 +82,7 @@ typedef struct ffb_config {
     double rho_out;       /* RHO_OUT (settings.f90:43) */
     int32_t storage_scale_log2; /* s of the 16-bit deviation forms; ignored otherwise */
     int32_t device;       /* HIP device ordinal */
+    double body_force;    /* FFB_BODY_FORCE_CHANNEL: force density along x; ignored otherwise */
 } ffb_config_t;
 
 typedef struct ffb_lattice ffb_lattice_t;

@@ -138,13 +138,13 @@ void ffbo_s_quantize(int kind, int scale_log2, int nx, int ny, float *f)
 void ffbo_s_run_quantized(int kind, int scale_log2, int mode, int nx, int ny, int n_steps,
                           float omega, float rho_0, float u_wall, float u_lid, float rho_in,
                           float rho_out, float *f_a, float *f_b, float *pf_state,
-                          int write_macro_last, float *rho, float *u_x, float *u_y)
+                          int write_macro_last, float *rho, float *u_x, float *u_y, float force)
 {
     float *cur = f_a, *nxt = f_b;
     for (int step = 1; step <= n_steps; ++step) {
         int wm = write_macro_last && step == n_steps;
         ffbo_s_step(mode, nx, ny, omega, rho_0, u_wall, u_lid, rho_in, rho_out, cur, nxt,
-                    pf_state, wm, rho, u_x, u_y);
+                    pf_state, wm, rho, u_x, u_y, force);
         ffbo_s_quantize(kind, scale_log2, nx, ny, nxt);
         float *t = cur;
         cur = nxt;

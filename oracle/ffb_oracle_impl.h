@@ -92,7 +92,7 @@ static void NAME(exchange_1domain)(int mode, int nx, int ny, REAL *f)
 {
     const long pitch = nx + 2;
     const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
-    int plan_x = (mode == 1 || mode == 2 || mode == 3); /* SW, CF, PF: left/right */
+    int plan_x = (mode == 1 || mode == 2 || mode == 3 || mode == 5); /* SW, CF, PF, BF: left/right */
     int plan_y = (mode == 1);                            /* SW only: bottom/top */
     if (plan_x) {
         for (int y = 1; y <= ny; ++y) {
@@ -328,6 +328,57 @@ static void NAME(kernel)(int nx, int ny, int write_macro, REAL omega,
     }
 }
 
+/* Body-force channel collision with Guo forcing, F = (force, 0) (no
+ * reference counterpart; Guo, Zheng & Shi, Phys. Rev. E 65, 046308 (2002)):
+ * u = (sum c f + F/2)/rho, f' = f + omega (f_eq(rho, u) - f) + S,
+ * S_i = (1 - omega/2) w_i [3 (c_i - u) + 9 (c_i . u) c_i] . F. */
+static void NAME(kernel_guo)(int nx, int ny, int write_macro, REAL omega, REAL force,
+                             const REAL *f, REAL *f_next, REAL *rho_o, REAL *ux_o, REAL *uy_o)
+{
+    const long pitch = nx + 2;
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+    static const int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static const int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    const REAL one = 1, three = 3, c45 = (REAL)4.5, c15 = (REAL)1.5, nine = 9, half = (REAL)0.5;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int y = 1; y <= ny; ++y) {
+        for (int x = 1; x <= nx; ++x) {
+            REAL v[9];
+            v[0] = FI(f, x, y, 1);
+            v[1] = FI(f, x - 1, y, 2);
+            v[2] = FI(f, x, y - 1, 3);
+            v[3] = FI(f, x + 1, y, 4);
+            v[4] = FI(f, x, y + 1, 5);
+            v[5] = FI(f, x - 1, y - 1, 6);
+            v[6] = FI(f, x + 1, y - 1, 7);
+            v[7] = FI(f, x + 1, y + 1, 8);
+            v[8] = FI(f, x - 1, y + 1, 9);
+            REAL rho = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7] + v[8];
+            REAL ux = v[1] - v[3] + v[5] - v[6] - v[7] + v[8];
+            REAL uy = v[2] - v[4] + v[5] + v[6] - v[7] - v[8];
+            ux = (ux + half * force) / rho;
+            uy = uy / rho;
+            REAL usq = ux * ux + uy * uy;
+            if (write_macro) {
+                size_t c = (size_t)(y - 1) * (size_t)nx + (size_t)(x - 1);
+                rho_o[c] = rho;
+                ux_o[c] = ux;
+                uy_o[c] = uy;
+            }
+            REAL pre = (one - half * omega) * force;
+            for (int d = 0; d < 9; ++d) {
+                REAL ccx = (REAL)cx[d], ccy = (REAL)cy[d];
+                REAL cu = ccx * ux + ccy * uy;
+                REAL feq = NAME(w)(d + 1) * rho * (((one + three * cu) + c45 * cu * cu) - c15 * usq);
+                REAL src = NAME(w)(d + 1) * pre * (three * (ccx - ux) + nine * cu * ccx);
+                FI(f_next, x, y, d + 1) = v[d] + omega * (feq - v[d]) + src;
+            }
+        }
+    }
+}
+
 /* One time step of the single-domain driver loop, source/layouts/xyd/main.f90:194-262:
  *   exchange_halos (exchange.f90:161-378, one domain),
  *   [PF] prepare_PF(recv_macro) + update strips + macro exchange (main.f90:212-246,
@@ -337,7 +388,7 @@ static void NAME(kernel)(int nx, int ny, int write_macro, REAL omega,
  * window(MACRO_LEFT), window(MACRO_RIGHT)]. */
 void NAME(step)(int mode, int nx, int ny, REAL omega, REAL rho_0, REAL u_wall, REAL u_lid,
                 REAL rho_in, REAL rho_out, REAL *f, REAL *f_next, REAL *pf_state,
-                int write_macro, REAL *rho, REAL *u_x, REAL *u_y)
+                int write_macro, REAL *rho, REAL *u_x, REAL *u_y, REAL force)
 {
     NAME(exchange_1domain)(mode, nx, ny, f);
     if (mode == 3) {
@@ -354,9 +405,16 @@ void NAME(step)(int mode, int nx, int ny, REAL omega, REAL rho_0, REAL u_wall, R
     case 1: NAME(halos_sw)(nx, ny, f); break;
     case 2: NAME(halos_cf)(nx, ny, rho_0, u_wall, f); break;
     case 3: NAME(halos_pf)(nx, ny, rho_in, rho_out, pf_state, pf_state + (size_t)ny * 3, f); break;
+    case 5: /* body-force channel: periodic x (exchange above), resting walls in y */
+        NAME(bb_bottom)(nx, ny, f);
+        NAME(bb_top)(nx, ny, 0, 0, 0, f);
+        break;
     default: NAME(halos_sl)(nx, ny, rho_0, u_lid, f); break;
     }
-    NAME(kernel)(nx, ny, write_macro, omega, f, f_next, rho, u_x, u_y);
+    if (mode == 5)
+        NAME(kernel_guo)(nx, ny, write_macro, omega, force, f, f_next, rho, u_x, u_y);
+    else
+        NAME(kernel)(nx, ny, write_macro, omega, f, f_next, rho, u_x, u_y);
 }
 
 /* PF macro-strip seeds, main.f90:139-155. */
@@ -379,13 +437,13 @@ void NAME(pf_seed)(int ny, REAL rho_in, REAL rho_out, REAL *pf_state)
  * are those of the last step when write_macro_last != 0. */
 void NAME(run)(int mode, int nx, int ny, int n_steps, REAL omega, REAL rho_0, REAL u_wall,
                REAL u_lid, REAL rho_in, REAL rho_out, REAL *f_a, REAL *f_b, REAL *pf_state,
-               int write_macro_last, REAL *rho, REAL *u_x, REAL *u_y)
+               int write_macro_last, REAL *rho, REAL *u_x, REAL *u_y, REAL force)
 {
     REAL *cur = f_a, *nxt = f_b;
     for (int step = 1; step <= n_steps; ++step) {
         int wm = write_macro_last && step == n_steps;
         NAME(step)(mode, nx, ny, omega, rho_0, u_wall, u_lid, rho_in, rho_out, cur, nxt,
-                   pf_state, wm, rho, u_x, u_y);
+                   pf_state, wm, rho, u_x, u_y, force);
         REAL *t = cur;
         cur = nxt;
         nxt = t;

@@ -21,7 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libffb_oracle.so")
 
-SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID = 1, 2, 3, 4
+SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID, BODY_FORCE_CHANNEL = 1, 2, 3, 4, 5
 
 _lib = None
 
@@ -48,14 +48,14 @@ def lib() -> ctypes.CDLL:
             f.restype = None
             f = getattr(_lib, f"ffbo_{p}_run")
             f.argtypes = [i, i, i, i, real, real, real, real, real, real,
-                          ptr, ptr, ptr, i, ptr, ptr, ptr]
+                          ptr, ptr, ptr, i, ptr, ptr, ptr, real]
             f.restype = None
             f = getattr(_lib, f"ffbo_{p}_pf_seed")
             f.argtypes = [i, real, real, ptr]
             f.restype = None
         f = _lib.ffbo_s_run_quantized
         f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_int] * 4 + [ctypes.c_float] * 6 + \
-            [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+            [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_float]
         f.restype = None
         _lib.ffbo_s_quantize.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p]
         _lib.ffbo_s_quantize.restype = None
@@ -84,6 +84,7 @@ class Params:
     u_lid: float = 0.1
     rho_in: float = 1.00005
     rho_out: float = 0.99995
+    force: float = 0.0  # body-force channel (mode 5): force density along x
 
 
 def _p(a: np.ndarray) -> int:
@@ -139,13 +140,13 @@ class OracleLattice:
                 STORAGE_KINDS[self.storage], self.scale_log2, self.mode, self.nx, self.ny,
                 int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
                 _p(self.f), _p(self.f_b), _p(self.pf_state), 1 if write_macro_last else 0,
-                _p(self.rho), _p(self.u_x), _p(self.u_y))
+                _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
             self.steps += int(n_steps)
             return self
         getattr(lib(), f"ffbo_{self.p}_run")(
             self.mode, self.nx, self.ny, int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid,
             P.rho_in, P.rho_out, _p(self.f), _p(self.f_b), _p(self.pf_state),
-            1 if write_macro_last else 0, _p(self.rho), _p(self.u_x), _p(self.u_y))
+            1 if write_macro_last else 0, _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
         self.steps += int(n_steps)
         return self
 
