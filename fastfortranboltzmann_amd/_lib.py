@@ -20,7 +20,27 @@ EXPORTS = (
     "ffb_read_macros", "ffb_read_populations", "ffb_write_populations", "ffb_read_pf_strips",
     "ffb_elapsed_ms", "ffb_reset_timer", "ffb_step_count", "ffb_moments", "ffb_device_info",
     "ffb_bench_copy", "ffb_last_error", "ffb_version",
+    "ffb_batch_create", "ffb_batch_destroy", "ffb_batch_init", "ffb_batch_step", "ffb_batch_sync",
+    "ffb_batch_elapsed_ms", "ffb_batch_reset_timer", "ffb_batch_info", "ffb_batch_set_tuning",
+    "ffb_batch_autotune", "ffb_batch_read_macros", "ffb_batch_read_populations",
 )
+
+
+class FfbBatchLattice(ctypes.Structure):
+    """Mirror of ffb_batch_lattice_t (include/ffb.h)."""
+
+    _fields_ = [
+        ("n_x", ctypes.c_int32), ("n_y", ctypes.c_int32), ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32), ("omega", ctypes.c_double), ("rho_0", ctypes.c_double),
+        ("u_wall", ctypes.c_double), ("body_force", ctypes.c_double),
+        ("u_max", ctypes.c_double), ("n_sin", ctypes.c_double),
+    ]
+
+
+class FfbBatchTuning(ctypes.Structure):
+    """Mirror of ffb_batch_tuning_t (include/ffb.h)."""
+
+    _fields_ = [("tile_w", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
 
 
 class FfbConfig(ctypes.Structure):
@@ -79,6 +99,20 @@ def lib() -> ctypes.CDLL:
                              ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
                             ctypes.c_int),
         "ffb_bench_copy": ([i32, ctypes.c_int64, i32, dp], ctypes.c_int),
+        "ffb_batch_create": ([ctypes.POINTER(FfbBatchLattice), i32, i32, i32, ctypes.POINTER(vp)],
+                             ctypes.c_int),
+        "ffb_batch_destroy": ([vp], ctypes.c_int),
+        "ffb_batch_init": ([vp], ctypes.c_int),
+        "ffb_batch_step": ([vp, i32, i32], ctypes.c_int),
+        "ffb_batch_sync": ([vp], ctypes.c_int),
+        "ffb_batch_elapsed_ms": ([vp, dp], ctypes.c_int),
+        "ffb_batch_reset_timer": ([vp], ctypes.c_int),
+        "ffb_batch_info": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32),
+                            ctypes.POINTER(FfbBatchTuning)], ctypes.c_int),
+        "ffb_batch_set_tuning": ([vp, ctypes.POINTER(FfbBatchTuning)], ctypes.c_int),
+        "ffb_batch_autotune": ([vp, i32, ctypes.POINTER(FfbBatchTuning), dp], ctypes.c_int),
+        "ffb_batch_read_macros": ([vp, i32, vp, vp, vp], ctypes.c_int),
+        "ffb_batch_read_populations": ([vp, i32, vp], ctypes.c_int),
         "ffb_last_error": ([], ctypes.c_char_p),
         "ffb_version": ([], ctypes.c_char_p),
     }

@@ -1,0 +1,206 @@
+"""Batched form: hundreds of independent lattices of uneven size, each with
+its own omega and boundary kind, advanced by one persistent launch per step
+(hipGraph-replayed step pairs) in libffb_hip.so; see ffb_batch_* in
+include/ffb.h.  No reference counterpart.
+
+The mixed workload (``mixed_batch``): side lengths uniform on [64, 1024]
+(x and y drawn separately, so non-square), omega uniform on [0.6, 1.9], kinds
+shear wave / Couette (top wall u = 0.05) / body-force channel (force density
+1e-6 along x, periodic x) in equal shares, shuffled, from numpy's PCG64
+generator seeded with ``seed``.  Shear waves start from u_x = 0.01 sin(2 pi y / n_y).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, PREC_FP32, PREC_FP64, SHEAR_WAVE, SLIDING_LID,
+                   FfbBatchLattice, FfbBatchTuning, check, lib)
+
+
+@dataclass
+class BatchLattice:
+    n_x: int
+    n_y: int
+    kind: int
+    omega: float
+    rho_0: float = 1.0
+    u_wall: float = 0.0
+    body_force: float = 0.0
+    u_max: float = 0.0
+    n_sin: float = 1.0
+
+    def c(self) -> FfbBatchLattice:
+        return FfbBatchLattice(n_x=self.n_x, n_y=self.n_y, kind=self.kind, reserved=0,
+                               omega=self.omega, rho_0=self.rho_0, u_wall=self.u_wall,
+                               body_force=self.body_force, u_max=self.u_max, n_sin=self.n_sin)
+
+
+def mixed_batch(seed: int = 0, count: int = 512, side=(64, 1024), omega=(0.6, 1.9),
+                u_wall: float = 0.05, body_force: float = 1e-6, sw_u_max: float = 0.01):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kinds = np.array([SHEAR_WAVE, COUETTE_FLOW, BODY_FORCE_CHANNEL] * (count // 3 + 1))[:count]
+    rng.shuffle(kinds)
+    out = []
+    for k in kinds:
+        nx, ny = (int(v) for v in rng.integers(side[0], side[1] + 1, size=2))
+        om = float(rng.uniform(omega[0], omega[1]))
+        if k == SHEAR_WAVE:
+            out.append(BatchLattice(nx, ny, int(k), om, u_max=sw_u_max, n_sin=1.0))
+        elif k == COUETTE_FLOW:
+            out.append(BatchLattice(nx, ny, int(k), om, u_wall=u_wall))
+        else:
+            out.append(BatchLattice(nx, ny, int(k), om, body_force=body_force))
+    return out
+
+
+class Batch:
+    def __init__(self, lattices, precision: str = "fp32", device: int = 0):
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("the batched form stores FP32 or FP64")
+        self.lattices = list(lattices)
+        self.dtype = np.dtype(np.float64 if precision == "fp64" else np.float32)
+        arr = (FfbBatchLattice * len(self.lattices))(*[l.c() for l in self.lattices])
+        h = ctypes.c_void_p()
+        check(lib().ffb_batch_create(arr, len(self.lattices),
+                                     PREC_FP64 if precision == "fp64" else PREC_FP32, device,
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ffb_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self):
+        check(lib().ffb_batch_init(self._h))
+        return self
+
+    def step(self, n: int, write_macro_last: bool = False):
+        check(lib().ffb_batch_step(self._h, int(n), 1 if write_macro_last else 0))
+        return self
+
+    def sync(self):
+        check(lib().ffb_batch_sync(self._h))
+
+    def elapsed_ms(self) -> float:
+        ms = ctypes.c_double()
+        check(lib().ffb_batch_elapsed_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def reset_timer(self):
+        check(lib().ffb_batch_reset_timer(self._h))
+
+    def info(self):
+        cells, tiles, t = ctypes.c_int64(), ctypes.c_int32(), FfbBatchTuning()
+        check(lib().ffb_batch_info(self._h, ctypes.byref(cells), ctypes.byref(tiles), ctypes.byref(t)))
+        return {"cells": cells.value, "tiles": tiles.value,
+                "tuning": {"tile_w": t.tile_w, "grid_blocks": t.grid_blocks}}
+
+    def set_tuning(self, tile_w: int, grid_blocks: int = 0):
+        t = FfbBatchTuning(tile_w=tile_w, grid_blocks=grid_blocks)
+        check(lib().ffb_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, trial_steps: int = 20):
+        t, ms = FfbBatchTuning(), ctypes.c_double()
+        check(lib().ffb_batch_autotune(self._h, int(trial_steps), ctypes.byref(t), ctypes.byref(ms)))
+        return {"tile_w": t.tile_w, "grid_blocks": t.grid_blocks, "ms_per_step": ms.value}
+
+    def macros(self, i: int):
+        L = self.lattices[i]
+        rho = np.empty((L.n_y, L.n_x), self.dtype)
+        ux, uy = np.empty_like(rho), np.empty_like(rho)
+        check(lib().ffb_batch_read_macros(self._h, i, ctypes.c_void_p(rho.ctypes.data),
+                                          ctypes.c_void_p(ux.ctypes.data), ctypes.c_void_p(uy.ctypes.data)))
+        return rho, ux, uy
+
+    def populations(self, i: int) -> np.ndarray:
+        L = self.lattices[i]
+        f = np.empty((9, L.n_y + 2, L.n_x + 2), self.dtype)
+        check(lib().ffb_batch_read_populations(self._h, i, ctypes.c_void_p(f.ctypes.data)))
+        return f
+
+
+def batch_study(lattices, steps: int, autotune: bool = True, trial_steps: int = 20,
+                compare_fp64: bool = True, log=print) -> dict:
+    """Aggregate MLUPS of the FP32 batch (events around `steps` steps) and the
+    per-instance relative L2 error of |u| against an FP64 batch of the same
+    lattices."""
+    res = {}
+    with Batch(lattices, "fp32") as b:
+        if autotune:
+            res["tuning"] = b.autotune(trial_steps)
+            log(f"batch: autotuned {res['tuning']}")
+        info = b.info()
+        res.update(cells=info["cells"], tiles=info["tiles"], count=len(lattices))
+        b.init()
+        b.step(2)
+        b.sync()
+        b.reset_timer()
+        t0 = time.perf_counter()
+        b.step(steps, write_macro_last=True)
+        b.sync()
+        wall = time.perf_counter() - t0
+        ms = b.elapsed_ms()
+        res["mlups"] = info["cells"] * steps / (ms * 1e-3) / 1e6
+        res["mlups_wall"] = info["cells"] * steps / wall / 1e6
+        res["steps"] = steps + 2
+        u32 = [b.macros(i)[1:] for i in range(len(lattices))] if compare_fp64 else None
+    if compare_fp64:
+        errs = []
+        with Batch(lattices, "fp64") as b64:
+            b64.init()
+            b64.step(steps + 2, write_macro_last=True)
+            for i in range(len(lattices)):
+                _, ux, uy = b64.macros(i)
+                ux32, uy32 = u32[i]
+                num = math.sqrt(float(np.sum((ux32 - ux) ** 2 + (uy32 - uy) ** 2)))
+                den = math.sqrt(float(np.sum(ux ** 2 + uy ** 2)))
+                errs.append(num / den if den > 0 else 0.0)
+        errs = np.array(errs)
+        res["rel_l2_u_vs_fp64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
+                                   "p90": float(np.quantile(errs, 0.9))}
+        res["per_instance_rel_l2_u"] = [float(e) for e in errs]
+    return res
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="batched form: aggregate MLUPS and error vs FP64")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--trial-steps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args(argv)
+    lats = mixed_batch(args.seed, args.count)
+    res = batch_study(lats, args.steps, trial_steps=args.trial_steps, compare_fp64=not args.no_fp64,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    res.pop("per_instance_rel_l2_u", None) if args.count > 64 else None
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())

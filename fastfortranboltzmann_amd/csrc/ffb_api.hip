@@ -64,6 +64,35 @@ static int cells_per_thread(const ffb_lattice *L) { return L->vec ? (int)(16 / L
 
 namespace {
 
+// Second pass of the FP64 moment sums: reduces the per-block partials.
+__global__ void __launch_bounds__(256) moments_final_kernel(const double *partial, int n, double *out) {
+    __shared__ double sh[3][256];
+    double m = 0, mx = 0, my = 0;
+    for (int k = (int)threadIdx.x; k < n; k += 256) {
+        m += partial[3 * k];
+        mx += partial[3 * k + 1];
+        my += partial[3 * k + 2];
+    }
+    sh[0][threadIdx.x] = m;
+    sh[1][threadIdx.x] = mx;
+    sh[2][threadIdx.x] = my;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+
 template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macro) {
     StepArgs<C> a{};
     const ffb_config_t &c = L->cfg;

@@ -1,0 +1,567 @@
+// ffb_batch.hip -- batched form: many independent lattices of uneven size,
+// each with its own omega and boundary kind, advanced together by ONE
+// persistent kernel launch per step; the two launches of a ping-pong pair
+// are captured once in a hipGraph and replayed.  A project-written
+// autotuner picks the workgroup size and the persistent grid.  No reference
+// counterpart (the reference runs one domain per process).
+//
+// Layout: every lattice keeps the single-domain layout (9 SoA planes, row
+// pitch rounded to 64 elements, no halo) inside two pools (ping-pong); a
+// descriptor table holds each lattice's offsets and parameters.  Work is cut
+// into tiles of TW cells in row-major order within a lattice (a tile may
+// cover several short rows); tile_lat[t] names the lattice of tile t.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "d2q9_kernels.h"
+#include "ffb.h"
+#include "ffb_internal.h"
+
+using namespace ffb;
+
+#define BHIPCHK(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ffb::fail(FFB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+namespace {
+
+template <typename C> struct BatchLat {
+    long long off;        // element offset of the lattice's planes in a pool
+    long long plane;      // elements per plane
+    long long macro_off;  // element offset in the macro arrays
+    long long prof_off;   // element offset of the shear-wave profile row table
+    int nx, ny, pitch, kind;
+    int tile_start, n_cells;
+    C omega, corr8, corr9, force, rho_0;
+};
+
+template <typename C> struct BatchArgs {
+    const void *src_pool;
+    void *dst_pool;
+    const BatchLat<C> *lats;
+    const int *tile_lat;
+    int n_tiles;
+    int write_macro;
+    C *rho_o, *ux_o, *uy_o;
+    const C *prof;        // shear-wave u_x(y) rows of all SW lattices
+    C w[9];
+};
+
+template <typename C>
+__device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const BatchLat<C> &L) {
+    StepArgs<C> a;
+    using R = C;  // the batch stores FP32 or FP64 populations
+    a.src = static_cast<const R *>(b.src_pool) + L.off;
+    a.dst = static_cast<R *>(b.dst_pool) + L.off;
+    a.nx = L.nx;
+    a.ny = L.ny;
+    a.pitch = L.pitch;
+    a.plane = L.plane;
+    a.omega = L.omega;
+    a.corr8 = L.corr8;
+    a.corr9 = L.corr9;
+    a.rho_in = 0;
+    a.rho_out = 0;
+    a.m_old = nullptr;
+    a.m_new = nullptr;
+    a.rho_o = b.rho_o + L.macro_off;
+    a.ux_o = b.ux_o + L.macro_off;
+    a.uy_o = b.uy_o + L.macro_off;
+    a.write_macro = b.write_macro;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) a.w[d] = b.w[d];
+    a.scale = 1;
+    a.inv_scale = 1;
+    a.n_tiles_x = 0;
+    a.n_blocks = 0;
+    a.xcd_order = 0;
+    a.force = L.force;
+    return a;
+}
+
+// One step of every lattice: a persistent grid walks the tiles.
+template <typename C, int TW>
+__global__ void __launch_bounds__(TW) batch_step_kernel(BatchArgs<C> b) {
+    for (int t = (int)blockIdx.x; t < b.n_tiles; t += (int)gridDim.x) {
+        const BatchLat<C> L = b.lats[b.tile_lat[t]];
+        const int q = (t - L.tile_start) * TW + (int)threadIdx.x;
+        if (q >= L.n_cells) continue;
+        const int y = q / L.nx;
+        const int x = q - y * L.nx;
+        const StepArgs<C> a = lattice_args<C>(b, L);
+        const C *f = static_cast<const C *>(a.src);
+        C *g = static_cast<C *>(a.dst);
+        switch (L.kind) {
+        case SW: step_cell<C, C, SW>(a, f, g, x, y); break;
+        case CF: step_cell<C, C, CF>(a, f, g, x, y); break;
+        case SL: step_cell<C, C, SL>(a, f, g, x, y); break;
+        default: step_cell<C, C, BF>(a, f, g, x, y); break;
+        }
+    }
+}
+
+// Equilibrium initial state of every lattice (as init_kernel: shear-wave
+// lattices start from their u_x(y) profile, the others at rest).
+template <typename C, int TW>
+__global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
+    for (int t = (int)blockIdx.x; t < b.n_tiles; t += (int)gridDim.x) {
+        const BatchLat<C> L = b.lats[b.tile_lat[t]];
+        const int q = (t - L.tile_start) * TW + (int)threadIdx.x;
+        if (q >= L.n_cells) continue;
+        const int y = q / L.nx;
+        const int x = q - y * L.nx;
+        C *g = static_cast<C *>(b.dst_pool) + L.off;
+        const size_t c = (size_t)y * (size_t)L.pitch + (size_t)x;
+        const bool sw = L.kind == SW;
+        const C ux = sw ? b.prof[L.prof_off + y] : (C)0;
+        const C uy = 0;
+        const C usq = ux * ux;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            C feq;
+            if (sw) {
+                const C cu = (C)cx_of(d) * ux + (C)cy_of(d) * uy;
+                feq = b.w[d] * L.rho_0 * ((((C)1 + (C)3 * cu) + (C)4.5 * cu * cu) - (C)1.5 * usq);
+            } else {
+                feq = b.w[d] * L.rho_0;
+            }
+            g[(size_t)d * (size_t)L.plane + c] = feq;
+        }
+        const size_t m = (size_t)L.macro_off + (size_t)q;
+        b.rho_o[m] = L.rho_0;
+        b.ux_o[m] = ux;
+        b.uy_o[m] = uy;
+    }
+}
+
+}  // namespace
+
+struct ffb_batch {
+    int precision = FFB_PREC_FP32;
+    int device = 0;
+    size_t elem = 4;
+    std::vector<ffb_batch_lattice_t> desc;
+    std::vector<long long> off, plane, macro_off, prof_off;
+    std::vector<int> pitch;
+    long long pool_elems = 0, macro_elems = 0, prof_elems = 0, cells = 0;
+    void *pool[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *macro[3] = {nullptr, nullptr, nullptr};
+    void *prof = nullptr;
+    void *lats = nullptr;     // BatchLat<C>[count]
+    int *tile_lat = nullptr;  // per tile
+    int n_tiles = 0;
+    ffb_batch_tuning_t tuning{256, 0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    long long steps = 0;
+};
+
+namespace {
+
+int grid_for(const ffb_batch *B) {
+    const int g = B->tuning.grid_blocks;
+    return (g <= 0 || g > B->n_tiles) ? B->n_tiles : g;
+}
+
+// (Re)builds the descriptor and tile tables for the current tile width.
+template <typename C> int upload_tables(ffb_batch *B) {
+    const int TW = B->tuning.tile_w;
+    const int n = (int)B->desc.size();
+    std::vector<BatchLat<C>> lats((size_t)n);
+    std::vector<int> tiles;
+    for (int i = 0; i < n; ++i) {
+        const ffb_batch_lattice_t &d = B->desc[(size_t)i];
+        BatchLat<C> &L = lats[(size_t)i];
+        L.off = B->off[(size_t)i];
+        L.plane = B->plane[(size_t)i];
+        L.macro_off = B->macro_off[(size_t)i];
+        L.prof_off = B->prof_off[(size_t)i];
+        L.nx = d.n_x;
+        L.ny = d.n_y;
+        L.pitch = B->pitch[(size_t)i];
+        L.kind = d.kind;
+        L.n_cells = d.n_x * d.n_y;
+        L.tile_start = (int)tiles.size();
+        const int nt = (L.n_cells + TW - 1) / TW;
+        tiles.insert(tiles.end(), (size_t)nt, i);
+        L.omega = (C)d.omega;
+        const C w6 = (C)1 / (C)36;
+        L.corr8 = (C)6 * w6 * (C)d.rho_0 * (C)d.u_wall;  // as couette_flow.f90:118-119
+        L.corr9 = L.corr8;
+        L.force = (C)d.body_force;
+        L.rho_0 = (C)d.rho_0;
+    }
+    if (B->lats) (void)hipFree(B->lats);
+    if (B->tile_lat) (void)hipFree(B->tile_lat);
+    B->lats = nullptr;
+    B->tile_lat = nullptr;
+    BHIPCHK(hipMalloc(&B->lats, lats.size() * sizeof(BatchLat<C>)));
+    BHIPCHK(hipMalloc((void **)&B->tile_lat, tiles.size() * sizeof(int)));
+    BHIPCHK(hipMemcpy(B->lats, lats.data(), lats.size() * sizeof(BatchLat<C>), hipMemcpyHostToDevice));
+    BHIPCHK(hipMemcpy(B->tile_lat, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice));
+    B->n_tiles = (int)tiles.size();
+    return FFB_OK;
+}
+
+template <typename C> BatchArgs<C> batch_args(const ffb_batch *B, int parity, int write_macro) {
+    BatchArgs<C> b{};
+    b.src_pool = B->pool[parity];
+    b.dst_pool = B->pool[1 - parity];
+    b.lats = static_cast<const BatchLat<C> *>(B->lats);
+    b.tile_lat = B->tile_lat;
+    b.n_tiles = B->n_tiles;
+    b.write_macro = write_macro;
+    b.rho_o = static_cast<C *>(B->macro[0]);
+    b.ux_o = static_cast<C *>(B->macro[1]);
+    b.uy_o = static_cast<C *>(B->macro[2]);
+    b.prof = static_cast<const C *>(B->prof);
+    b.w[0] = (C)4 / (C)9;
+    for (int d = 1; d <= 4; ++d) b.w[d] = (C)1 / (C)9;
+    for (int d = 5; d <= 8; ++d) b.w[d] = (C)1 / (C)36;
+    return b;
+}
+
+template <typename C> void launch_step_c(const ffb_batch *B, int parity, int write_macro) {
+    const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
+    const dim3 grid((unsigned)grid_for(B));
+    switch (B->tuning.tile_w) {
+    case 128: hipLaunchKernelGGL((batch_step_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
+    case 512: hipLaunchKernelGGL((batch_step_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
+    case 1024: hipLaunchKernelGGL((batch_step_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
+    default: hipLaunchKernelGGL((batch_step_kernel<C, 256>), grid, dim3(256), 0, B->stream, b); break;
+    }
+}
+
+void launch_step(const ffb_batch *B, int parity, int write_macro) {
+    if (B->precision == FFB_PREC_FP64) launch_step_c<double>(B, parity, write_macro);
+    else launch_step_c<float>(B, parity, write_macro);
+}
+
+int drop_graph(ffb_batch *B) {
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->graph_exec = nullptr;
+    B->graph = nullptr;
+    return FFB_OK;
+}
+
+// Captures the two launches of one ping-pong pair (parity 0 then 1).
+int build_graph(ffb_batch *B) {
+    drop_graph(B);
+    BHIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(B, 0, 0);
+    launch_step(B, 1, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(B->stream, &g);
+    if (e != hipSuccess) return fail(FFB_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    B->graph = g;
+    BHIPCHK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+    return FFB_OK;
+}
+
+template <typename C> int init_c(ffb_batch *B) {
+    BatchArgs<C> b = batch_args<C>(B, 1, 0);  // dst = pool[0]
+    const dim3 grid((unsigned)B->n_tiles);
+    switch (B->tuning.tile_w) {
+    case 128: hipLaunchKernelGGL((batch_init_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
+    case 512: hipLaunchKernelGGL((batch_init_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
+    case 1024: hipLaunchKernelGGL((batch_init_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
+    default: hipLaunchKernelGGL((batch_init_kernel<C, 256>), grid, dim3(256), 0, B->stream, b); break;
+    }
+    BHIPCHK(hipGetLastError());
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    return FFB_OK;
+}
+
+int apply_tuning(ffb_batch *B, const ffb_batch_tuning_t &t) {
+    if (t.tile_w != 128 && t.tile_w != 256 && t.tile_w != 512 && t.tile_w != 1024)
+        return fail(FFB_E_INVALID, "ffb_batch: tile_w must be 128, 256, 512 or 1024");
+    if (t.grid_blocks < 0) return fail(FFB_E_INVALID, "ffb_batch: grid_blocks must be >= 0");
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    B->tuning = t;
+    int rc = B->precision == FFB_PREC_FP64 ? upload_tables<double>(B) : upload_tables<float>(B);
+    if (rc) return rc;
+    return build_graph(B);
+}
+
+template <typename C> int read_macros_c(ffb_batch *B, int i, void *rho, void *ux, void *uy) {
+    const size_t n = (size_t)B->desc[(size_t)i].n_x * (size_t)B->desc[(size_t)i].n_y;
+    void *dst[3] = {rho, ux, uy};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k])
+            BHIPCHK(hipMemcpyAsync(dst[k], static_cast<C *>(B->macro[k]) + B->macro_off[(size_t)i], n * sizeof(C),
+                                   hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    return FFB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t precision, int32_t device,
+                     ffb_batch_t **out) {
+    if (!lattices || !out || count < 1) return fail(FFB_E_INVALID, "ffb_batch_create: bad argument");
+    *out = nullptr;
+    if (precision != FFB_PREC_FP32 && precision != FFB_PREC_FP64)
+        return fail(FFB_E_INVALID, "ffb_batch_create: precision must be FFB_PREC_FP32 or FFB_PREC_FP64");
+    for (int i = 0; i < count; ++i) {
+        const ffb_batch_lattice_t &d = lattices[i];
+        if (d.n_x < 1 || d.n_y < 1 || (long long)d.n_x * d.n_y > (1LL << 30))
+            return fail(FFB_E_INVALID, "ffb_batch_create: lattice " + std::to_string(i) + " has a bad size");
+        if (d.kind != FFB_SHEAR_WAVE && d.kind != FFB_COUETTE_FLOW && d.kind != FFB_SLIDING_LID &&
+            d.kind != FFB_BODY_FORCE_CHANNEL)
+            return fail(FFB_E_INVALID, "ffb_batch_create: lattice " + std::to_string(i) +
+                                           " kind must be shear wave, Couette, sliding lid or body-force channel");
+    }
+    int ndev = 0;
+    BHIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(FFB_E_INVALID, "ffb_batch_create: device not present");
+    BHIPCHK(hipSetDevice(device));
+
+    ffb_batch *B = new ffb_batch();
+    B->precision = precision;
+    B->device = device;
+    B->elem = precision == FFB_PREC_FP64 ? 8 : 4;
+    B->desc.assign(lattices, lattices + count);
+    long long off = 0, moff = 0, poff = 0;
+    for (int i = 0; i < count; ++i) {
+        const ffb_batch_lattice_t &d = lattices[i];
+        const int pitch = ((d.n_x + 63) / 64) * 64;
+        const long long plane = (long long)pitch * d.n_y;
+        B->pitch.push_back(pitch);
+        B->plane.push_back(plane);
+        B->off.push_back(off);
+        B->macro_off.push_back(moff);
+        B->prof_off.push_back(poff);
+        off += 9 * plane;
+        moff += (long long)d.n_x * d.n_y;
+        poff += d.kind == FFB_SHEAR_WAVE ? d.n_y : 0;
+        B->cells += (long long)d.n_x * d.n_y;
+    }
+    B->pool_elems = off;
+    B->macro_elems = moff;
+    B->prof_elems = std::max(1LL, poff);
+
+    auto bail = [&](hipError_t e, const char *what) {
+        ffb_batch_destroy(B);
+        return fail(FFB_E_HIP, std::string("ffb_batch_create: ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    for (int k = 0; k < 2; ++k)
+        if ((e = hipMalloc(&B->pool[k], (size_t)B->pool_elems * B->elem)) != hipSuccess) return bail(e, "pools");
+    for (int k = 0; k < 3; ++k)
+        if ((e = hipMalloc(&B->macro[k], (size_t)B->macro_elems * B->elem)) != hipSuccess) return bail(e, "macros");
+    if ((e = hipMalloc(&B->prof, (size_t)B->prof_elems * B->elem)) != hipSuccess) return bail(e, "profiles");
+    if ((e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "stream");
+    if ((e = hipEventCreate(&B->ev_start)) != hipSuccess) return bail(e, "event");
+    if ((e = hipEventCreate(&B->ev_stop)) != hipSuccess) return bail(e, "event");
+
+    // shear-wave profiles u_x(y) = u_max sin(2 pi n_sin y / n_y), evaluated on the
+    // host with the C library's sin in the storage precision (as the single-domain
+    // initial condition, initialization.f90:42)
+    {
+        std::vector<unsigned char> prof((size_t)B->prof_elems * B->elem, 0);
+        for (int i = 0; i < count; ++i) {
+            const ffb_batch_lattice_t &d = lattices[i];
+            if (d.kind != FFB_SHEAR_WAVE) continue;
+            for (int y = 0; y < d.n_y; ++y) {
+                const size_t k = (size_t)B->prof_off[(size_t)i] + (size_t)y;
+                if (precision == FFB_PREC_FP64) {
+                    const double pi = 3.141592653589793238462643383279502884197;
+                    ((double *)prof.data())[k] = d.u_max * std::sin(((2.0 * pi) * d.n_sin * (double)y) / (double)d.n_y);
+                } else {
+                    const float pi = (float)3.141592653589793238462643383279502884197L;
+                    ((float *)prof.data())[k] = (float)d.u_max *
+                        std::sin((((float)2 * pi) * (float)d.n_sin * (float)y) / (float)d.n_y);
+                }
+            }
+        }
+        if ((e = hipMemcpy(B->prof, prof.data(), prof.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "profiles");
+    }
+    ffb_batch_tuning_t t{256, 0};
+    int rc = apply_tuning(B, t);
+    if (rc) {
+        ffb_batch_destroy(B);
+        return rc;
+    }
+    *out = B;
+    return FFB_OK;
+}
+
+int ffb_batch_destroy(ffb_batch_t *B) {
+    if (!B) return FFB_OK;
+    (void)hipSetDevice(B->device);
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    drop_graph(B);
+    for (void *p : B->pool) if (p) (void)hipFree(p);
+    for (void *p : B->macro) if (p) (void)hipFree(p);
+    if (B->prof) (void)hipFree(B->prof);
+    if (B->lats) (void)hipFree(B->lats);
+    if (B->tile_lat) (void)hipFree(B->tile_lat);
+    if (B->ev_start) (void)hipEventDestroy(B->ev_start);
+    if (B->ev_stop) (void)hipEventDestroy(B->ev_stop);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return FFB_OK;
+}
+
+int ffb_batch_init(ffb_batch_t *B) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_init: null handle");
+    BHIPCHK(hipSetDevice(B->device));
+    B->cur = 0;
+    B->steps = 0;
+    B->timing = false;
+    return B->precision == FFB_PREC_FP64 ? init_c<double>(B) : init_c<float>(B);
+}
+
+int ffb_batch_step(ffb_batch_t *B, int32_t n_steps, int32_t write_macro_last) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_step: null handle");
+    if (n_steps < 0) return fail(FFB_E_INVALID, "ffb_batch_step: n_steps < 0");
+    if (n_steps == 0) return FFB_OK;
+    BHIPCHK(hipSetDevice(B->device));
+    if (!B->timing) {
+        BHIPCHK(hipEventRecord(B->ev_start, B->stream));
+        B->timing = true;
+    }
+    int left = n_steps;
+    // the last step runs outside the graph when it writes the macro fields
+    const int tail = write_macro_last ? 1 : 0;
+    // align to an even parity so the captured pair (0 then 1) applies
+    if (B->cur == 1 && left - tail > 0) {
+        launch_step(B, 1, 0);
+        B->cur = 0;
+        --left;
+    }
+    while (left - tail >= 2) {
+        BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
+        left -= 2;
+    }
+    while (left > 0) {
+        launch_step(B, B->cur, (left == 1 && write_macro_last) ? 1 : 0);
+        B->cur = 1 - B->cur;
+        --left;
+    }
+    BHIPCHK(hipGetLastError());
+    BHIPCHK(hipEventRecord(B->ev_stop, B->stream));
+    B->steps += n_steps;
+    return FFB_OK;
+}
+
+int ffb_batch_sync(ffb_batch_t *B) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_sync: null handle");
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    return FFB_OK;
+}
+
+int ffb_batch_elapsed_ms(ffb_batch_t *B, double *ms) {
+    if (!B || !ms) return fail(FFB_E_INVALID, "ffb_batch_elapsed_ms: null argument");
+    *ms = 0;
+    if (!B->timing) return FFB_OK;
+    BHIPCHK(hipEventSynchronize(B->ev_stop));
+    float t = 0.f;
+    BHIPCHK(hipEventElapsedTime(&t, B->ev_start, B->ev_stop));
+    *ms = t;
+    return FFB_OK;
+}
+
+int ffb_batch_reset_timer(ffb_batch_t *B) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_reset_timer: null handle");
+    B->timing = false;
+    return FFB_OK;
+}
+
+int ffb_batch_info(ffb_batch_t *B, int64_t *cells, int32_t *tiles, ffb_batch_tuning_t *tuning) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_info: null handle");
+    if (cells) *cells = B->cells;
+    if (tiles) *tiles = B->n_tiles;
+    if (tuning) *tuning = B->tuning;
+    return FFB_OK;
+}
+
+int ffb_batch_set_tuning(ffb_batch_t *B, const ffb_batch_tuning_t *t) {
+    if (!B || !t) return fail(FFB_E_INVALID, "ffb_batch_set_tuning: null argument");
+    BHIPCHK(hipSetDevice(B->device));
+    return apply_tuning(B, *t);
+}
+
+int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *best_out, double *best_ms) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_autotune: null handle");
+    if (trial_steps < 2) trial_steps = 2;
+    trial_steps += trial_steps & 1;  // whole ping-pong pairs
+    BHIPCHK(hipSetDevice(B->device));
+    int cus = 0;
+    BHIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device));
+    const int tws[] = {128, 256, 512, 1024};
+    const int per_cu[] = {0, 2, 4, 8, 16};  // 0: one tile per workgroup (no persistence)
+    ffb_batch_tuning_t best{256, 0};
+    double best_t = 1e30;
+    int rc = ffb_batch_init(B);
+    if (rc) return rc;
+    for (int tw : tws) {
+        for (int pc : per_cu) {
+            ffb_batch_tuning_t t{tw, pc ? pc * cus : 0};
+            if ((rc = apply_tuning(B, t))) return rc;
+            // warm-up pair, then the timed trial
+            BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
+            BHIPCHK(hipEventRecord(B->ev_start, B->stream));
+            for (int k = 0; k < trial_steps / 2; ++k) BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
+            BHIPCHK(hipEventRecord(B->ev_stop, B->stream));
+            BHIPCHK(hipEventSynchronize(B->ev_stop));
+            float ms = 0.f;
+            BHIPCHK(hipEventElapsedTime(&ms, B->ev_start, B->ev_stop));
+            if (ms < best_t) {
+                best_t = ms;
+                best = t;
+            }
+        }
+    }
+    if ((rc = apply_tuning(B, best))) return rc;
+    if ((rc = ffb_batch_init(B))) return rc;  // trials advanced the state: start again
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = best_t / trial_steps;
+    return FFB_OK;
+}
+
+int ffb_batch_read_macros(ffb_batch_t *B, int32_t index, void *rho, void *u_x, void *u_y) {
+    if (!B) return fail(FFB_E_INVALID, "ffb_batch_read_macros: null handle");
+    if (index < 0 || index >= (int)B->desc.size()) return fail(FFB_E_INVALID, "ffb_batch_read_macros: bad index");
+    BHIPCHK(hipSetDevice(B->device));
+    return B->precision == FFB_PREC_FP64 ? read_macros_c<double>(B, index, rho, u_x, u_y)
+                                         : read_macros_c<float>(B, index, rho, u_x, u_y);
+}
+
+int ffb_batch_read_populations(ffb_batch_t *B, int32_t index, void *f_halo) {
+    if (!B || !f_halo) return fail(FFB_E_INVALID, "ffb_batch_read_populations: null argument");
+    if (index < 0 || index >= (int)B->desc.size())
+        return fail(FFB_E_INVALID, "ffb_batch_read_populations: bad index");
+    BHIPCHK(hipSetDevice(B->device));
+    const ffb_batch_lattice_t &d = B->desc[(size_t)index];
+    const size_t plane = (size_t)B->plane[(size_t)index], pitch = (size_t)B->pitch[(size_t)index];
+    std::vector<unsigned char> stage(9 * plane * B->elem);
+    BHIPCHK(hipMemcpyAsync(stage.data(), (unsigned char *)B->pool[B->cur] + (size_t)B->off[(size_t)index] * B->elem,
+                           stage.size(), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    const size_t hp = (size_t)d.n_x + 2, hplane = hp * ((size_t)d.n_y + 2);
+    std::memset(f_halo, 0, 9 * hplane * B->elem);
+    for (int dd = 0; dd < 9; ++dd)
+        for (int y = 0; y < d.n_y; ++y)
+            std::memcpy((unsigned char *)f_halo + ((size_t)dd * hplane + (size_t)(y + 1) * hp + 1) * B->elem,
+                        stage.data() + ((size_t)dd * plane + (size_t)y * pitch) * B->elem, (size_t)d.n_x * B->elem);
+    return FFB_OK;
+}
+
+}  // extern "C"

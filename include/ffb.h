@@ -152,6 +152,57 @@ int ffb_device_info(ffb_lattice_t *lat, void **cur, void **stream, int64_t *pitc
  * step kernel is compared with. */
 int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
 
+/* ------------------------------------------------------------------------
+ * Batched form (no reference counterpart: the reference runs one domain per
+ * process).  Many independent lattices of uneven size, each with its own
+ * omega and boundary kind, advance together: one persistent kernel launch per
+ * step covers all of them, and the launches of a ping-pong pair are captured
+ * once in a hipGraph and replayed.  ffb_batch_autotune() picks the workgroup
+ * size and the persistent grid by timing the candidates on the batch.
+ * ------------------------------------------------------------------------ */
+typedef struct ffb_batch_lattice {
+    int32_t n_x, n_y;
+    int32_t kind;         /* FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_SLIDING_LID or
+                             FFB_BODY_FORCE_CHANNEL (Poiseuille flow needs the
+                             single-domain API) */
+    int32_t reserved;
+    double omega;
+    double rho_0;
+    double u_wall;        /* Couette top wall / sliding lid speed */
+    double body_force;    /* body-force channel: force density along x */
+    double u_max, n_sin;  /* shear wave: initial u_x = u_max sin(2 pi n_sin y / n_y) */
+} ffb_batch_lattice_t;
+
+typedef struct ffb_batch_tuning {
+    int32_t tile_w;       /* threads per workgroup (cells per tile): 128, 256, 512, 1024 */
+    int32_t grid_blocks;  /* persistent grid size; 0 = one workgroup per tile */
+} ffb_batch_tuning_t;
+
+typedef struct ffb_batch ffb_batch_t;
+
+/* precision: FFB_PREC_FP32 or FFB_PREC_FP64. */
+int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t precision, int32_t device,
+                     ffb_batch_t **out);
+int ffb_batch_destroy(ffb_batch_t *batch);
+/* Equilibrium initial state of every lattice (shear wave: its sine profile;
+ * the others at rest with rho_0). */
+int ffb_batch_init(ffb_batch_t *batch);
+/* Enqueues n_steps steps of every lattice (graph replays of step pairs). */
+int ffb_batch_step(ffb_batch_t *batch, int32_t n_steps, int32_t write_macro_last);
+int ffb_batch_sync(ffb_batch_t *batch);
+int ffb_batch_elapsed_ms(ffb_batch_t *batch, double *ms);
+int ffb_batch_reset_timer(ffb_batch_t *batch);
+/* Total cells, number of tiles, and the tuning in use. */
+int ffb_batch_info(ffb_batch_t *batch, int64_t *cells, int32_t *tiles, ffb_batch_tuning_t *tuning);
+int ffb_batch_set_tuning(ffb_batch_t *batch, const ffb_batch_tuning_t *tuning);
+/* Times every candidate (tile_w x persistent grid) for trial_steps steps,
+ * keeps the fastest, and re-initialises the batch (the trials advance it). */
+int ffb_batch_autotune(ffb_batch_t *batch, int32_t trial_steps, ffb_batch_tuning_t *best, double *best_ms_per_step);
+/* Macro fields of lattice `index` after a write_macro step (n_x*n_y each). */
+int ffb_batch_read_macros(ffb_batch_t *batch, int32_t index, void *rho, void *u_x, void *u_y);
+/* Populations of lattice `index` as f(0:NX+1, 0:NY+1, 1:9) (halo zero). */
+int ffb_batch_read_populations(ffb_batch_t *batch, int32_t index, void *f_halo);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *ffb_last_error(void);
 

@@ -183,3 +183,41 @@ def test_fp32_oracle_tracks_fp64():
     a = OracleLattice(4, 20, 16, p, np.float64).init().run(200)
     b = OracleLattice(4, 20, 16, p, np.float32).init().run(200)
     assert np.abs(a.interior - b.interior).max() < 1e-5
+
+
+def test_body_force_channel_without_force_is_resting_couette_bitwise():
+    p = Params(omega=1.4, u_wall=0.0, force=0.0)
+    f0 = random_state(19, 11, 9)
+    a = OracleLattice(5, 19, 11, p).init()
+    b = OracleLattice(2, 19, 11, p).init()
+    a.f[:] = f0
+    b.f[:] = f0
+    a.run(30)
+    b.run(30)
+    np.testing.assert_array_equal(a.interior, b.interior)
+
+
+def test_guo_forcing_adds_force_to_momentum_each_step():
+    """Away from walls, Guo forcing changes sum_i c_i f_i by exactly F per cell
+    per step (the collision conserves the half-force-shifted momentum)."""
+    nx, ny, F = 16, 12, 1e-4
+    p = Params(omega=1.2, force=F)
+    o = OracleLattice(5, nx, ny, p).init()
+    mx0 = (CX[:, None, None] * o.interior).sum()
+    o.run(1)
+    mx1 = (CX[:, None, None] * o.interior).sum()
+    # from rest the walls return no momentum in the first step
+    assert mx1 - mx0 == pytest.approx(nx * ny * F, rel=1e-10)
+
+
+def test_body_force_channel_steady_profile_is_parabolic():
+    """u_x(y) = F/(2 rho nu) (y - 1/2)(N_Y + 1/2 - y) with halfway walls; BGK's
+    omega-dependent wall slip is O(1/N_Y^2) relative."""
+    ny, F = 24, 1e-6
+    for omega in (0.8, 1.0, 1.5):
+        o = OracleLattice(5, 4, ny, Params(omega=omega, force=F)).init().run(int(8 * ny * ny / ((1 / omega - 0.5) / 3) / 10) + 2000)
+        y = np.arange(1, ny + 1)
+        nu = (1 / omega - 0.5) / 3
+        expect = F / (2 * nu) * (y - 0.5) * (ny + 0.5 - y)
+        got = o.u_x.mean(1)
+        assert np.abs(got - expect).max() / expect.max() < 0.02, omega
