@@ -66,6 +66,69 @@ def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0
     }
 
 
+def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000) -> dict:
+    """16-bit deviation storage: MLUPS at n^2 (cavity, Re 5000) against FP32 at
+    the same size, and the relative L2 velocity error against FP64 on a
+    smaller cavity run (err_n^2, Re 1000, err_steps steps)."""
+    from fastfortranboltzmann_amd.precision import (cavity_settings, compare, default_scale_log2,
+                                                    run_form)
+
+    out = {"size": n, "steps": steps}
+    rates = {}
+    for form in ("fp32", "fp16dev", "bf16dev"):
+        s = default_scale_log2(form) if form.endswith("dev") else 0
+        rates[form] = run_form(cavity_settings(n, 5000.0, 0.1, form, s), steps, chunk=steps)["mlups"]
+    ref = run_form(cavity_settings(err_n, 1000.0, 0.1, "fp64"), err_steps, chunk=err_steps)
+    for form in ("fp16dev", "bf16dev"):
+        s = default_scale_log2(form)
+        r = run_form(cavity_settings(err_n, 1000.0, 0.1, form, s), err_steps, chunk=err_steps)
+        out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
+                     "achieved_gbs": round(rates[form] * 1e6 * 36 / 1e9, 1),
+                     "scale_log2": s, "rel_l2_u_vs_fp64": compare(ref, r)["rel_l2_u"],
+                     "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
+    out["fp32_mlups"] = round(rates["fp32"], 1)
+    return out
+
+
+def batched(single_fp32_mlups: float, steps: int = 200) -> dict:
+    """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
+    Couette / body-force channel), FP32, autotuned persistent launch."""
+    from fastfortranboltzmann_amd.batch import Batch, mixed_batch
+
+    lats = mixed_batch(0, 512)
+    with Batch(lats, "fp32") as b:
+        tun = b.autotune(20)
+        info = b.info()
+        b.init()
+        b.step(2)
+        b.sync()
+        b.reset_timer()
+        b.step(steps)
+        b.sync()
+        ms = b.elapsed_ms()
+    mlups = info["cells"] * steps / (ms * 1e-3) / 1e6
+    return {"lattices": len(lats), "cells": info["cells"], "steps": steps, "mlups": round(mlups, 1),
+            "vs_single_4096_fp32": round(mlups / single_fp32_mlups, 3),
+            "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]}}
+
+
+def adjoint_cost(n: int = 512, steps: int = 2000) -> dict:
+    """Body-force channel n^2, FP32: forward (with checkpoints every 100 steps)
+    and reverse sweep (recomputation + adjoint) device times."""
+    import numpy as np
+
+    from fastfortranboltzmann_amd.adjoint import Adjoint, channel_settings
+
+    with Adjoint(channel_settings(n, 1.2, 1e-6, "fp32"), steps, 100) as a:
+        for _ in range(2):
+            _, ux, uy = a.forward()
+            a.backward(np.ones_like(ux), np.zeros_like(uy))
+        f_ms, b_ms = a.timing()
+    return {"size": n, "steps": steps, "forward_ms": round(f_ms, 2), "backward_ms": round(b_ms, 2),
+            "backward_over_forward": round(b_ms / f_ms, 3),
+            "forward_mlups": round(n * n * steps / (f_ms * 1e-3) / 1e6, 1)}
+
+
 def copy_ceiling() -> float | None:
     from fastfortranboltzmann_amd import _lib
     import ctypes
@@ -137,6 +200,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 run of the workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the reduced-storage, batched and adjoint measurements")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the domain is one device-resident "
@@ -154,6 +219,13 @@ def main() -> int:
         extra["fp64"] = {"mlups": round(r64["mlups"], 1), "achieved_gbs": round(r64["achieved_gbs"], 1),
                          "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
                          "kernel_ms": round(r64["kernel_ms"], 5)}
+    if not args.no_extras and args.precision == "fp32":
+        print("bench: reduced storage", file=sys.stderr, flush=True)
+        extra["reduced_storage"] = reduced_storage(2 * n if n <= 4096 else n, max(50, args.steps // 10))
+        print("bench: batched", file=sys.stderr, flush=True)
+        extra["batched"] = batched(main_run["mlups"])
+        print("bench: adjoint", file=sys.stderr, flush=True)
+        extra["adjoint"] = adjoint_cost()
     cc = copy_ceiling()
     workload = f"sl{n}_{args.precision}"
     traffic = traffic_from_profiles(workload)

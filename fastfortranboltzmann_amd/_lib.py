@@ -23,6 +23,8 @@ EXPORTS = (
     "ffb_batch_create", "ffb_batch_destroy", "ffb_batch_init", "ffb_batch_step", "ffb_batch_sync",
     "ffb_batch_elapsed_ms", "ffb_batch_reset_timer", "ffb_batch_info", "ffb_batch_set_tuning",
     "ffb_batch_autotune", "ffb_batch_read_macros", "ffb_batch_read_populations",
+    "ffb_set_parameters", "ffb_adjoint_create", "ffb_adjoint_destroy", "ffb_adjoint_forward",
+    "ffb_adjoint_backward", "ffb_adjoint_timing",
 )
 
 
@@ -113,6 +115,13 @@ def lib() -> ctypes.CDLL:
         "ffb_batch_autotune": ([vp, i32, ctypes.POINTER(FfbBatchTuning), dp], ctypes.c_int),
         "ffb_batch_read_macros": ([vp, i32, vp, vp, vp], ctypes.c_int),
         "ffb_batch_read_populations": ([vp, i32, vp], ctypes.c_int),
+        "ffb_set_parameters": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double], ctypes.c_int),
+        "ffb_adjoint_create": ([vp, i32, i32, ctypes.POINTER(vp)], ctypes.c_int),
+        "ffb_adjoint_destroy": ([vp], ctypes.c_int),
+        "ffb_adjoint_forward": ([vp, vp, vp, vp], ctypes.c_int),
+        "ffb_adjoint_backward": ([vp, vp, vp, dp, dp, dp], ctypes.c_int),
+        "ffb_adjoint_timing": ([vp, dp, dp], ctypes.c_int),
         "ffb_last_error": ([], ctypes.c_char_p),
         "ffb_version": ([], ctypes.c_char_p),
     }

@@ -1,0 +1,176 @@
+"""Differentiation form: gradients of a loss of the final velocity field with
+respect to omega, the body force and the wall/lid speed, from the hand-written
+adjoint in libffb_hip.so (ffb_adjoint_* in include/ffb.h; checkpoint every K
+steps, recomputation inside each segment).  No reference counterpart.
+
+``LatticeVelocity`` is a ``torch.autograd.Function``: its forward runs the
+solver for n_steps through the C ABI and returns (u_x, u_y); its backward
+hands dL/du to the adjoint and returns dL/domega, dL/dF, dL/du_wall.  Torch is
+only the autograd and tensor plumbing here; every step and every adjoint
+step runs in the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+from .lattice import Lattice, Settings
+
+
+class Adjoint:
+    """A lattice plus its adjoint workspace for a fixed number of steps."""
+
+    def __init__(self, settings: Settings, n_steps: int, checkpoint_interval: int = 100):
+        self.lattice = Lattice(settings)
+        self.settings = settings
+        self.n_steps = int(n_steps)
+        self.nx, self.ny = int(settings.N_X), int(settings.N_Y)
+        self.dtype = self.lattice.dtype
+        h = ctypes.c_void_p()
+        check(lib().ffb_adjoint_create(self.lattice.handle, self.n_steps, int(checkpoint_interval),
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ffb_adjoint_destroy(self._h)
+        self._h = None
+        if getattr(self, "lattice", None) is not None:
+            self.lattice.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_parameters(self, omega=None, body_force=None, u_wall=None):
+        s = self.settings
+        if omega is not None:
+            s.OMEGA = float(omega)
+        if body_force is not None:
+            s.BODY_FORCE = float(body_force)
+        if u_wall is not None:
+            s.U_WALL = float(u_wall)
+            s.U_LID = float(u_wall)
+        check(lib().ffb_set_parameters(self.lattice.handle, s.OMEGA, s.BODY_FORCE, s.U_WALL, s.U_LID))
+
+    def forward(self):
+        """Runs the n_steps steps; returns (rho, u_x, u_y) of the last step (numpy)."""
+        rho = np.empty((self.ny, self.nx), self.dtype)
+        ux, uy = np.empty_like(rho), np.empty_like(rho)
+        check(lib().ffb_adjoint_forward(self._h, ctypes.c_void_p(rho.ctypes.data),
+                                        ctypes.c_void_p(ux.ctypes.data), ctypes.c_void_p(uy.ctypes.data)))
+        return rho, ux, uy
+
+    def backward(self, grad_u_x: np.ndarray, grad_u_y: np.ndarray):
+        """dL/domega, dL/d(body force), dL/d(wall speed) for the seed dL/du."""
+        gx = np.ascontiguousarray(grad_u_x, dtype=self.dtype)
+        gy = np.ascontiguousarray(grad_u_y, dtype=self.dtype)
+        d = [ctypes.c_double() for _ in range(3)]
+        check(lib().ffb_adjoint_backward(self._h, ctypes.c_void_p(gx.ctypes.data),
+                                         ctypes.c_void_p(gy.ctypes.data), *[ctypes.byref(x) for x in d]))
+        return d[0].value, d[1].value, d[2].value
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().ffb_adjoint_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+
+def velocity_loss(ux, uy, tx, ty) -> float:
+    """L = sum over cells of |u - u_target|^2 (FP64 sum)."""
+    return float(np.sum((ux.astype(np.float64) - tx) ** 2 + (uy.astype(np.float64) - ty) ** 2))
+
+
+def loss_and_gradient(adj: Adjoint, tx, ty):
+    """Loss ||u - u_target||^2 and its gradient (d omega, d F, d u_wall)."""
+    _, ux, uy = adj.forward()
+    loss = velocity_loss(ux, uy, tx, ty)
+    g = adj.backward(2.0 * (ux.astype(np.float64) - tx), 2.0 * (uy.astype(np.float64) - ty))
+    return loss, g
+
+
+def finite_difference(adj: Adjoint, tx, ty, which: str, rel_h: float = 1e-4) -> float:
+    """Central difference of the loss in one parameter (h = rel_h * |value|)."""
+    s = adj.settings
+    base = {"omega": s.OMEGA, "body_force": s.BODY_FORCE, "u_wall": s.U_WALL}[which]
+    h = rel_h * abs(base)
+    vals = []
+    for sign in (1, -1):
+        adj.set_parameters(**{which: base + sign * h})
+        _, ux, uy = adj.forward()
+        vals.append(velocity_loss(ux, uy, tx, ty))
+    adj.set_parameters(**{which: base})
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+try:  # torch is plumbing only; the Function exists where torch is importable
+    import torch
+
+    class LatticeVelocity(torch.autograd.Function):
+        """(omega, body_force, u_wall) -> (u_x, u_y) after adj.n_steps steps."""
+
+        @staticmethod
+        def forward(ctx, omega, body_force, u_wall, adj: Adjoint):
+            adj.set_parameters(float(omega), float(body_force), float(u_wall))
+            _, ux, uy = adj.forward()
+            ctx.adj = adj
+            dev = omega.device
+            return (torch.from_numpy(ux.astype(np.float64)).to(dev),
+                    torch.from_numpy(uy.astype(np.float64)).to(dev))
+
+        @staticmethod
+        def backward(ctx, grad_ux, grad_uy):
+            adj = ctx.adj
+            d_om, d_f, d_w = adj.backward(grad_ux.detach().cpu().numpy(), grad_uy.detach().cpu().numpy())
+            dev = grad_ux.device
+            t = lambda v: torch.tensor(v, dtype=torch.float64, device=dev)  # noqa: E731
+            return t(d_om), t(d_f), t(d_w), None
+except ImportError:  # pragma: no cover
+    torch = None
+    LatticeVelocity = None
+
+
+def channel_settings(n: int, omega: float, force: float, precision: str) -> Settings:
+    return Settings(N_X=n, N_Y=n, SIM_MODE=5, OMEGA=omega, BODY_FORCE=force, precision=precision)
+
+
+def inverse_channel(n: int = 512, steps: int = 5000, interval: int = 100, seed: int = 0,
+                    iters: int = 50, lr: float = 0.05, precision: str = "fp32", log=print) -> dict:
+    """Inverse body-force channel: the target u field is the forward solution at
+    seeded true (omega, g) with omega in [0.8, 1.8], g in [1e-7, 1e-5]; Adam on
+    (omega, g / 1e-6) from a perturbed start recovers them."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    om_true = float(rng.uniform(0.8, 1.8))
+    g_true = float(10 ** rng.uniform(-7, -5))
+    with Adjoint(channel_settings(n, om_true, g_true, precision), steps, interval) as adj:
+        _, tx, ty = adj.forward()
+        tx, ty = tx.astype(np.float64), ty.astype(np.float64)
+        om0, g0 = om_true * 0.85 + 0.1, g_true * 1.6
+        theta = torch.tensor([om0, g0 / 1e-6], dtype=torch.float64, requires_grad=True)
+        opt = torch.optim.Adam([theta], lr=lr)
+        hist = []
+        for it in range(iters):
+            opt.zero_grad()
+            ux, uy = LatticeVelocity.apply(theta[0], theta[1] * 1e-6, torch.tensor(0.0, dtype=torch.float64), adj)
+            loss = torch.sum((ux - torch.from_numpy(tx)) ** 2 + (uy - torch.from_numpy(ty)) ** 2)
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                theta[0].clamp_(0.55, 1.95)
+                theta[1].clamp_(1e-3, 100.0)
+            hist.append((float(loss), float(theta[0]), float(theta[1]) * 1e-6))
+            if it % 10 == 0 or it == iters - 1:
+                log(f"adam {it}: loss {hist[-1][0]:.3e} omega {hist[-1][1]:.6f} g {hist[-1][2]:.4e}")
+    return {"omega_true": om_true, "g_true": g_true, "omega_start": om0, "g_start": g0,
+            "omega_final": hist[-1][1], "g_final": hist[-1][2], "loss_start": hist[0][0],
+            "loss_final": hist[-1][0], "history": hist}

@@ -17,7 +17,7 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libffb_hip.so")
 
-SOURCES = ["ffb_api.hip", "ffb_batch.hip"]
+SOURCES = ["ffb_api.hip", "ffb_batch.hip", "ffb_adjoint.hip"]
 HEADERS = ["d2q9_kernels.h", "ffb_internal.h"]
 
 HIPCC_FLAGS = [

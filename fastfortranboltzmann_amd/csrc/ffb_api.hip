@@ -32,35 +32,10 @@ int fail(int code, const std::string &msg) {
             return ffb::fail(FFB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
     } while (0)
 
-struct ffb_lattice {
-    ffb_config_t cfg;
-    int pitch = 0;           // elements per row (n_x rounded up to 64)
-    long long plane = 0;     // elements per population plane
-    size_t elem = 0;         // bytes per stored population
-    size_t celem = 0;        // bytes per compute value (macro fields, strips)
-    void *pop_alloc = nullptr;  // one allocation holding both population sets
-    void *buf[2] = {nullptr, nullptr};
-    int cur = 0;             // buffer the next step reads
-    void *macro[3] = {nullptr, nullptr, nullptr};
-    void *strips[2] = {nullptr, nullptr};  // Poiseuille strips, [side][3][ny] each
-    int strip_old = 0;
-    void *ux_row = nullptr;  // shear-wave initial profile
-    double *mom_partial = nullptr;
-    double *mom_out = nullptr;
-    int mom_blocks = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    bool timing = false;
-    long long steps = 0;
-    int tile_w = 512;        // threads per workgroup
-    int vec = 0;             // 1: V = 16 bytes / element cells per thread; 0: one cell per thread
-    int xcd_order = 0;       // 1: XCD-banded tile order (measured 10 % slower, kept as an option)
-    long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
-    long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
-};
+
 
 // cells per thread of the step kernel
-static int cells_per_thread(const ffb_lattice *L) { return L->vec ? (int)(16 / L->elem) : 1; }
+static int cells_per_thread(const ffb_lattice *L) { return L->vec > 1 ? L->vec : 1; }
 
 namespace {
 
@@ -131,14 +106,25 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     return a;
 }
 
+template <typename S, typename C, int MODE, int TW, int V>
+void launch_v(ffb_lattice *L, const StepArgs<C> &a) {
+    hipLaunchKernelGGL((step_kernel<S, C, MODE, TW, V>), dim3((unsigned)a.n_blocks), dim3(TW), 0, L->stream, a);
+}
+
 template <typename S, typename C, int MODE, int TW>
 int launch_step_tw(ffb_lattice *L, const StepArgs<C> &a) {
-    constexpr int VEC = 16 / (int)sizeof(typename Store<S>::R);  // cells per thread on the vector path
-    const dim3 grid((unsigned)a.n_blocks);
-    if (cells_per_thread(L) > 1)
-        hipLaunchKernelGGL((step_kernel<S, C, MODE, TW, VEC>), grid, dim3(TW), 0, L->stream, a);
-    else
-        hipLaunchKernelGGL((step_kernel<S, C, MODE, TW, 1>), grid, dim3(TW), 0, L->stream, a);
+    // cells per thread: 1, or 2/4/8 with one aligned vector load per plane
+    // (at most 16 bytes); wider tiles only for the 256- and 512-thread shapes
+    constexpr int RB = (int)sizeof(typename Store<S>::R);
+    const int v = cells_per_thread(L);
+    if constexpr (TW >= 256) {
+        if (v == 2 && 2 * RB <= 16) { launch_v<S, C, MODE, TW, 2>(L, a); HIPCHK(hipGetLastError()); return FFB_OK; }
+        if constexpr (4 * RB <= 16)
+            if (v == 4) { launch_v<S, C, MODE, TW, 4>(L, a); HIPCHK(hipGetLastError()); return FFB_OK; }
+        if constexpr (8 * RB <= 16)
+            if (v == 8) { launch_v<S, C, MODE, TW, 8>(L, a); HIPCHK(hipGetLastError()); return FFB_OK; }
+    }
+    launch_v<S, C, MODE, TW, 1>(L, a);
     HIPCHK(hipGetLastError());
     return FFB_OK;
 }
@@ -166,6 +152,22 @@ template <typename S, typename C> int launch_step(ffb_lattice *L, int write_macr
         return launch_step_mode<S, C, PF>(L, a);
     }
     default: return launch_step_mode<S, C, SL>(L, a);
+    }
+}
+
+// One step from src to dst (not the handle's ping-pong buffers), any mode but
+// Poiseuille flow (whose strips live in the handle).
+template <typename S, typename C>
+int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) {
+    StepArgs<C> a = make_args<C>(L, write_macro);
+    a.src = src;
+    a.dst = dst;
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: return launch_step_mode<S, C, SW>(L, a);
+    case FFB_COUETTE_FLOW: return launch_step_mode<S, C, CF>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_step_mode<S, C, BF>(L, a);
+    case FFB_SLIDING_LID: return launch_step_mode<S, C, SL>(L, a);
+    default: return fail(FFB_E_INVALID, "step between buffers: Poiseuille flow is not supported");
     }
 }
 
@@ -293,6 +295,16 @@ float weightf(int d) { return d == 0 ? 4.0f / 9.0f : (d <= 4 ? 1.0f / 9.0f : 1.0
 
 }  // namespace
 
+namespace ffb {
+int step_between(ffb_lattice *L, const void *src, void *dst, int write_macro) {
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: return launch_between<double, double>(L, src, dst, write_macro);
+    case FFB_PREC_FP32: return launch_between<float, float>(L, src, dst, write_macro);
+    default: return fail(FFB_E_INVALID, "step between buffers: FP32 or FP64 storage only");
+    }
+}
+}  // namespace ffb
+
 extern "C" {
 
 void ffb_config_default(ffb_config_t *cfg) {
@@ -338,12 +350,16 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     ffb_lattice *L = new ffb_lattice();
     L->cfg = *cfg;
     L->pitch = ((cfg->n_x + 63) / 64) * 64;
-    // Tuning parameters of the step kernel.  Defaults measured on MI355X
-    // (4096^2 cavity): one cell per thread, 512-thread tiles, dispatch-order
-    // tiles (round-robin over the XCDs keeps all of them on neighbouring rows:
-    // 202 us per step against 222 us with XCD-banded tiles); the 16-byte vector
-    // path (FFB_VEC=1) measured slower.
-    // FFB_TILE_W, FFB_VEC, FFB_XCD_ORDER, FFB_PLANE_PAD, FFB_BUF_PAD override.
+    // Tuning parameters of the step kernel, defaults measured on MI355X:
+    //   FP32 (4096^2 cavity): one cell per thread, 512-thread tiles (206 us;
+    //     2 or 4 cells per thread 234-250 us);
+    //   FP64: one cell per thread, 256-thread tiles (426 us against 464 us);
+    //   16-bit forms (8192^2): 4 cells per thread in 8-byte vectors, 512-thread
+    //     tiles (487 us against 498-563 us with one cell per thread);
+    //   dispatch-order tiles: round-robin over the XCDs keeps all of them on
+    //     neighbouring rows (202 us against 222 us with XCD-banded tiles).
+    // FFB_TILE_W, FFB_VEC (cells per thread), FFB_XCD_ORDER, FFB_PLANE_PAD,
+    // FFB_BUF_PAD override them.
     auto env_ll = [](const char *name, long long dflt) {
         const char *v = std::getenv(name);
         return v ? std::atoll(v) : dflt;
@@ -351,12 +367,17 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->plane_pad = ((env_ll("FFB_PLANE_PAD", 0) + 63) / 64) * 64;
     L->buf_pad = ((env_ll("FFB_BUF_PAD", 0) + 63) / 64) * 64;
     L->plane = (long long)L->pitch * cfg->n_y + L->plane_pad;
-    L->vec = (int)env_ll("FFB_VEC", 0);
     L->elem = cfg->precision == FFB_PREC_FP64 ? 8 : (cfg->precision == FFB_PREC_FP32 ? 4 : 2);
     L->celem = cfg->precision == FFB_PREC_FP64 ? 8 : 4;
-    L->tile_w = (int)env_ll("FFB_TILE_W", 512);
+    const bool half = L->elem == 2;
+    L->vec = (int)env_ll("FFB_VEC", half ? 4 : 1);
+    L->tile_w = (int)env_ll("FFB_TILE_W", cfg->precision == FFB_PREC_FP64 ? 256 : 512);
     L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 0);
     if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512) L->tile_w = 256;
+    // cells per thread must be 1, 2, 4 or 8 with at most 16 bytes per vector and
+    // a 256- or 512-thread tile; anything else falls back to one cell per thread
+    if (!(L->vec == 2 || L->vec == 4 || L->vec == 8) || (size_t)L->vec * L->elem > 16 || L->tile_w < 256)
+        L->vec = 1;
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);
@@ -438,6 +459,15 @@ int ffb_step(ffb_lattice_t *L, int32_t n_steps, int32_t write_macro_last) {
         ++L->steps;
     }
     HIPCHK(hipEventRecord(L->ev_stop, L->stream));
+    return FFB_OK;
+}
+
+int ffb_set_parameters(ffb_lattice_t *L, double omega, double body_force, double u_wall, double u_lid) {
+    if (!L) return fail(FFB_E_INVALID, "ffb_set_parameters: null handle");
+    L->cfg.omega = omega;
+    L->cfg.body_force = body_force;
+    L->cfg.u_wall = u_wall;
+    L->cfg.u_lid = u_lid;
     return FFB_OK;
 }
 

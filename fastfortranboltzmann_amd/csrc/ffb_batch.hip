@@ -8,8 +8,11 @@
 // Layout: every lattice keeps the single-domain layout (9 SoA planes, row
 // pitch rounded to 64 elements, no halo) inside two pools (ping-pong); a
 // descriptor table holds each lattice's offsets and parameters.  Work is cut
-// into tiles of TW cells in row-major order within a lattice (a tile may
-// cover several short rows); tile_lat[t] names the lattice of tile t.
+// into row segments of 64 cells (one wave each, so every load of a wave is
+// one aligned stretch of one row); tile_lat[t] names the lattice of segment t,
+// and the segments are ordered by boundary kind so that the waves resident
+// at one time run the same code.  A workgroup of TW threads takes TW/64
+// consecutive segments per iteration of its persistent loop.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +42,8 @@ template <typename C> struct BatchLat {
     long long macro_off;  // element offset in the macro arrays
     long long prof_off;   // element offset of the shear-wave profile row table
     int nx, ny, pitch, kind;
-    int tile_start, n_cells;
+    int tile_start;       // first row segment
+    int segs_per_row;     // ceil(nx / 64)
     C omega, corr8, corr9, force, rho_0;
 };
 
@@ -47,8 +51,8 @@ template <typename C> struct BatchArgs {
     const void *src_pool;
     void *dst_pool;
     const BatchLat<C> *lats;
-    const int *tile_lat;
-    int n_tiles;
+    const int *tile_lat;  // lattice of each row segment
+    int n_tiles;          // row segments
     int write_macro;
     C *rho_o, *ux_o, *uy_o;
     const C *prof;        // shear-wave u_x(y) rows of all SW lattices
@@ -87,15 +91,18 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     return a;
 }
 
-// One step of every lattice: a persistent grid walks the tiles.
+// One step of every lattice: a persistent grid walks the row segments, one
+// wave per segment.
 template <typename C, int TW>
 __global__ void __launch_bounds__(TW) batch_step_kernel(BatchArgs<C> b) {
-    for (int t = (int)blockIdx.x; t < b.n_tiles; t += (int)gridDim.x) {
+    constexpr int WAVES = TW / 64;
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    for (int t = (int)blockIdx.x * WAVES + wave; t < b.n_tiles; t += (int)gridDim.x * WAVES) {
         const BatchLat<C> L = b.lats[b.tile_lat[t]];
-        const int q = (t - L.tile_start) * TW + (int)threadIdx.x;
-        if (q >= L.n_cells) continue;
-        const int y = q / L.nx;
-        const int x = q - y * L.nx;
+        const int local = t - L.tile_start;
+        const int y = local / L.segs_per_row;
+        const int x = (local - y * L.segs_per_row) * 64 + lane;
+        if (x >= L.nx) continue;
         const StepArgs<C> a = lattice_args<C>(b, L);
         const C *f = static_cast<const C *>(a.src);
         C *g = static_cast<C *>(a.dst);
@@ -112,12 +119,15 @@ __global__ void __launch_bounds__(TW) batch_step_kernel(BatchArgs<C> b) {
 // lattices start from their u_x(y) profile, the others at rest).
 template <typename C, int TW>
 __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
-    for (int t = (int)blockIdx.x; t < b.n_tiles; t += (int)gridDim.x) {
+    constexpr int WAVES = TW / 64;
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    for (int t = (int)blockIdx.x * WAVES + wave; t < b.n_tiles; t += (int)gridDim.x * WAVES) {
         const BatchLat<C> L = b.lats[b.tile_lat[t]];
-        const int q = (t - L.tile_start) * TW + (int)threadIdx.x;
-        if (q >= L.n_cells) continue;
-        const int y = q / L.nx;
-        const int x = q - y * L.nx;
+        const int local = t - L.tile_start;
+        const int y = local / L.segs_per_row;
+        const int x = (local - y * L.segs_per_row) * 64 + lane;
+        if (x >= L.nx) continue;
+        const int q = y * L.nx + x;
         C *g = static_cast<C *>(b.dst_pool) + L.off;
         const size_t c = (size_t)y * (size_t)L.pitch + (size_t)x;
         const bool sw = L.kind == SW;
@@ -170,18 +180,25 @@ struct ffb_batch {
 
 namespace {
 
+// workgroups of the launch: one per TW/64 segments, or the persistent grid
 int grid_for(const ffb_batch *B) {
+    const int waves = B->tuning.tile_w / 64;
+    const int all = (B->n_tiles + waves - 1) / waves;
     const int g = B->tuning.grid_blocks;
-    return (g <= 0 || g > B->n_tiles) ? B->n_tiles : g;
+    return (g <= 0 || g > all) ? all : g;
 }
 
 // (Re)builds the descriptor and tile tables for the current tile width.
 template <typename C> int upload_tables(ffb_batch *B) {
-    const int TW = B->tuning.tile_w;
     const int n = (int)B->desc.size();
     std::vector<BatchLat<C>> lats((size_t)n);
     std::vector<int> tiles;
-    for (int i = 0; i < n; ++i) {
+    // lattices grouped by kind (stable), so resident waves share their code path
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return B->desc[(size_t)a].kind < B->desc[(size_t)b].kind; });
+    for (int i : order) {
         const ffb_batch_lattice_t &d = B->desc[(size_t)i];
         BatchLat<C> &L = lats[(size_t)i];
         L.off = B->off[(size_t)i];
@@ -192,10 +209,9 @@ template <typename C> int upload_tables(ffb_batch *B) {
         L.ny = d.n_y;
         L.pitch = B->pitch[(size_t)i];
         L.kind = d.kind;
-        L.n_cells = d.n_x * d.n_y;
+        L.segs_per_row = (d.n_x + 63) / 64;
         L.tile_start = (int)tiles.size();
-        const int nt = (L.n_cells + TW - 1) / TW;
-        tiles.insert(tiles.end(), (size_t)nt, i);
+        tiles.insert(tiles.end(), (size_t)L.segs_per_row * (size_t)d.n_y, i);
         L.omega = (C)d.omega;
         const C w6 = (C)1 / (C)36;
         L.corr8 = (C)6 * w6 * (C)d.rho_0 * (C)d.u_wall;  // as couette_flow.f90:118-119
@@ -237,6 +253,7 @@ template <typename C> void launch_step_c(const ffb_batch *B, int parity, int wri
     const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
     const dim3 grid((unsigned)grid_for(B));
     switch (B->tuning.tile_w) {
+    case 64: hipLaunchKernelGGL((batch_step_kernel<C, 64>), grid, dim3(64), 0, B->stream, b); break;
     case 128: hipLaunchKernelGGL((batch_step_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
     case 512: hipLaunchKernelGGL((batch_step_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
     case 1024: hipLaunchKernelGGL((batch_step_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
@@ -273,8 +290,9 @@ int build_graph(ffb_batch *B) {
 
 template <typename C> int init_c(ffb_batch *B) {
     BatchArgs<C> b = batch_args<C>(B, 1, 0);  // dst = pool[0]
-    const dim3 grid((unsigned)B->n_tiles);
+    const dim3 grid((unsigned)((B->n_tiles + B->tuning.tile_w / 64 - 1) / (B->tuning.tile_w / 64)));
     switch (B->tuning.tile_w) {
+    case 64: hipLaunchKernelGGL((batch_init_kernel<C, 64>), grid, dim3(64), 0, B->stream, b); break;
     case 128: hipLaunchKernelGGL((batch_init_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
     case 512: hipLaunchKernelGGL((batch_init_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
     case 1024: hipLaunchKernelGGL((batch_init_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
@@ -286,8 +304,8 @@ template <typename C> int init_c(ffb_batch *B) {
 }
 
 int apply_tuning(ffb_batch *B, const ffb_batch_tuning_t &t) {
-    if (t.tile_w != 128 && t.tile_w != 256 && t.tile_w != 512 && t.tile_w != 1024)
-        return fail(FFB_E_INVALID, "ffb_batch: tile_w must be 128, 256, 512 or 1024");
+    if (t.tile_w != 64 && t.tile_w != 128 && t.tile_w != 256 && t.tile_w != 512 && t.tile_w != 1024)
+        return fail(FFB_E_INVALID, "ffb_batch: tile_w must be 64, 128, 256, 512 or 1024");
     if (t.grid_blocks < 0) return fail(FFB_E_INVALID, "ffb_batch: grid_blocks must be >= 0");
     BHIPCHK(hipStreamSynchronize(B->stream));
     B->tuning = t;
@@ -505,7 +523,7 @@ int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *
     BHIPCHK(hipSetDevice(B->device));
     int cus = 0;
     BHIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device));
-    const int tws[] = {128, 256, 512, 1024};
+    const int tws[] = {64, 128, 256, 512, 1024};
     const int per_cu[] = {0, 2, 4, 8, 16};  // 0: one tile per workgroup (no persistence)
     ffb_batch_tuning_t best{256, 0};
     double best_t = 1e30;

@@ -1,10 +1,50 @@
-// ffb_internal.h -- helpers shared by the library's translation units.
+// ffb_internal.h -- state and helpers shared by the library's translation units.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
 #include <string>
+
+#include "ffb.h"
 
 namespace ffb {
 // Records msg as the thread's last error (returned by ffb_last_error) and
 // returns code.
 int fail(int code, const std::string &msg);
+}  // namespace ffb
+
+// One device-resident domain (the handle behind ffb_lattice_t).
+struct ffb_lattice {
+    ffb_config_t cfg;
+    int pitch = 0;           // elements per row (n_x rounded up to 64)
+    long long plane = 0;     // elements per population plane
+    size_t elem = 0;         // bytes per stored population
+    size_t celem = 0;        // bytes per compute value (macro fields, strips)
+    void *pop_alloc = nullptr;  // one allocation holding both population sets
+    void *buf[2] = {nullptr, nullptr};
+    int cur = 0;             // buffer the next step reads
+    void *macro[3] = {nullptr, nullptr, nullptr};
+    void *strips[2] = {nullptr, nullptr};  // Poiseuille strips, [side][3][ny] each
+    int strip_old = 0;
+    void *ux_row = nullptr;  // shear-wave initial profile
+    double *mom_partial = nullptr;
+    double *mom_out = nullptr;
+    int mom_blocks = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timing = false;
+    long long steps = 0;
+    int tile_w = 512;        // threads per workgroup
+    int vec = 1;             // cells per thread of the step kernel (1, 2, 4, 8)
+    int xcd_order = 0;       // 1: XCD-banded tile order (measured 10 % slower, kept as an option)
+    long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
+    long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
+};
+
+namespace ffb {
+// Enqueues one step of L's configuration reading populations at src and
+// writing dst (both in L's layout), on L's stream; Poiseuille flow and the
+// 16-bit forms are not supported.
+int step_between(ffb_lattice *L, const void *src, void *dst, int write_macro);
 }  // namespace ffb

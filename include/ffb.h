@@ -107,6 +107,10 @@ int ffb_init(ffb_lattice_t *lat);
  * writes rho, u_x, u_y (the reference's write_macro_fields). */
 int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
 
+/* Changes the physical parameters of an existing handle (the next steps use
+ * them): omega, body-force density, Couette wall speed, lid speed. */
+int ffb_set_parameters(ffb_lattice_t *lat, double omega, double body_force, double u_wall, double u_lid);
+
 /* Waits for all work of the handle. */
 int ffb_sync(ffb_lattice_t *lat);
 
@@ -174,7 +178,7 @@ typedef struct ffb_batch_lattice {
 } ffb_batch_lattice_t;
 
 typedef struct ffb_batch_tuning {
-    int32_t tile_w;       /* threads per workgroup (cells per tile): 128, 256, 512, 1024 */
+    int32_t tile_w;       /* threads per workgroup (64-cell row segments x 64): 64 .. 1024 */
     int32_t grid_blocks;  /* persistent grid size; 0 = one workgroup per tile */
 } ffb_batch_tuning_t;
 
@@ -202,6 +206,30 @@ int ffb_batch_autotune(ffb_batch_t *batch, int32_t trial_steps, ffb_batch_tuning
 int ffb_batch_read_macros(ffb_batch_t *batch, int32_t index, void *rho, void *u_x, void *u_y);
 /* Populations of lattice `index` as f(0:NX+1, 0:NY+1, 1:9) (halo zero). */
 int ffb_batch_read_populations(ffb_batch_t *batch, int32_t index, void *f_halo);
+
+/* ------------------------------------------------------------------------
+ * Differentiation form (no reference counterpart): discrete adjoint of n_steps
+ * steps of a handle (shear wave, Couette, sliding lid or body-force channel;
+ * FP32 or FP64), with a checkpoint every checkpoint_interval steps and
+ * recomputation of each segment during the reverse sweep.  The loss is any
+ * function of the velocity field u written by the last step; the caller
+ * passes dL/du and gets dL/domega, dL/d(body force) and dL/d(wall or lid
+ * speed).  Parameters are those of the handle (ffb_set_parameters).
+ * ------------------------------------------------------------------------ */
+typedef struct ffb_adjoint ffb_adjoint_t;
+
+int ffb_adjoint_create(ffb_lattice_t *lat, int32_t n_steps, int32_t checkpoint_interval, ffb_adjoint_t **out);
+int ffb_adjoint_destroy(ffb_adjoint_t *adj);
+/* ffb_init + n_steps steps, storing the checkpoints; copies the final macro
+ * fields (n_x*n_y each, element type of the handle; host or device pointers,
+ * any may be NULL). */
+int ffb_adjoint_forward(ffb_adjoint_t *adj, void *rho, void *u_x, void *u_y);
+/* Reverse sweep for the seed dL/du_x, dL/du_y (host or device arrays of
+ * n_x*n_y); the three gradients are returned as FP64 sums. */
+int ffb_adjoint_backward(ffb_adjoint_t *adj, const void *grad_u_x, const void *grad_u_y, double *d_omega,
+                         double *d_body_force, double *d_u_wall);
+/* Device time of the last forward and backward calls, in milliseconds. */
+int ffb_adjoint_timing(ffb_adjoint_t *adj, double *forward_ms, double *backward_ms);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *ffb_last_error(void);

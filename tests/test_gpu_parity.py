@@ -63,7 +63,7 @@ SHAPES = [(1, 1), (1, 5), (5, 1), (2, 2), (2, 7), (7, 2), (3, 3), (8, 3), (9, 4)
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
-@pytest.mark.parametrize("vec", [1, 0], ids=["vec", "scalar"])
+@pytest.mark.parametrize("vec", [4, 2, 1], ids=["vec4", "vec2", "scalar"])
 def test_gpu_matches_oracle_small_shapes(shape, mode, precision, vec, monkeypatch):
     monkeypatch.setenv("FFB_VEC", str(vec))
     nx, ny = shape

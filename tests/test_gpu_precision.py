@@ -20,9 +20,11 @@ SHAPES = [(13, 9), (65, 17), (300, 7), (1030, 5)]
 @pytest.mark.parametrize("storage,scale", [("fp16dev", 16), ("fp16dev", 11), ("bf16dev", 0)])
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
-def test_16bit_storage_matches_oracle_emulation(shape, mode, storage, scale):
+@pytest.mark.parametrize("vec", [1, 2, 8])
+def test_16bit_storage_matches_oracle_emulation(shape, mode, storage, scale, vec, monkeypatch):
     from fastfortranboltzmann_amd import Lattice, Settings
 
+    monkeypatch.setenv("FFB_VEC", str(vec))
     nx, ny = shape
     print(f"[gpu] storage {storage} s={scale} {nx}x{ny} mode {mode}", flush=True)
     p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002, rho_out=0.998)
