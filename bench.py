@@ -29,7 +29,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2}
+BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2, "fp32dev": 4}
 
 
 def cavity_omega(re: float = 1000.0, u: float = 0.1, n: int = 4096) -> float:
@@ -67,25 +67,28 @@ def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0
 
 
 def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000) -> dict:
-    """16-bit deviation storage: MLUPS at n^2 (cavity, Re 5000) against FP32 at
-    the same size, and the relative L2 velocity error against FP64 on a
-    smaller cavity run (err_n^2, Re 1000, err_steps steps)."""
+    """Deviation storage forms: MLUPS at n^2 (cavity, Re 5000) against FP32 at
+    the same size, and the relative L2 velocity error and mass drift against
+    FP64 on a smaller cavity run (err_n^2, Re 1000, err_steps steps), for the
+    16-bit forms, FP32 deviations and plain FP32."""
     from fastfortranboltzmann_amd.precision import (cavity_settings, compare, default_scale_log2,
                                                     run_form)
 
-    out = {"size": n, "steps": steps}
+    forms = ("fp32", "fp32dev", "fp16dev", "bf16dev")
+    out = {"size": n, "steps": steps, "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
     rates = {}
-    for form in ("fp32", "fp16dev", "bf16dev"):
+    for form in forms:
         s = default_scale_log2(form) if form.endswith("dev") else 0
         rates[form] = run_form(cavity_settings(n, 5000.0, 0.1, form, s), steps, chunk=steps)["mlups"]
     ref = run_form(cavity_settings(err_n, 1000.0, 0.1, "fp64"), err_steps, chunk=err_steps)
-    for form in ("fp16dev", "bf16dev"):
-        s = default_scale_log2(form)
+    for form in forms:
+        s = default_scale_log2(form) if form.endswith("dev") else 0
         r = run_form(cavity_settings(err_n, 1000.0, 0.1, form, s), err_steps, chunk=err_steps)
         out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
-                     "achieved_gbs": round(rates[form] * 1e6 * 36 / 1e9, 1),
+                     "achieved_gbs": round(rates[form] * 1e6 * 18 * BYTES_PER_POP[form] / 1e9, 1),
                      "scale_log2": s, "rel_l2_u_vs_fp64": compare(ref, r)["rel_l2_u"],
-                     "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
+                     "mass_drift": r["mass_drift"]}
+    out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
     return out
 
@@ -244,7 +247,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "fp64": "f64", "fp16dev": "f16-storage/f32", "bf16dev": "bf16-storage/f32"}[args.precision],
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp16dev": "f16-storage/f32", "bf16dev": "bf16-storage/f32",
+                  "fp32dev": "f32-deviation"}[args.precision],
         "data": "synthetic (fluid at rest, deterministic initial state)",
         "config": {"workload": f"lid-driven cavity {n}x{n}, bounce-back walls + moving lid u=0.1, "
                                f"Re=1000 (omega={cavity_omega(n=n):.6f}), {args.precision} populations",

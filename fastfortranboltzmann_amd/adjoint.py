@@ -174,3 +174,61 @@ def inverse_channel(n: int = 512, steps: int = 5000, interval: int = 100, seed: 
     return {"omega_true": om_true, "g_true": g_true, "omega_start": om0, "g_start": g0,
             "omega_final": hist[-1][1], "g_final": hist[-1][2], "loss_start": hist[0][0],
             "loss_final": hist[-1][0], "history": hist}
+
+
+def gradient_check(n: int, steps: int, interval: int, omega: float, force: float,
+                   t_omega: float, t_force: float, rel_h: float = 1e-4) -> dict:
+    """Adjoint gradients of ||u - u_target||^2 in FP32 and FP64 against central
+    differences of the FP64 forward solver (h = rel_h relative)."""
+    out = {}
+    with Adjoint(channel_settings(n, t_omega, t_force, "fp64"), steps, interval) as a:
+        _, tx, ty = a.forward()
+    for prec in ("fp64", "fp32"):
+        with Adjoint(channel_settings(n, omega, force, prec), steps, interval) as a:
+            loss, g = loss_and_gradient(a, tx, ty)
+            out[prec] = {"loss": loss, "d_omega": g[0], "d_force": g[1]}
+            if prec == "fp64":
+                out["fd"] = {"d_omega": finite_difference(a, tx, ty, "omega", rel_h),
+                             "d_force": finite_difference(a, tx, ty, "body_force", rel_h)}
+    for k in ("d_omega", "d_force"):
+        fd = out["fd"][k]
+        out[f"rel_err_fp64_vs_fd_{k}"] = abs(out["fp64"][k] - fd) / abs(fd)
+        out[f"rel_err_fp32_vs_fp64_{k}"] = abs(out["fp32"][k] - out["fp64"][k]) / abs(out["fp64"][k])
+    return out
+
+
+def main(argv=None) -> int:
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="inverse body-force channel: gradient check and Adam recovery")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--interval", type=int, default=100)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--lr", type=float, default=0.05)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    t_om, t_g = float(rng.uniform(0.8, 1.8)), float(10 ** rng.uniform(-7, -5))
+    log(f"gradient check at ({t_om * 0.85 + 0.1:.4f}, {t_g * 1.6:.3e}) against target ({t_om:.4f}, {t_g:.3e})")
+    gc = gradient_check(args.n, args.steps, args.interval, t_om * 0.85 + 0.1, t_g * 1.6, t_om, t_g)
+    log(json.dumps(gc))
+    inv = inverse_channel(args.n, args.steps, args.interval, args.seed, args.iters, args.lr, "fp32", log)
+    inv.pop("history")
+    with Adjoint(channel_settings(args.n, t_om, t_g, "fp32"), args.steps, args.interval) as a:
+        for _ in range(2):
+            _, ux, uy = a.forward()
+            a.backward(np.ones_like(ux), np.zeros_like(uy))
+        f_ms, b_ms = a.timing()
+    print(json.dumps({"gradient_check": gc, "inverse": inv,
+                      "cost": {"forward_ms": f_ms, "backward_ms": b_ms, "ratio": b_ms / f_ms}}))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, PREC_FP32, PREC_FP64, SHEAR_WAVE, SLIDING_LID,
+from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, PREC_FP32, PREC_FP32_DEV, PREC_FP64, SHEAR_WAVE, SLIDING_LID,
                    FfbBatchLattice, FfbBatchTuning, check, lib)
 
 
@@ -60,15 +60,14 @@ def mixed_batch(seed: int = 0, count: int = 512, side=(64, 1024), omega=(0.6, 1.
 
 class Batch:
     def __init__(self, lattices, precision: str = "fp32", device: int = 0):
-        if precision not in ("fp32", "fp64"):
-            raise ValueError("the batched form stores FP32 or FP64")
+        codes = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp32dev": PREC_FP32_DEV}
+        if precision not in codes:
+            raise ValueError("the batched form stores FP32, FP64 or FP32 deviations (fp32dev)")
         self.lattices = list(lattices)
         self.dtype = np.dtype(np.float64 if precision == "fp64" else np.float32)
         arr = (FfbBatchLattice * len(self.lattices))(*[l.c() for l in self.lattices])
         h = ctypes.c_void_p()
-        check(lib().ffb_batch_create(arr, len(self.lattices),
-                                     PREC_FP64 if precision == "fp64" else PREC_FP32, device,
-                                     ctypes.byref(h)))
+        check(lib().ffb_batch_create(arr, len(self.lattices), codes[precision], device, ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -138,45 +137,54 @@ class Batch:
 
 
 def batch_study(lattices, steps: int, autotune: bool = True, trial_steps: int = 20,
-                compare_fp64: bool = True, log=print) -> dict:
-    """Aggregate MLUPS of the FP32 batch (events around `steps` steps) and the
-    per-instance relative L2 error of |u| against an FP64 batch of the same
-    lattices."""
-    res = {}
-    with Batch(lattices, "fp32") as b:
-        if autotune:
-            res["tuning"] = b.autotune(trial_steps)
-            log(f"batch: autotuned {res['tuning']}")
-        info = b.info()
-        res.update(cells=info["cells"], tiles=info["tiles"], count=len(lattices))
-        b.init()
-        b.step(2)
-        b.sync()
-        b.reset_timer()
-        t0 = time.perf_counter()
-        b.step(steps, write_macro_last=True)
-        b.sync()
-        wall = time.perf_counter() - t0
-        ms = b.elapsed_ms()
-        res["mlups"] = info["cells"] * steps / (ms * 1e-3) / 1e6
-        res["mlups_wall"] = info["cells"] * steps / wall / 1e6
-        res["steps"] = steps + 2
-        u32 = [b.macros(i)[1:] for i in range(len(lattices))] if compare_fp64 else None
+                compare_fp64: bool = True, log=print, forms=("fp32",)) -> dict:
+    """Aggregate MLUPS of the batch in each storage form of `forms` ("fp32",
+    "fp32dev"; events around `steps` steps) and the per-instance relative L2
+    error of u against an FP64 batch of the same lattices.  The first form's
+    figures are also the top-level keys."""
+    res = {"forms": {}}
+    fields = {}
+    for form in forms:
+        rec = {}
+        with Batch(lattices, form) as b:
+            if autotune:
+                rec["tuning"] = b.autotune(trial_steps)
+                log(f"batch {form}: autotuned {rec['tuning']}")
+            info = b.info()
+            res.update(cells=info["cells"], tiles=info["tiles"], count=len(lattices))
+            b.init()
+            b.step(2)
+            b.sync()
+            b.reset_timer()
+            t0 = time.perf_counter()
+            b.step(steps, write_macro_last=True)
+            b.sync()
+            wall = time.perf_counter() - t0
+            ms = b.elapsed_ms()
+            rec["mlups"] = info["cells"] * steps / (ms * 1e-3) / 1e6
+            rec["mlups_wall"] = info["cells"] * steps / wall / 1e6
+            res["steps"] = steps + 2
+            if compare_fp64:
+                fields[form] = [b.macros(i)[1:] for i in range(len(lattices))]
+        res["forms"][form] = rec
     if compare_fp64:
-        errs = []
+        errs = {form: [] for form in forms}
         with Batch(lattices, "fp64") as b64:
             b64.init()
             b64.step(steps + 2, write_macro_last=True)
             for i in range(len(lattices)):
                 _, ux, uy = b64.macros(i)
-                ux32, uy32 = u32[i]
-                num = math.sqrt(float(np.sum((ux32 - ux) ** 2 + (uy32 - uy) ** 2)))
                 den = math.sqrt(float(np.sum(ux ** 2 + uy ** 2)))
-                errs.append(num / den if den > 0 else 0.0)
-        errs = np.array(errs)
-        res["rel_l2_u_vs_fp64"] = {"max": float(errs.max()), "median": float(np.median(errs)),
-                                   "p90": float(np.quantile(errs, 0.9))}
-        res["per_instance_rel_l2_u"] = [float(e) for e in errs]
+                for form in forms:
+                    ux32, uy32 = fields[form][i]
+                    num = math.sqrt(float(np.sum((ux32 - ux) ** 2 + (uy32 - uy) ** 2)))
+                    errs[form].append(num / den if den > 0 else 0.0)
+        for form in forms:
+            e = np.array(errs[form])
+            res["forms"][form]["rel_l2_u_vs_fp64"] = {"max": float(e.max()), "median": float(np.median(e)),
+                                                      "p90": float(np.quantile(e, 0.9))}
+            res["forms"][form]["per_instance_rel_l2_u"] = [float(x) for x in e]
+    res.update({k: v for k, v in res["forms"][forms[0]].items()})
     return res
 
 
@@ -191,11 +199,15 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--trial-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--forms", default="fp32,fp32dev", help="storage forms, comma-separated")
     args = ap.parse_args(argv)
     lats = mixed_batch(args.seed, args.count)
     res = batch_study(lats, args.steps, trial_steps=args.trial_steps, compare_fp64=not args.no_fp64,
-                      log=lambda m: print(m, file=sys.stderr, flush=True))
-    res.pop("per_instance_rel_l2_u", None) if args.count > 64 else None
+                      log=lambda m: print(m, file=sys.stderr, flush=True), forms=tuple(args.forms.split(",")))
+    if args.count > 64:
+        res.pop("per_instance_rel_l2_u", None)
+        for rec in res["forms"].values():
+            rec.pop("per_instance_rel_l2_u", None)
     print(json.dumps(res))
     return 0
 

@@ -48,56 +48,78 @@ static_assert(opp_of(1) == 3 && opp_of(2) == 4 && opp_of(3) == 1 && opp_of(4) ==
 static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) == 6, "opp diag");
 
 // ---------------------------------------------------------------------------
-// storage of one population value.  FP64 / FP32 store f itself.  The 16-bit
-// forms store d = (f - w) * 2^s and convert to FP32 for the collision.
+// storage of one population value.  FP64 / FP32 store f itself and the step
+// works on f.  The 16-bit forms store D = (f - w) * 2^s; the step reads D as
+// FP32, works on D throughout (boundary rules and collision in deviation
+// space, collide_dev) and rounds the new D to 16 bits.  `in` gives the value
+// the step works on, `out` the stored form of it, `from_f` turns a population
+// f into that value (initial state).
 // ---------------------------------------------------------------------------
 template <typename S> struct Store;
 
 template <> struct Store<double> {
     using C = double;  // compute type
     using R = double;  // raw stored type
-    static __device__ __forceinline__ C in(R r, C, C) { return r; }
-    static __device__ __forceinline__ R out(C v, C, C) { return v; }
+    static constexpr bool dev = false;
+    static __device__ __forceinline__ C in(R r) { return r; }
+    static __device__ __forceinline__ R out(C v) { return v; }
+    static __device__ __forceinline__ C from_f(C f, C, C) { return f; }
 };
 template <> struct Store<float> {
     using C = float;
     using R = float;
-    static __device__ __forceinline__ C in(R r, C, C) { return r; }
-    static __device__ __forceinline__ R out(C v, C, C) { return v; }
+    static constexpr bool dev = false;
+    static __device__ __forceinline__ C in(R r) { return r; }
+    static __device__ __forceinline__ R out(C v) { return v; }
+    static __device__ __forceinline__ C from_f(C f, C, C) { return f; }
+};
+// FP32 deviations: D stored unrounded in FP32 (FFB_PREC_FP32_DEV).
+struct f32dev {};
+template <> struct Store<f32dev> {
+    using C = float;
+    using R = float;
+    static constexpr bool dev = true;
+    static __device__ __forceinline__ C in(R r) { return r; }
+    static __device__ __forceinline__ R out(C v) { return v; }
+    static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
 template <> struct Store<__half> {
     using C = float;
     using R = unsigned short;
-    static __device__ __forceinline__ C in(R r, C w, C inv_scale) {
-        return w + __half2float(__ushort_as_half(r)) * inv_scale;
+    static constexpr bool dev = true;
+    static __device__ __forceinline__ C in(R r) { return __half2float(__ushort_as_half(r)); }
+    static __device__ __forceinline__ R out(C v) {
+        // v is the FP32 result of the collision; the empty asm makes it an
+        // opaque register value so that the conversion rounds that FP32 value.
+        // Without it the compiler fuses the collision's last FMA and the
+        // conversion into v_fma_mixlo_f16, which rounds the exact result to
+        // FP16 once: a different value in about 1 of 2^14 conversions.
+        asm("" : "+v"(v));
+        return __half_as_ushort(__float2half_rn(v));  // round to nearest even
     }
-    static __device__ __forceinline__ R out(C v, C w, C scale) {
-        return __half_as_ushort(__float2half_rn((v - w) * scale));  // round to nearest even
-    }
+    static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
 template <> struct Store<__hip_bfloat16> {
     using C = float;
     using R = unsigned short;
-    static __device__ __forceinline__ C in(R r, C w, C inv_scale) {
-        return w + __uint_as_float((unsigned)r << 16) * inv_scale;
-    }
-    static __device__ __forceinline__ R out(C v, C w, C scale) {
-        unsigned u = __float_as_uint((v - w) * scale);
+    static constexpr bool dev = true;
+    static __device__ __forceinline__ C in(R r) { return __uint_as_float((unsigned)r << 16); }
+    static __device__ __forceinline__ R out(C v) {
+        unsigned u = __float_as_uint(v);
         if ((u & 0x7fffffffu) > 0x7f800000u) return (R)((u >> 16) | 0x40u);  // quiet NaN
         u += 0x7fffu + ((u >> 16) & 1u);                                       // nearest even
         return (R)(u >> 16);
     }
+    static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
 
 template <typename S>
-__device__ __forceinline__ typename Store<S>::C ld_pop(const typename Store<S>::R *p, size_t k,
-                                                      typename Store<S>::C w, typename Store<S>::C inv) {
-    return Store<S>::in(p[k], w, inv);
+__device__ __forceinline__ typename Store<S>::C ld_pop(const typename Store<S>::R *p, size_t k) {
+    return Store<S>::in(p[k]);
 }
 template <typename S>
-__device__ __forceinline__ void st_pop(typename Store<S>::R *p, size_t k, typename Store<S>::C v,
-                                       typename Store<S>::C w, typename Store<S>::C scale) {
-    __builtin_nontemporal_store(Store<S>::out(v, w, scale), p + k);
+__device__ __forceinline__ void st_pop(typename Store<S>::R *p, size_t k, typename Store<S>::C v) {
+    __builtin_nontemporal_store(Store<S>::out(v), p + k);
 }
 
 // Parameters of one step over one domain.
@@ -108,6 +130,7 @@ template <typename C> struct StepArgs {
     long long plane;
     C omega;
     C corr8, corr9;        // moving-wall terms ((6*w_6)*rho_0)*u_wall, couette_flow.f90:118-119
+                           // (times 2^s in the 16-bit forms)
     C rho_in, rho_out;     // Poiseuille boundary densities
     const C *m_old;        // Poiseuille strips of the previous step [side][rho,ux,uy][ny]
     C *m_new;              // strips of this step (written by the strip pass, read by the step)
@@ -116,6 +139,11 @@ template <typename C> struct StepArgs {
     C w[9];                // lattice weights in C precision
     C scale, inv_scale;    // 2^s and 2^-s of the 16-bit forms
     C force;               // BF: body-force density along x (lattice units)
+    // deviation-space collision of the 16-bit forms (collide_dev):
+    C om1;                 // 1 - omega
+    C kw[9];               // (omega * w_d) * 2^s
+    C kf[9];               // BF: (w_d * ((1 - omega/2) * force)) * 2^s
+    C half_force;          // BF: force / 2
     int n_tiles_x;         // tiles per row
     int n_blocks;          // grid size (for the XCD-aware tile order)
     int xcd_order;         // 1: give each XCD a contiguous band of tiles
@@ -136,8 +164,7 @@ __device__ __forceinline__ int tile_of_block(int p, int n, int xcd_order) {
 
 template <typename S, typename C>
 __device__ __forceinline__ C LD(const StepArgs<C> &a, const typename Store<S>::R *f, int d, int x, int y) {
-    return ld_pop<S>(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x, a.w[d],
-                     a.inv_scale);
+    return ld_pop<S>(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x);
 }
 
 // Pressure-periodic increment of pressure_periodic_distribution
@@ -153,6 +180,12 @@ __device__ __forceinline__ C pf_delta(const StepArgs<C> &a, const C *m, int d, i
     const C cu = (C)cx_of(d) * ux + (C)cy_of(d) * uy;
     const C fac = (((C)1 + (C)3 * cu) + (C)4.5 * cu * cu) - (C)1.5 * usq;
     return a.w[d] * (rho_b - rho_s) * fac;
+}
+// The increment in the units the step works on (times 2^s in the 16-bit forms).
+template <typename S, typename C>
+__device__ __forceinline__ C pf_inc(const StepArgs<C> &a, const C *m, int d, int side, int row) {
+    const C v = pf_delta<C>(a, m, d, side, row);
+    return Store<S>::dev ? v * a.scale : v;
 }
 
 // Value of f_d pulled into cell (x, y) when the source (x - c_d) may lie
@@ -192,7 +225,7 @@ __device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const typename Stor
         }
         if (MODE == PF && xo) {
             const int side = xs < 0 ? 0 : 1;
-            v = v + pf_delta<C>(a, PHASE == 1 ? a.m_old : a.m_new, d, side, y);
+            v = v + pf_inc<S, C>(a, PHASE == 1 ? a.m_old : a.m_new, d, side, y);
         }
         return v;
     }
@@ -201,8 +234,8 @@ __device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const typename Stor
         C v = LD<S, C>(a, f, d, xw, ys);
         if (MODE == PF) {
             const int side = xs < 0 ? 0 : 1;
-            v = v + pf_delta<C>(a, a.m_old, d, side, ys);
-            if (PHASE == 2) v = v + pf_delta<C>(a, a.m_new, d, side, ys);
+            v = v + pf_inc<S, C>(a, a.m_old, d, side, ys);
+            if (PHASE == 2) v = v + pf_inc<S, C>(a, a.m_new, d, side, ys);
         }
         return v;
     }
@@ -281,9 +314,79 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
     }
 }
 
+// BGK collision of the 16-bit forms in deviation space.  D[] holds the
+// scaled deviations D_d = (f_d - w_d) * 2^s on entry and the new ones on exit
+// (not yet rounded to 16 bits).  With Sum w_d = 1 and Sum w_d c_d = 0,
+//   rho - 1 = 2^-s Sum D_d,   rho u = 2^-s Sum c_d D_d (+ F/2 for BF),
+// and the equilibrium deviation is
+//   2^s (f_eq_d - w_d) = 2^s w_d ((rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)),
+// so D'_d = (1 - omega) D_d + (omega w_d 2^s) * inner_d (+ Guo source for BF).
+// Working on deviations keeps the FP32 arithmetic on values of the size of
+// the deviations instead of f ~ w (rounding errors ~1e3 times smaller for
+// the cavity), and needs one division (1/rho) instead of two.  Opposite
+// directions share the even part of the equilibrium:
+//   inner_{d,opp} = (base + (rho/2) h^2) +- rho h,  h = 3 c_d.u,
+//   base = (rho - 1) - 1.5 rho u^2.
+// The operation order here is the definition of the 16-bit forms (the oracle
+// restates it, ffbo_q_step).  No reference counterpart.
 template <typename C, int MODE>
+__device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int x, int y) {
+    const C inv = a.inv_scale;
+    const C a13 = D[1] + D[3], a24 = D[2] + D[4], a57 = D[5] + D[7], a68 = D[6] + D[8];
+    const C b13 = D[1] - D[3], b24 = D[2] - D[4], b57 = D[5] - D[7], b68 = D[6] - D[8];
+    const C m0 = ((D[0] + a13) + a24) + (a57 + a68);
+    const C jx = (b13 + b57) - b68;
+    const C jy = (b24 + b57) + b68;
+    const C drho = m0 * inv;
+    const C rho = (C)1 + drho;
+    const C r = (C)1 / rho;
+    const C ux = (MODE == BF ? __builtin_fmaf(jx, inv, a.half_force) : jx * inv) * r;
+    const C uy = (jy * inv) * r;
+    if (a.write_macro) {
+        const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
+        a.rho_o[m] = rho;
+        a.ux_o[m] = ux;
+        a.uy_o[m] = uy;
+    }
+    const C usq = __builtin_fmaf(ux, ux, uy * uy);
+    const C base = __builtin_fmaf(-((C)1.5 * usq), rho, drho);
+    const C hr = (C)0.5 * rho;
+    const C om1 = a.om1;
+    const C h1 = (C)3 * ux, h2 = (C)3 * uy;
+    const C h5 = h1 + h2, h8 = h1 - h2;
+    D[0] = __builtin_fmaf(om1, D[0], a.kw[0] * base);
+    // pair (d, e) with c_e = -c_d and h = 3 c_d.u
+    auto pair = [&](int d, int e, C h) {
+        const C p = __builtin_fmaf(hr, h * h, base);
+        const C q = rho * h;
+        D[d] = __builtin_fmaf(om1, D[d], a.kw[d] * (p + q));
+        D[e] = __builtin_fmaf(om1, D[e], a.kw[e] * (p - q));
+    };
+    pair(1, 3, h1);
+    pair(2, 4, h2);
+    pair(5, 7, h5);
+    pair(8, 6, h8);
+    if (MODE == BF) {
+        // Guo source 2^s w_d (1 - omega/2) [3 (c_d - u) + 9 (c_d.u) c_d] . F,
+        // F = (force, 0): kf_d * (9 (c_d.u) c_dx + 3 c_dx - 3 u_x)
+        const C g = (C)3 * ux;
+        D[0] = __builtin_fmaf(a.kf[0], -g, D[0]);
+        D[2] = __builtin_fmaf(a.kf[2], -g, D[2]);
+        D[4] = __builtin_fmaf(a.kf[4], -g, D[4]);
+        // c_x = +1: 3 h + 3 - g; c_x = -1: 3 h - 3 - g, with h = 3 c_d.u of that direction
+        D[1] = __builtin_fmaf(a.kf[1], ((C)3 * h1 + (C)3) - g, D[1]);
+        D[3] = __builtin_fmaf(a.kf[3], ((C)3 * h1 - (C)3) - g, D[3]);
+        D[5] = __builtin_fmaf(a.kf[5], ((C)3 * h5 + (C)3) - g, D[5]);
+        D[7] = __builtin_fmaf(a.kf[7], ((C)3 * h5 - (C)3) - g, D[7]);
+        D[8] = __builtin_fmaf(a.kf[8], ((C)3 * h8 + (C)3) - g, D[8]);
+        D[6] = __builtin_fmaf(a.kf[6], ((C)3 * h8 - (C)3) - g, D[6]);
+    }
+}
+
+template <typename S, typename C, int MODE>
 __device__ __forceinline__ void collide_mode(const StepArgs<C> &a, C (&f)[9], int x, int y) {
-    if (MODE == BF) collide_guo<C>(a, f, x, y);
+    if constexpr (Store<S>::dev) collide_dev<C, MODE>(a, f, x, y);
+    else if (MODE == BF) collide_guo<C>(a, f, x, y);
     else collide<C>(a, f, x, y);
 }
 
@@ -302,16 +405,15 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
     if (!edge) {
 #pragma unroll
         for (int d = 0; d < 9; ++d)
-            v[d] = ld_pop<S>(f, (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W + pull_dx(d), a.w[d],
-                             a.inv_scale);
+            v[d] = ld_pop<S>(f, (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W + pull_dx(d));
     } else {
         v[0] = LD<S, C>(a, f, 0, x, y);
 #pragma unroll
         for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
     }
-    collide_mode<C, MODE>(a, v, x, y);
+    collide_mode<S, C, MODE>(a, v, x, y);
 #pragma unroll
-    for (int d = 0; d < 9; ++d) st_pop<S>(g, (size_t)d * P + c, v[d], a.w[d], a.scale);
+    for (int d = 0; d < 9; ++d) st_pop<S>(g, (size_t)d * P + c, v[d]);
 }
 
 // One fused pull-stream-collide step over one domain.  Each thread updates V
@@ -379,10 +481,10 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     for (int k = 0; k < V; ++k) {
         C v[9];
 #pragma unroll
-        for (int d = 0; d < 9; ++d) v[d] = Store<S>::in(in[d][k], a.w[d], a.inv_scale);
-        collide_mode<C, MODE>(a, v, x0 + k, y);
+        for (int d = 0; d < 9; ++d) v[d] = Store<S>::in(in[d][k]);
+        collide_mode<S, C, MODE>(a, v, x0 + k, y);
 #pragma unroll
-        for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d], a.w[d], a.scale);
+        for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d]);
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(g + (size_t)d * P + c));
@@ -400,22 +502,28 @@ __global__ void __launch_bounds__(256) pf_strip_kernel(StepArgs<C> a) {
     const int side = k / a.ny, y = k - side * a.ny;
     const int x = side == 0 ? a.nx - 1 : 0;
     const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(a.src);
-    const C f1 = pull_edge<S, C, PF, 1>(a, f, 0, x, y);
-    const C f2 = pull_edge<S, C, PF, 1>(a, f, 1, x, y);
-    const C f3 = pull_edge<S, C, PF, 1>(a, f, 2, x, y);
-    const C f4 = pull_edge<S, C, PF, 1>(a, f, 3, x, y);
-    const C f5 = pull_edge<S, C, PF, 1>(a, f, 4, x, y);
-    const C f6 = pull_edge<S, C, PF, 1>(a, f, 5, x, y);
-    const C f7 = pull_edge<S, C, PF, 1>(a, f, 6, x, y);
-    const C f8 = pull_edge<S, C, PF, 1>(a, f, 7, x, y);
-    const C f9 = pull_edge<S, C, PF, 1>(a, f, 8, x, y);
-    const C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
-    const C ux = f2 - f4 + f6 - f7 - f8 + f9;
-    const C uy = f3 - f5 + f6 + f7 - f8 - f9;
+    C v[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) v[d] = pull_edge<S, C, PF, 1>(a, f, d, x, y);
     C *m = a.m_new;
-    m[(side * 3 + 0) * a.ny + y] = rho;
-    m[(side * 3 + 1) * a.ny + y] = ux / rho;
-    m[(side * 3 + 2) * a.ny + y] = uy / rho;
+    if constexpr (Store<S>::dev) {
+        // deviation form: the sums of collide_dev
+        const C inv = a.inv_scale;
+        const C a13 = v[1] + v[3], a24 = v[2] + v[4], a57 = v[5] + v[7], a68 = v[6] + v[8];
+        const C b13 = v[1] - v[3], b24 = v[2] - v[4], b57 = v[5] - v[7], b68 = v[6] - v[8];
+        const C rho = (C)1 + (((v[0] + a13) + a24) + (a57 + a68)) * inv;
+        m[(side * 3 + 0) * a.ny + y] = rho;
+        m[(side * 3 + 1) * a.ny + y] = (((b13 + b57) - b68) * inv) / rho;
+        m[(side * 3 + 2) * a.ny + y] = (((b24 + b57) + b68) * inv) / rho;
+    } else {
+        const C f1 = v[0], f2 = v[1], f3 = v[2], f4 = v[3], f5 = v[4], f6 = v[5], f7 = v[6], f8 = v[7], f9 = v[8];
+        const C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
+        const C ux = f2 - f4 + f6 - f7 - f8 + f9;
+        const C uy = f3 - f5 + f6 + f7 - f8 - f9;
+        m[(side * 3 + 0) * a.ny + y] = rho;
+        m[(side * 3 + 1) * a.ny + y] = ux / rho;
+        m[(side * 3 + 2) * a.ny + y] = uy / rho;
+    }
 }
 
 // Equilibrium initial state (initialization.f90:9-198): f_d = w_d * rho_0 *
@@ -441,7 +549,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepArgs<C> a, C rho_0, const
         } else {
             feq = a.w[d] * rho_0;
         }
-        st_pop<S>(g, (size_t)d * (size_t)a.plane + c, feq, a.w[d], a.scale);
+        st_pop<S>(g, (size_t)d * (size_t)a.plane + c, Store<S>::from_f(feq, a.w[d], a.scale));
     }
     if (a.rho_o) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
@@ -465,11 +573,10 @@ __global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, dou
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
             double v;
-            if (sizeof(S) == 2)
-                v = (double)a.w[d] + (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, 0, 1) *
-                                         (double)a.inv_scale;
+            if (Store<S>::dev)
+                v = (double)a.w[d] + (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c) * (double)a.inv_scale;
             else
-                v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, a.w[d], a.inv_scale);
+                v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c);
             m += v;
             mx += (double)cx_of(d) * v;
             my += (double)cy_of(d) * v;

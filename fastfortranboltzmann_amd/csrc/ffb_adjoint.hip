@@ -74,63 +74,84 @@ __device__ __forceinline__ void reader_of(int x, int y, int j, int nx, int ny, i
     tj = j;
 }
 
-// Pulled populations p of cell (x, y) from s.src (the forward rule).
-template <typename C, int MODE>
+// Pulled values of cell (x, y) from s.src by the forward rule: populations p
+// (FP32 / FP64 storage) or scaled deviations D = (p - w) 2^s (deviation form).
+template <typename S, typename C, int MODE>
 __device__ __forceinline__ void pull_cell(const StepArgs<C> &s, int x, int y, C (&p)[9]) {
-    const C *f = static_cast<const C *>(s.src);
+    const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(s.src);
     const bool edge = (x == 0) | (x == s.nx - 1) | (y == 0) | (y == s.ny - 1);
     if (!edge) {
         const size_t c = (size_t)y * (size_t)s.pitch + (size_t)x;
 #pragma unroll
         for (int d = 0; d < 9; ++d)
-            p[d] = f[(size_t)d * (size_t)s.plane + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)s.pitch + pull_dx(d)];
+            p[d] = ld_pop<S>(f, (size_t)d * (size_t)s.plane + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)s.pitch + pull_dx(d));
     } else {
-        p[0] = LD<C, C>(s, f, 0, x, y);
+        p[0] = LD<S, C>(s, f, 0, x, y);
 #pragma unroll
-        for (int d = 1; d < 9; ++d) p[d] = pull_edge<C, C, MODE, 2>(s, f, d, x, y);
+        for (int d = 1; d < 9; ++d) p[d] = pull_edge<S, C, MODE, 2>(s, f, d, x, y);
     }
 }
 
 template <typename C> struct Moments {
     C rho, ux, uy, usq;
+    C drho;  // rho - 1, from the deviations in the deviation form
 };
 
-template <typename C, int MODE> __device__ __forceinline__ Moments<C> moments_of(const StepArgs<C> &s, const C (&p)[9]) {
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ Moments<C> moments_of(const StepArgs<C> &s, const C (&p)[9]) {
     Moments<C> m;
-    m.rho = p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8];
-    const C jx = p[1] - p[3] + p[5] - p[6] - p[7] + p[8];
-    const C jy = p[2] - p[4] + p[5] + p[6] - p[7] - p[8];
     const C F = MODE == BF ? s.force : (C)0;
-    m.ux = (jx + (C)0.5 * F) / m.rho;
-    m.uy = jy / m.rho;
+    if constexpr (Store<S>::dev) {
+        const C inv = s.inv_scale;
+        m.drho = (((p[0] + (p[1] + p[3])) + (p[2] + p[4])) + ((p[5] + p[7]) + (p[6] + p[8]))) * inv;
+        m.rho = (C)1 + m.drho;
+        const C jx = (((p[1] - p[3]) + (p[5] - p[7])) - (p[6] - p[8])) * inv;
+        const C jy = (((p[2] - p[4]) + (p[5] - p[7])) + (p[6] - p[8])) * inv;
+        m.ux = (jx + (C)0.5 * F) / m.rho;
+        m.uy = jy / m.rho;
+    } else {
+        m.rho = p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8];
+        m.drho = m.rho - (C)1;
+        const C jx = p[1] - p[3] + p[5] - p[6] - p[7] + p[8];
+        const C jy = p[2] - p[4] + p[5] + p[6] - p[7] - p[8];
+        m.ux = (jx + (C)0.5 * F) / m.rho;
+        m.uy = jy / m.rho;
+    }
     m.usq = m.ux * m.ux + m.uy * m.uy;
     return m;
 }
 
-// Block sum of three doubles into acc[blockIdx.x*3 + k] (+=).
+// Block sum of three doubles into acc[blockIdx.x*3 + k] (+=): butterfly sums
+// inside each 64-wide wave, then the ATW/64 wave sums in a fixed order (the
+// result does not depend on timing).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 __device__ __forceinline__ void block_accumulate(double *acc, double g0, double g1, double g2) {
-    __shared__ double sh[3][ATW];
-    sh[0][threadIdx.x] = g0;
-    sh[1][threadIdx.x] = g1;
-    sh[2][threadIdx.x] = g2;
-    __syncthreads();
-    for (int k = ATW / 2; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
-            sh[2][threadIdx.x] += sh[2][threadIdx.x + k];
-        }
-        __syncthreads();
+    constexpr int W = ATW / 64;
+    __shared__ double sh[3][W];
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    const int wave = (int)threadIdx.x >> 6;
+    if (((int)threadIdx.x & 63) == 0) {
+        sh[0][wave] = g0;
+        sh[1][wave] = g1;
+        sh[2][wave] = g2;
     }
-    if (threadIdx.x == 0) {
-        acc[3 * blockIdx.x + 0] += sh[0][0];
-        acc[3 * blockIdx.x + 1] += sh[1][0];
-        acc[3 * blockIdx.x + 2] += sh[2][0];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) t += sh[threadIdx.x][k];
+        acc[3 * blockIdx.x + threadIdx.x] += t;
     }
 }
 
 // Loss seed at the last step: lambda_p^N and the direct dL/dF, dL/du_wall terms.
-template <typename C, int MODE>
+template <typename S, typename C, int MODE>
 __global__ void __launch_bounds__(ATW) adjoint_seed_kernel(AdjArgs<C> a) {
     const StepArgs<C> &s = a.s;
     const int y = (int)blockIdx.x / s.n_tiles_x;
@@ -138,18 +159,19 @@ __global__ void __launch_bounds__(ATW) adjoint_seed_kernel(AdjArgs<C> a) {
     double gF = 0, gW = 0;
     if (x < s.nx) {
         C p[9];
-        pull_cell<C, MODE>(s, x, y, p);
-        const Moments<C> m = moments_of<C, MODE>(s, p);
+        pull_cell<S, C, MODE>(s, x, y, p);
+        const Moments<C> m = moments_of<S, C, MODE>(s, p);
+        const C r = (C)1 / m.rho;
         const size_t mi = (size_t)y * (size_t)s.nx + (size_t)x;
         const C gx = a.g_ux[mi], gy = a.g_uy[mi];
         const size_t c = (size_t)y * (size_t)s.pitch + (size_t)x;
         C lam[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            lam[k] = (gx * ((C)cx_of(k) - m.ux) + gy * ((C)cy_of(k) - m.uy)) / m.rho;
+            lam[k] = (gx * ((C)cx_of(k) - m.ux) + gy * ((C)cy_of(k) - m.uy)) * r;
             a.lam_out[(size_t)k * (size_t)s.plane + c] = lam[k];
         }
-        if (MODE == BF) gF = (double)gx * 0.5 / (double)m.rho;
+        if (MODE == BF) gF = (double)gx * 0.5 * (double)r;
         if ((MODE == CF || MODE == SL) && y == s.ny - 1)
             gW = -(double)lam[7] * (double)a.dk_du + (double)lam[8] * (double)a.dk_du;
     }
@@ -158,7 +180,7 @@ __global__ void __launch_bounds__(ATW) adjoint_seed_kernel(AdjArgs<C> a) {
 
 // One reverse step: lambda_p^{t+1} (lam_in) -> lambda_p^t (lam_out), with
 // p^t recomputed from f^{t-1} (s.src).
-template <typename C, int MODE>
+template <typename S, typename C, int MODE>
 __global__ void __launch_bounds__(ATW) adjoint_step_kernel(AdjArgs<C> a) {
     const StepArgs<C> &s = a.s;
     const int y = (int)blockIdx.x / s.n_tiles_x;
@@ -174,8 +196,9 @@ __global__ void __launch_bounds__(ATW) adjoint_step_kernel(AdjArgs<C> a) {
             lf[j] = a.lam_in[(size_t)tj * (size_t)s.plane + (size_t)ty * (size_t)s.pitch + (size_t)tx];
         }
         C p[9];
-        pull_cell<C, MODE>(s, x, y, p);
-        const Moments<C> m = moments_of<C, MODE>(s, p);
+        pull_cell<S, C, MODE>(s, x, y, p);
+        const Moments<C> m = moments_of<S, C, MODE>(s, p);
+        const C r = (C)1 / m.rho;
         const C om = s.omega;
         const C F = MODE == BF ? s.force : (C)0;
         const C half_pre = (C)1 - (C)0.5 * om;  // 1 - omega/2
@@ -185,17 +208,23 @@ __global__ void __launch_bounds__(ATW) adjoint_step_kernel(AdjArgs<C> a) {
             const C cx = (C)cx_of(i), cy = (C)cy_of(i), w = s.w[i];
             const C cu = cx * m.ux + cy * m.uy;
             const C feq = w * m.rho * (((C)1 + (C)3 * cu) + (C)4.5 * cu * cu - (C)1.5 * m.usq);
-            A += lf[i] * feq / m.rho;
+            // f_eq - p: in the deviation form from rho - 1 and D, without the
+            // cancellation of two values near w
+            const C neq = Store<S>::dev
+                              ? w * (m.drho + m.rho * (((C)3 * cu + (C)4.5 * cu * cu) - (C)1.5 * m.usq)) -
+                                    p[i] * s.inv_scale
+                              : feq - p[i];
+            A += lf[i] * feq * r;
             Bx += lf[i] * w * m.rho * ((C)3 * cx + (C)9 * cu * cx - (C)3 * m.ux);
             By += lf[i] * w * m.rho * ((C)3 * cy + (C)9 * cu * cy - (C)3 * m.uy);
             if (MODE == BF) {
                 const C sfac = (C)3 * (cx - m.ux) + (C)9 * cu * cx;  // [3(c-u) + 9(c.u)c] . e_x
                 Dx += lf[i] * half_pre * w * F * ((C)9 * cx * cx - (C)3);
                 Dy += lf[i] * half_pre * w * F * ((C)9 * cx * cy);
-                GO += lf[i] * (feq - p[i] - (C)0.5 * w * F * sfac);
+                GO += lf[i] * (neq - (C)0.5 * w * F * sfac);
                 GF += lf[i] * half_pre * w * sfac;
             } else {
-                GO += lf[i] * (feq - p[i]);
+                GO += lf[i] * neq;
             }
         }
         const C ax = om * Bx + Dx, ay = om * By + Dy;
@@ -203,11 +232,11 @@ __global__ void __launch_bounds__(ATW) adjoint_step_kernel(AdjArgs<C> a) {
         C lp[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            lp[k] = ((C)1 - om) * lf[k] + om * A + (ax * ((C)cx_of(k) - m.ux) + ay * ((C)cy_of(k) - m.uy)) / m.rho;
+            lp[k] = ((C)1 - om) * lf[k] + om * A + (ax * ((C)cx_of(k) - m.ux) + ay * ((C)cy_of(k) - m.uy)) * r;
             a.lam_out[(size_t)k * (size_t)s.plane + c] = lp[k];
         }
         gO = (double)GO;
-        if (MODE == BF) gF = (double)GF + (double)ax * 0.5 / (double)m.rho;
+        if (MODE == BF) gF = (double)GF + (double)ax * 0.5 * (double)r;
         if ((MODE == CF || MODE == SL) && y == s.ny - 1)
             gW = -(double)lp[7] * (double)a.dk_du + (double)lp[8] * (double)a.dk_du;
     }
@@ -233,7 +262,7 @@ struct ffb_adjoint {
 
 namespace {
 
-template <typename C> StepArgs<C> adj_step_args(const ffb_adjoint *A, const void *src) {
+template <typename S, typename C> StepArgs<C> adj_step_args(const ffb_adjoint *A, const void *src) {
     const ffb_lattice *L = A->lat;
     const ffb_config_t &c = L->cfg;
     StepArgs<C> s{};
@@ -253,15 +282,22 @@ template <typename C> StepArgs<C> adj_step_args(const ffb_adjoint *A, const void
     for (int d = 5; d <= 8; ++d) s.w[d] = (C)1 / (C)36;
     s.scale = 1;
     s.inv_scale = 1;
+    if constexpr (Store<S>::dev) {  // pulled deviations: scaled wall terms (as the forward step)
+        s.scale = (C)std::ldexp(1.0, c.storage_scale_log2);
+        s.inv_scale = (C)std::ldexp(1.0, -c.storage_scale_log2);
+        s.corr8 = s.corr8 * s.scale;
+        s.corr9 = s.corr9 * s.scale;
+    }
     s.force = (C)c.body_force;
     s.n_tiles_x = (c.n_x + ATW - 1) / ATW;
     s.n_blocks = s.n_tiles_x * c.n_y;
     return s;
 }
 
-template <typename C, int MODE> int launch_adj(ffb_adjoint *A, bool seed, const void *state, int lam_in, int lam_out) {
+template <typename S, typename C, int MODE>
+int launch_adj(ffb_adjoint *A, bool seed, const void *state, int lam_in, int lam_out) {
     AdjArgs<C> a{};
-    a.s = adj_step_args<C>(A, state);
+    a.s = adj_step_args<S, C>(A, state);
     a.lam_in = static_cast<const C *>(A->lam[lam_in]);
     a.lam_out = static_cast<C *>(A->lam[lam_out]);
     a.g_ux = static_cast<const C *>(A->gseed[0]);
@@ -271,24 +307,28 @@ template <typename C, int MODE> int launch_adj(ffb_adjoint *A, bool seed, const 
     a.dk_du = (c.sim_mode == FFB_COUETTE_FLOW || c.sim_mode == FFB_SLIDING_LID)
                   ? (C)6 * ((C)1 / (C)36) * (C)c.rho_0 : (C)0;
     const dim3 grid((unsigned)a.s.n_blocks);
-    if (seed) hipLaunchKernelGGL((adjoint_seed_kernel<C, MODE>), grid, dim3(ATW), 0, A->lat->stream, a);
-    else hipLaunchKernelGGL((adjoint_step_kernel<C, MODE>), grid, dim3(ATW), 0, A->lat->stream, a);
+    if (seed) hipLaunchKernelGGL((adjoint_seed_kernel<S, C, MODE>), grid, dim3(ATW), 0, A->lat->stream, a);
+    else hipLaunchKernelGGL((adjoint_step_kernel<S, C, MODE>), grid, dim3(ATW), 0, A->lat->stream, a);
     AHIPCHK(hipGetLastError());
     return FFB_OK;
 }
 
-template <typename C> int launch_adj_mode(ffb_adjoint *A, bool seed, const void *state, int lam_in, int lam_out) {
+template <typename S, typename C>
+int launch_adj_mode(ffb_adjoint *A, bool seed, const void *state, int lam_in, int lam_out) {
     switch (A->lat->cfg.sim_mode) {
-    case FFB_SHEAR_WAVE: return launch_adj<C, SW>(A, seed, state, lam_in, lam_out);
-    case FFB_COUETTE_FLOW: return launch_adj<C, CF>(A, seed, state, lam_in, lam_out);
-    case FFB_SLIDING_LID: return launch_adj<C, SL>(A, seed, state, lam_in, lam_out);
-    default: return launch_adj<C, BF>(A, seed, state, lam_in, lam_out);
+    case FFB_SHEAR_WAVE: return launch_adj<S, C, SW>(A, seed, state, lam_in, lam_out);
+    case FFB_COUETTE_FLOW: return launch_adj<S, C, CF>(A, seed, state, lam_in, lam_out);
+    case FFB_SLIDING_LID: return launch_adj<S, C, SL>(A, seed, state, lam_in, lam_out);
+    default: return launch_adj<S, C, BF>(A, seed, state, lam_in, lam_out);
     }
 }
 
 int adj(ffb_adjoint *A, bool seed, const void *state, int lam_in, int lam_out) {
-    return A->lat->cfg.precision == FFB_PREC_FP64 ? launch_adj_mode<double>(A, seed, state, lam_in, lam_out)
-                                                  : launch_adj_mode<float>(A, seed, state, lam_in, lam_out);
+    switch (A->lat->cfg.precision) {
+    case FFB_PREC_FP64: return launch_adj_mode<double, double>(A, seed, state, lam_in, lam_out);
+    case FFB_PREC_FP32_DEV: return launch_adj_mode<f32dev, float>(A, seed, state, lam_in, lam_out);
+    default: return launch_adj_mode<float, float>(A, seed, state, lam_in, lam_out);
+    }
 }
 
 unsigned char *slot(void *base, size_t bytes, int k) { return static_cast<unsigned char *>(base) + bytes * (size_t)k; }
@@ -305,15 +345,16 @@ int ffb_adjoint_create(ffb_lattice_t *L, int32_t n_steps, int32_t checkpoint_int
     if (mode == FFB_POISEUILLE_FLOW)
         return fail(FFB_E_INVALID, "ffb_adjoint_create: the pressure-driven Poiseuille mode has no adjoint; "
                                    "use FFB_BODY_FORCE_CHANNEL");
-    if (L->cfg.precision != FFB_PREC_FP32 && L->cfg.precision != FFB_PREC_FP64)
-        return fail(FFB_E_INVALID, "ffb_adjoint_create: FP32 or FP64 storage only");
+    if (L->cfg.precision != FFB_PREC_FP32 && L->cfg.precision != FFB_PREC_FP64 &&
+        L->cfg.precision != FFB_PREC_FP32_DEV)
+        return fail(FFB_E_INVALID, "ffb_adjoint_create: FP32, FP32 deviation or FP64 storage only");
     AHIPCHK(hipSetDevice(L->cfg.device));
     ffb_adjoint *A = new ffb_adjoint();
     A->lat = L;
     A->n_steps = n_steps;
     A->interval = checkpoint_interval;
     A->n_ck = (n_steps - 1) / checkpoint_interval + 1;
-    A->state_bytes = (size_t)9 * (size_t)L->plane * L->elem;
+    A->state_bytes = (size_t)L->set_elems * L->elem;
     const int seg_slots = std::min(checkpoint_interval, n_steps);
     A->blocks = ((L->cfg.n_x + ATW - 1) / ATW) * L->cfg.n_y;
     auto bail = [&](hipError_t e, const char *what) {

@@ -95,12 +95,25 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     a.w[0] = (C)4 / (C)9;
     for (int d = 1; d <= 4; ++d) a.w[d] = (C)1 / (C)9;
     for (int d = 5; d <= 8; ++d) a.w[d] = (C)1 / (C)36;
-    const int s = (c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV) ? c.storage_scale_log2 : 0;
+    const bool dev = c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV ||
+                     c.precision == FFB_PREC_FP32_DEV;
+    const int s = dev ? c.storage_scale_log2 : 0;
     a.scale = (C)std::ldexp(1.0, s);
     a.inv_scale = (C)std::ldexp(1.0, -s);
     const int cells_per_tile = L->tile_w * cells_per_thread(L);
     a.n_tiles_x = (c.n_x + cells_per_tile - 1) / cells_per_tile;
     a.force = (C)c.body_force;
+    if (dev) {  // the 16-bit forms work on deviations times 2^s (collide_dev)
+        a.corr8 = a.corr8 * a.scale;
+        a.corr9 = a.corr9 * a.scale;
+    }
+    a.om1 = (C)1 - a.omega;
+    const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
+    for (int d = 0; d < 9; ++d) {
+        a.kw[d] = (a.omega * a.w[d]) * a.scale;
+        a.kf[d] = (a.w[d] * pre) * a.scale;
+    }
+    a.half_force = (C)0.5 * a.force;
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_order = L->xcd_order;
     return a;
@@ -176,6 +189,7 @@ int dispatch_step(ffb_lattice *L, int write_macro) {
     case FFB_PREC_FP64: return launch_step<double, double>(L, write_macro);
     case FFB_PREC_FP16_DEV: return launch_step<__half, float>(L, write_macro);
     case FFB_PREC_BF16_DEV: return launch_step<__hip_bfloat16, float>(L, write_macro);
+    case FFB_PREC_FP32_DEV: return launch_step<f32dev, float>(L, write_macro);
     default: return launch_step<float, float>(L, write_macro);
     }
 }
@@ -220,6 +234,7 @@ int dispatch_init(ffb_lattice *L) {
     case FFB_PREC_FP64: return launch_init<double, double>(L);
     case FFB_PREC_FP16_DEV: return launch_init<__half, float>(L);
     case FFB_PREC_BF16_DEV: return launch_init<__hip_bfloat16, float>(L);
+    case FFB_PREC_FP32_DEV: return launch_init<f32dev, float>(L);
     default: return launch_init<float, float>(L);
     }
 }
@@ -272,15 +287,9 @@ __global__ void __launch_bounds__(256) copy_kernel(const v4f *__restrict__ src, 
     if (VARIANT == 0) {
         for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256)
             __builtin_nontemporal_store(src[k], dst + k);
-    } else if (VARIANT == 3) {
-        const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
-        v4f v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (base + j * 256 < n) v[j] = src[base + j * 256];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (base + j * 256 < n) __builtin_nontemporal_store(v[j], dst + base + j * 256);
+    } else if (VARIANT == 3) {  // nontemporal load and store (fastest on MI355X)
+        const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+        if (k < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + k), dst + k);
     } else {
         const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
         if (k < n) {
@@ -300,7 +309,8 @@ int step_between(ffb_lattice *L, const void *src, void *dst, int write_macro) {
     switch (L->cfg.precision) {
     case FFB_PREC_FP64: return launch_between<double, double>(L, src, dst, write_macro);
     case FFB_PREC_FP32: return launch_between<float, float>(L, src, dst, write_macro);
-    default: return fail(FFB_E_INVALID, "step between buffers: FP32 or FP64 storage only");
+    case FFB_PREC_FP32_DEV: return launch_between<f32dev, float>(L, src, dst, write_macro);
+    default: return fail(FFB_E_INVALID, "step between buffers: FP32, FP32 deviation or FP64 storage only");
     }
 }
 }  // namespace ffb
@@ -336,9 +346,10 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         return fail(FFB_E_INVALID, "ffb_create: domain larger than 2^34 cells");
     if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_BODY_FORCE_CHANNEL)
         return fail(FFB_E_INVALID, "ffb_create: unknown sim mode " + std::to_string(cfg->sim_mode));
-    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_BF16_DEV)
+    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_FP32_DEV)
         return fail(FFB_E_INVALID, "ffb_create: unknown precision " + std::to_string(cfg->precision));
-    if ((cfg->precision == FFB_PREC_FP16_DEV || cfg->precision == FFB_PREC_BF16_DEV) &&
+    if ((cfg->precision == FFB_PREC_FP16_DEV || cfg->precision == FFB_PREC_BF16_DEV ||
+         cfg->precision == FFB_PREC_FP32_DEV) &&
         (cfg->storage_scale_log2 < -30 || cfg->storage_scale_log2 > 30))
         return fail(FFB_E_INVALID, "ffb_create: storage_scale_log2 outside [-30, 30]");
     int ndev = 0;
@@ -349,29 +360,45 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
 
     ffb_lattice *L = new ffb_lattice();
     L->cfg = *cfg;
-    L->pitch = ((cfg->n_x + 63) / 64) * 64;
-    // Tuning parameters of the step kernel, defaults measured on MI355X:
-    //   FP32 (4096^2 cavity): one cell per thread, 512-thread tiles (206 us;
-    //     2 or 4 cells per thread 234-250 us);
-    //   FP64: one cell per thread, 256-thread tiles (426 us against 464 us);
-    //   16-bit forms (8192^2): 4 cells per thread in 8-byte vectors, 512-thread
-    //     tiles (487 us against 498-563 us with one cell per thread);
+    const int px = ((cfg->n_x + 63) / 64) * 64;
+    // Layout and launch shape of the step kernel, defaults measured on MI355X
+    // (cavity, one launch per step; us per step):
+    //   layout 1 (rows: the nine direction rows of each y next to each other)
+    //     against layout 0 (nine planes): 18 streams of one step fall into a
+    //     few neighbouring DRAM regions instead of 18 distant ones --
+    //     FP32 4096^2 193 against 206, FP64 393 against 428, FP16 8192^2 424
+    //     against 473, BF16 451 against 491;
+    //   one cell per thread: FP32/FP64 with 256-/512-thread tiles (FP32 197,
+    //     FP64 410 with the other width); 16-bit forms with 256-thread tiles
+    //     (2 or 4 cells per thread in 4- or 8-byte vectors 440-489);
     //   dispatch-order tiles: round-robin over the XCDs keeps all of them on
-    //     neighbouring rows (202 us against 222 us with XCD-banded tiles).
-    // FFB_TILE_W, FFB_VEC (cells per thread), FFB_XCD_ORDER, FFB_PLANE_PAD,
-    // FFB_BUF_PAD override them.
+    //     neighbouring rows (202 against 222 with XCD-banded tiles, planes).
+    // FFB_LAYOUT, FFB_TILE_W, FFB_VEC (cells per thread), FFB_XCD_ORDER,
+    // FFB_PLANE_PAD, FFB_BUF_PAD override them.
     auto env_ll = [](const char *name, long long dflt) {
         const char *v = std::getenv(name);
         return v ? std::atoll(v) : dflt;
     };
     L->plane_pad = ((env_ll("FFB_PLANE_PAD", 0) + 63) / 64) * 64;
     L->buf_pad = ((env_ll("FFB_BUF_PAD", 0) + 63) / 64) * 64;
-    L->plane = (long long)L->pitch * cfg->n_y + L->plane_pad;
-    L->elem = cfg->precision == FFB_PREC_FP64 ? 8 : (cfg->precision == FFB_PREC_FP32 ? 4 : 2);
+    L->layout = (int)env_ll("FFB_LAYOUT", 1) == 0 ? 0 : 1;
+    if (L->layout == 1) {
+        L->plane = px;
+        L->pitch = 9 * px;
+        L->set_elems = (long long)L->pitch * cfg->n_y;
+    } else {
+        L->pitch = px;
+        L->plane = (long long)px * cfg->n_y + L->plane_pad;
+        L->set_elems = 9 * L->plane;
+    }
+    L->elem = cfg->precision == FFB_PREC_FP64
+                  ? 8
+                  : (cfg->precision == FFB_PREC_FP32 || cfg->precision == FFB_PREC_FP32_DEV ? 4 : 2);
     L->celem = cfg->precision == FFB_PREC_FP64 ? 8 : 4;
     const bool half = L->elem == 2;
-    L->vec = (int)env_ll("FFB_VEC", half ? 4 : 1);
-    L->tile_w = (int)env_ll("FFB_TILE_W", cfg->precision == FFB_PREC_FP64 ? 256 : 512);
+    const bool rows = L->layout == 1;
+    L->vec = (int)env_ll("FFB_VEC", half && !rows ? 4 : 1);
+    L->tile_w = (int)env_ll("FFB_TILE_W", cfg->precision == FFB_PREC_FP64 ? (rows ? 512 : 256) : (rows ? 256 : 512));
     L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 0);
     if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512) L->tile_w = 256;
     // cells per thread must be 1, 2, 4 or 8 with at most 16 bytes per vector and
@@ -384,7 +411,7 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         return fail(FFB_E_HIP, std::string("ffb_create: ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    const size_t set_elems = (size_t)9 * (size_t)L->plane + (size_t)L->buf_pad;
+    const size_t set_elems = (size_t)L->set_elems + (size_t)L->buf_pad;
     if ((e = hipMalloc(&L->pop_alloc, 2 * set_elems * L->elem)) != hipSuccess)
         return cleanup_fail(e, "population buffers");
     L->buf[0] = L->pop_alloc;
@@ -492,7 +519,7 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
     if (!L || !f_halo) return fail(FFB_E_INVALID, "ffb_read_populations: null argument");
     HIPCHK(hipSetDevice(L->cfg.device));
     const int nx = L->cfg.n_x, ny = L->cfg.n_y;
-    std::vector<unsigned char> stage((size_t)9 * (size_t)L->plane * L->elem);
+    std::vector<unsigned char> stage((size_t)L->set_elems * L->elem);
     HIPCHK(hipMemcpyAsync(stage.data(), L->buf[L->cur], stage.size(), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
@@ -505,6 +532,9 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
                 switch (L->cfg.precision) {
                 case FFB_PREC_FP64: ((double *)f_halo)[dst] = ((const double *)stage.data())[src]; break;
                 case FFB_PREC_FP32: ((float *)f_halo)[dst] = ((const float *)stage.data())[src]; break;
+                case FFB_PREC_FP32_DEV:
+                    ((float *)f_halo)[dst] = weightf(d) + ((const float *)stage.data())[src] * (float)std::ldexp(1.0, -s);
+                    break;
                 case FFB_PREC_FP16_DEV:
                     ((float *)f_halo)[dst] = weightf(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
                                                                     (float)std::ldexp(1.0, -s);
@@ -535,7 +565,7 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
     if (!L || !f_halo) return fail(FFB_E_INVALID, "ffb_write_populations: null argument");
     HIPCHK(hipSetDevice(L->cfg.device));
     const int nx = L->cfg.n_x, ny = L->cfg.n_y;
-    std::vector<unsigned char> stage((size_t)9 * (size_t)L->plane * L->elem, 0);
+    std::vector<unsigned char> stage((size_t)L->set_elems * L->elem, 0);
     const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
     const int s = L->cfg.storage_scale_log2;
     for (int d = 0; d < 9; ++d) {
@@ -546,6 +576,9 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
                 switch (L->cfg.precision) {
                 case FFB_PREC_FP64: ((double *)stage.data())[dst] = ((const double *)f_halo)[src]; break;
                 case FFB_PREC_FP32: ((float *)stage.data())[dst] = ((const float *)f_halo)[src]; break;
+                case FFB_PREC_FP32_DEV:
+                    ((float *)stage.data())[dst] = (((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s);
+                    break;
                 case FFB_PREC_FP16_DEV:
                     ((uint16_t *)stage.data())[dst] =
                         float_to_half((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
@@ -605,6 +638,7 @@ int ffb_moments(ffb_lattice_t *L, double *mass, double *mom_x, double *mom_y) {
     case FFB_PREC_FP64: rc = launch_moments<double, double>(L); break;
     case FFB_PREC_FP16_DEV: rc = launch_moments<__half, float>(L); break;
     case FFB_PREC_BF16_DEV: rc = launch_moments<__hip_bfloat16, float>(L); break;
+    case FFB_PREC_FP32_DEV: rc = launch_moments<f32dev, float>(L); break;
     default: rc = launch_moments<float, float>(L); break;
     }
     if (rc) return rc;
@@ -645,9 +679,7 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps) {
         if ((e = hipEventCreate(&e1)) != hipSuccess) return e;
         float best = 0.f;
         for (int variant = 0; variant < 4; ++variant) {
-            const unsigned grid = variant == 0 ? 256u * 32u
-                                  : variant == 3 ? (unsigned)((n + 1023) / 1024)
-                                                 : (unsigned)((n + 255) / 256);
+            const unsigned grid = variant == 0 ? 256u * 32u : (unsigned)((n + 255) / 256);
             auto launch = [&]() {
                 switch (variant) {
                 case 0: hipLaunchKernelGGL(copy_kernel<0>, dim3(grid), dim3(256), 0, st, (const v4f *)a, (v4f *)b, n); break;

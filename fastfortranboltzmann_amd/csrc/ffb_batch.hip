@@ -5,11 +5,12 @@
 // autotuner picks the workgroup size and the persistent grid.  No reference
 // counterpart (the reference runs one domain per process).
 //
-// Layout: every lattice keeps the single-domain layout (9 SoA planes, row
-// pitch rounded to 64 elements, no halo) inside two pools (ping-pong); a
+// Layout: every lattice keeps the single-domain row-blocked layout (per row
+// the nine direction rows, each n_x rounded up to 64 elements; no halo) inside
+// two pools (ping-pong); a
 // descriptor table holds each lattice's offsets and parameters.  Work is cut
 // into row segments of 64 cells (one wave each, so every load of a wave is
-// one aligned stretch of one row); tile_lat[t] names the lattice of segment t,
+// one aligned stretch of one row); segs[t] names the lattice and row of segment t,
 // and the segments are ordered by boundary kind so that the waves resident
 // at one time run the same code.  A workgroup of TW threads takes TW/64
 // consecutive segments per iteration of its persistent loop.
@@ -38,7 +39,7 @@ namespace {
 
 template <typename C> struct BatchLat {
     long long off;        // element offset of the lattice's planes in a pool
-    long long plane;      // elements per plane
+    long long plane;      // direction stride in elements (row stride: pitch)
     long long macro_off;  // element offset in the macro arrays
     long long prof_off;   // element offset of the shear-wave profile row table
     int nx, ny, pitch, kind;
@@ -47,11 +48,15 @@ template <typename C> struct BatchLat {
     C omega, corr8, corr9, force, rho_0;
 };
 
+struct Seg {
+    int lat, y;  // lattice index and row of one 64-cell row segment
+};
+
 template <typename C> struct BatchArgs {
     const void *src_pool;
     void *dst_pool;
     const BatchLat<C> *lats;
-    const int *tile_lat;  // lattice of each row segment
+    const Seg *segs;      // (lattice, row) of each row segment
     int n_tiles;          // row segments
     int write_macro;
     C *rho_o, *ux_o, *uy_o;
@@ -59,10 +64,23 @@ template <typename C> struct BatchArgs {
     C w[9];
 };
 
-template <typename C>
+// The segment and descriptor tables are read-only during a launch and every
+// wave reads one entry of each: read word by word through the constant
+// address space, the compiler loads them with scalar loads (uniform per wave).
+template <typename T> __device__ __forceinline__ T load_uniform(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "table entries are whole dwords");
+    T v;
+    const __attribute__((address_space(4))) int *q = (const __attribute__((address_space(4))) int *)(p);
+    int *d = reinterpret_cast<int *>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = q[i];
+    return v;
+}
+
+template <typename S, typename C>
 __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const BatchLat<C> &L) {
     StepArgs<C> a;
-    using R = C;  // the batch stores FP32 or FP64 populations
+    using R = C;  // the batch stores FP32 / FP64 populations or FP32 deviations (scale 1)
     a.src = static_cast<const R *>(b.src_pool) + L.off;
     a.dst = static_cast<R *>(b.dst_pool) + L.off;
     a.nx = L.nx;
@@ -88,44 +106,54 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     a.n_blocks = 0;
     a.xcd_order = 0;
     a.force = L.force;
+    if constexpr (Store<S>::dev) {  // deviation-space constants of collide_dev (2^s = 1)
+        a.om1 = (C)1 - a.omega;
+        const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            a.kw[d] = a.omega * a.w[d];
+            a.kf[d] = a.w[d] * pre;
+        }
+        a.half_force = (C)0.5 * a.force;
+    }
     return a;
 }
 
 // One step of every lattice: a persistent grid walks the row segments, one
 // wave per segment.
-template <typename C, int TW>
+template <typename S, typename C, int TW>
 __global__ void __launch_bounds__(TW) batch_step_kernel(BatchArgs<C> b) {
     constexpr int WAVES = TW / 64;
-    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     for (int t = (int)blockIdx.x * WAVES + wave; t < b.n_tiles; t += (int)gridDim.x * WAVES) {
-        const BatchLat<C> L = b.lats[b.tile_lat[t]];
-        const int local = t - L.tile_start;
-        const int y = local / L.segs_per_row;
-        const int x = (local - y * L.segs_per_row) * 64 + lane;
+        const Seg sg = load_uniform(b.segs + t);
+        const BatchLat<C> L = load_uniform(b.lats + sg.lat);
+        const int y = sg.y;
+        const int x = (t - L.tile_start - y * L.segs_per_row) * 64 + lane;
         if (x >= L.nx) continue;
-        const StepArgs<C> a = lattice_args<C>(b, L);
+        const StepArgs<C> a = lattice_args<S, C>(b, L);
         const C *f = static_cast<const C *>(a.src);
         C *g = static_cast<C *>(a.dst);
         switch (L.kind) {
-        case SW: step_cell<C, C, SW>(a, f, g, x, y); break;
-        case CF: step_cell<C, C, CF>(a, f, g, x, y); break;
-        case SL: step_cell<C, C, SL>(a, f, g, x, y); break;
-        default: step_cell<C, C, BF>(a, f, g, x, y); break;
+        case SW: step_cell<S, C, SW>(a, f, g, x, y); break;
+        case CF: step_cell<S, C, CF>(a, f, g, x, y); break;
+        case SL: step_cell<S, C, SL>(a, f, g, x, y); break;
+        default: step_cell<S, C, BF>(a, f, g, x, y); break;
         }
     }
 }
 
 // Equilibrium initial state of every lattice (as init_kernel: shear-wave
 // lattices start from their u_x(y) profile, the others at rest).
-template <typename C, int TW>
+template <typename S, typename C, int TW>
 __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
     constexpr int WAVES = TW / 64;
-    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     for (int t = (int)blockIdx.x * WAVES + wave; t < b.n_tiles; t += (int)gridDim.x * WAVES) {
-        const BatchLat<C> L = b.lats[b.tile_lat[t]];
-        const int local = t - L.tile_start;
-        const int y = local / L.segs_per_row;
-        const int x = (local - y * L.segs_per_row) * 64 + lane;
+        const Seg sg = load_uniform(b.segs + t);
+        const BatchLat<C> L = load_uniform(b.lats + sg.lat);
+        const int y = sg.y;
+        const int x = (t - L.tile_start - y * L.segs_per_row) * 64 + lane;
         if (x >= L.nx) continue;
         const int q = y * L.nx + x;
         C *g = static_cast<C *>(b.dst_pool) + L.off;
@@ -143,7 +171,7 @@ __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
             } else {
                 feq = b.w[d] * L.rho_0;
             }
-            g[(size_t)d * (size_t)L.plane + c] = feq;
+            g[(size_t)d * (size_t)L.plane + c] = Store<S>::from_f(feq, b.w[d], (C)1);
         }
         const size_t m = (size_t)L.macro_off + (size_t)q;
         b.rho_o[m] = L.rho_0;
@@ -167,7 +195,7 @@ struct ffb_batch {
     void *macro[3] = {nullptr, nullptr, nullptr};
     void *prof = nullptr;
     void *lats = nullptr;     // BatchLat<C>[count]
-    int *tile_lat = nullptr;  // per tile
+    void *segs = nullptr;     // Seg per row segment: (lattice, row)
     int n_tiles = 0;
     ffb_batch_tuning_t tuning{256, 0};
     hipStream_t stream = nullptr;
@@ -192,7 +220,7 @@ int grid_for(const ffb_batch *B) {
 template <typename C> int upload_tables(ffb_batch *B) {
     const int n = (int)B->desc.size();
     std::vector<BatchLat<C>> lats((size_t)n);
-    std::vector<int> tiles;
+    std::vector<Seg> tiles;
     // lattices grouped by kind (stable), so resident waves share their code path
     std::vector<int> order((size_t)n);
     for (int i = 0; i < n; ++i) order[(size_t)i] = i;
@@ -211,7 +239,8 @@ template <typename C> int upload_tables(ffb_batch *B) {
         L.kind = d.kind;
         L.segs_per_row = (d.n_x + 63) / 64;
         L.tile_start = (int)tiles.size();
-        tiles.insert(tiles.end(), (size_t)L.segs_per_row * (size_t)d.n_y, i);
+        for (int y = 0; y < d.n_y; ++y)
+            for (int k = 0; k < L.segs_per_row; ++k) tiles.push_back(Seg{i, y});
         L.omega = (C)d.omega;
         const C w6 = (C)1 / (C)36;
         L.corr8 = (C)6 * w6 * (C)d.rho_0 * (C)d.u_wall;  // as couette_flow.f90:118-119
@@ -220,13 +249,13 @@ template <typename C> int upload_tables(ffb_batch *B) {
         L.rho_0 = (C)d.rho_0;
     }
     if (B->lats) (void)hipFree(B->lats);
-    if (B->tile_lat) (void)hipFree(B->tile_lat);
+    if (B->segs) (void)hipFree(B->segs);
     B->lats = nullptr;
-    B->tile_lat = nullptr;
+    B->segs = nullptr;
     BHIPCHK(hipMalloc(&B->lats, lats.size() * sizeof(BatchLat<C>)));
-    BHIPCHK(hipMalloc((void **)&B->tile_lat, tiles.size() * sizeof(int)));
+    BHIPCHK(hipMalloc(&B->segs, tiles.size() * sizeof(Seg)));
     BHIPCHK(hipMemcpy(B->lats, lats.data(), lats.size() * sizeof(BatchLat<C>), hipMemcpyHostToDevice));
-    BHIPCHK(hipMemcpy(B->tile_lat, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice));
+    BHIPCHK(hipMemcpy(B->segs, tiles.data(), tiles.size() * sizeof(Seg), hipMemcpyHostToDevice));
     B->n_tiles = (int)tiles.size();
     return FFB_OK;
 }
@@ -236,7 +265,7 @@ template <typename C> BatchArgs<C> batch_args(const ffb_batch *B, int parity, in
     b.src_pool = B->pool[parity];
     b.dst_pool = B->pool[1 - parity];
     b.lats = static_cast<const BatchLat<C> *>(B->lats);
-    b.tile_lat = B->tile_lat;
+    b.segs = static_cast<const Seg *>(B->segs);
     b.n_tiles = B->n_tiles;
     b.write_macro = write_macro;
     b.rho_o = static_cast<C *>(B->macro[0]);
@@ -249,21 +278,22 @@ template <typename C> BatchArgs<C> batch_args(const ffb_batch *B, int parity, in
     return b;
 }
 
-template <typename C> void launch_step_c(const ffb_batch *B, int parity, int write_macro) {
+template <typename S, typename C> void launch_step_c(const ffb_batch *B, int parity, int write_macro) {
     const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
     const dim3 grid((unsigned)grid_for(B));
     switch (B->tuning.tile_w) {
-    case 64: hipLaunchKernelGGL((batch_step_kernel<C, 64>), grid, dim3(64), 0, B->stream, b); break;
-    case 128: hipLaunchKernelGGL((batch_step_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
-    case 512: hipLaunchKernelGGL((batch_step_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
-    case 1024: hipLaunchKernelGGL((batch_step_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
-    default: hipLaunchKernelGGL((batch_step_kernel<C, 256>), grid, dim3(256), 0, B->stream, b); break;
+    case 64: hipLaunchKernelGGL((batch_step_kernel<S, C, 64>), grid, dim3(64), 0, B->stream, b); break;
+    case 128: hipLaunchKernelGGL((batch_step_kernel<S, C, 128>), grid, dim3(128), 0, B->stream, b); break;
+    case 512: hipLaunchKernelGGL((batch_step_kernel<S, C, 512>), grid, dim3(512), 0, B->stream, b); break;
+    case 1024: hipLaunchKernelGGL((batch_step_kernel<S, C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
+    default: hipLaunchKernelGGL((batch_step_kernel<S, C, 256>), grid, dim3(256), 0, B->stream, b); break;
     }
 }
 
 void launch_step(const ffb_batch *B, int parity, int write_macro) {
-    if (B->precision == FFB_PREC_FP64) launch_step_c<double>(B, parity, write_macro);
-    else launch_step_c<float>(B, parity, write_macro);
+    if (B->precision == FFB_PREC_FP64) launch_step_c<double, double>(B, parity, write_macro);
+    else if (B->precision == FFB_PREC_FP32_DEV) launch_step_c<f32dev, float>(B, parity, write_macro);
+    else launch_step_c<float, float>(B, parity, write_macro);
 }
 
 int drop_graph(ffb_batch *B) {
@@ -288,15 +318,15 @@ int build_graph(ffb_batch *B) {
     return FFB_OK;
 }
 
-template <typename C> int init_c(ffb_batch *B) {
+template <typename S, typename C> int init_c(ffb_batch *B) {
     BatchArgs<C> b = batch_args<C>(B, 1, 0);  // dst = pool[0]
     const dim3 grid((unsigned)((B->n_tiles + B->tuning.tile_w / 64 - 1) / (B->tuning.tile_w / 64)));
     switch (B->tuning.tile_w) {
-    case 64: hipLaunchKernelGGL((batch_init_kernel<C, 64>), grid, dim3(64), 0, B->stream, b); break;
-    case 128: hipLaunchKernelGGL((batch_init_kernel<C, 128>), grid, dim3(128), 0, B->stream, b); break;
-    case 512: hipLaunchKernelGGL((batch_init_kernel<C, 512>), grid, dim3(512), 0, B->stream, b); break;
-    case 1024: hipLaunchKernelGGL((batch_init_kernel<C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
-    default: hipLaunchKernelGGL((batch_init_kernel<C, 256>), grid, dim3(256), 0, B->stream, b); break;
+    case 64: hipLaunchKernelGGL((batch_init_kernel<S, C, 64>), grid, dim3(64), 0, B->stream, b); break;
+    case 128: hipLaunchKernelGGL((batch_init_kernel<S, C, 128>), grid, dim3(128), 0, B->stream, b); break;
+    case 512: hipLaunchKernelGGL((batch_init_kernel<S, C, 512>), grid, dim3(512), 0, B->stream, b); break;
+    case 1024: hipLaunchKernelGGL((batch_init_kernel<S, C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
+    default: hipLaunchKernelGGL((batch_init_kernel<S, C, 256>), grid, dim3(256), 0, B->stream, b); break;
     }
     BHIPCHK(hipGetLastError());
     BHIPCHK(hipStreamSynchronize(B->stream));
@@ -333,8 +363,9 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
                      ffb_batch_t **out) {
     if (!lattices || !out || count < 1) return fail(FFB_E_INVALID, "ffb_batch_create: bad argument");
     *out = nullptr;
-    if (precision != FFB_PREC_FP32 && precision != FFB_PREC_FP64)
-        return fail(FFB_E_INVALID, "ffb_batch_create: precision must be FFB_PREC_FP32 or FFB_PREC_FP64");
+    if (precision != FFB_PREC_FP32 && precision != FFB_PREC_FP64 && precision != FFB_PREC_FP32_DEV)
+        return fail(FFB_E_INVALID,
+                    "ffb_batch_create: precision must be FFB_PREC_FP32, FFB_PREC_FP64 or FFB_PREC_FP32_DEV");
     for (int i = 0; i < count; ++i) {
         const ffb_batch_lattice_t &d = lattices[i];
         if (d.n_x < 1 || d.n_y < 1 || (long long)d.n_x * d.n_y > (1LL << 30))
@@ -355,16 +386,18 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     B->elem = precision == FFB_PREC_FP64 ? 8 : 4;
     B->desc.assign(lattices, lattices + count);
     long long off = 0, moff = 0, poff = 0;
+    // each lattice in the row-blocked layout of the single domain: per row y
+    // the nine direction rows (n_x rounded up to 64) one after another
     for (int i = 0; i < count; ++i) {
         const ffb_batch_lattice_t &d = lattices[i];
-        const int pitch = ((d.n_x + 63) / 64) * 64;
-        const long long plane = (long long)pitch * d.n_y;
+        const long long plane = ((d.n_x + 63) / 64) * 64;  // direction stride
+        const int pitch = (int)(9 * plane);                  // row stride
         B->pitch.push_back(pitch);
         B->plane.push_back(plane);
         B->off.push_back(off);
         B->macro_off.push_back(moff);
         B->prof_off.push_back(poff);
-        off += 9 * plane;
+        off += (long long)pitch * d.n_y;
         moff += (long long)d.n_x * d.n_y;
         poff += d.kind == FFB_SHEAR_WAVE ? d.n_y : 0;
         B->cells += (long long)d.n_x * d.n_y;
@@ -429,7 +462,7 @@ int ffb_batch_destroy(ffb_batch_t *B) {
     for (void *p : B->macro) if (p) (void)hipFree(p);
     if (B->prof) (void)hipFree(B->prof);
     if (B->lats) (void)hipFree(B->lats);
-    if (B->tile_lat) (void)hipFree(B->tile_lat);
+    if (B->segs) (void)hipFree(B->segs);
     if (B->ev_start) (void)hipEventDestroy(B->ev_start);
     if (B->ev_stop) (void)hipEventDestroy(B->ev_stop);
     if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -443,7 +476,8 @@ int ffb_batch_init(ffb_batch_t *B) {
     B->cur = 0;
     B->steps = 0;
     B->timing = false;
-    return B->precision == FFB_PREC_FP64 ? init_c<double>(B) : init_c<float>(B);
+    if (B->precision == FFB_PREC_FP64) return init_c<double, double>(B);
+    return B->precision == FFB_PREC_FP32_DEV ? init_c<f32dev, float>(B) : init_c<float, float>(B);
 }
 
 int ffb_batch_step(ffb_batch_t *B, int32_t n_steps, int32_t write_macro_last) {
@@ -569,7 +603,7 @@ int ffb_batch_read_populations(ffb_batch_t *B, int32_t index, void *f_halo) {
     BHIPCHK(hipSetDevice(B->device));
     const ffb_batch_lattice_t &d = B->desc[(size_t)index];
     const size_t plane = (size_t)B->plane[(size_t)index], pitch = (size_t)B->pitch[(size_t)index];
-    std::vector<unsigned char> stage(9 * plane * B->elem);
+    std::vector<unsigned char> stage(pitch * (size_t)d.n_y * B->elem);
     BHIPCHK(hipMemcpyAsync(stage.data(), (unsigned char *)B->pool[B->cur] + (size_t)B->off[(size_t)index] * B->elem,
                            stage.size(), hipMemcpyDeviceToHost, B->stream));
     BHIPCHK(hipStreamSynchronize(B->stream));
@@ -579,6 +613,14 @@ int ffb_batch_read_populations(ffb_batch_t *B, int32_t index, void *f_halo) {
         for (int y = 0; y < d.n_y; ++y)
             std::memcpy((unsigned char *)f_halo + ((size_t)dd * hplane + (size_t)(y + 1) * hp + 1) * B->elem,
                         stage.data() + ((size_t)dd * plane + (size_t)y * pitch) * B->elem, (size_t)d.n_x * B->elem);
+    if (B->precision == FFB_PREC_FP32_DEV) {  // stored deviations: f = w + D
+        float *f = static_cast<float *>(f_halo);
+        for (int dd = 0; dd < 9; ++dd) {
+            const float w = dd == 0 ? 4.0f / 9.0f : (dd <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f);
+            for (int y = 1; y <= d.n_y; ++y)
+                for (size_t x = 1; x <= (size_t)d.n_x; ++x) f[(size_t)dd * hplane + (size_t)y * hp + x] += w;
+        }
+    }
     return FFB_OK;
 }
 

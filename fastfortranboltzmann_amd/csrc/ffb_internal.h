@@ -17,8 +17,15 @@ int fail(int code, const std::string &msg);
 // One device-resident domain (the handle behind ffb_lattice_t).
 struct ffb_lattice {
     ffb_config_t cfg;
-    int pitch = 0;           // elements per row (n_x rounded up to 64)
-    long long plane = 0;     // elements per population plane
+    // element (d, x, y) of a population set is at d * plane + y * pitch + x:
+    //   layout 0 (planes): 9 planes of n_y rows, pitch = n_x rounded up to 64,
+    //     plane = pitch * n_y (+ plane_pad);
+    //   layout 1 (rows): per row y the 9 direction rows one after another,
+    //     plane = n_x rounded up to 64, pitch = 9 * plane.
+    int pitch = 0;           // row stride in elements
+    long long plane = 0;     // direction stride in elements
+    long long set_elems = 0; // elements of one population set
+    int layout = 0;
     size_t elem = 0;         // bytes per stored population
     size_t celem = 0;        // bytes per compute value (macro fields, strips)
     void *pop_alloc = nullptr;  // one allocation holding both population sets

@@ -55,8 +55,10 @@ program ffb_host
         cfg%precision = FFB_PREC_FP16_DEV
     case ("bf16dev")
         cfg%precision = FFB_PREC_BF16_DEV
+    case ("fp32dev")
+        cfg%precision = FFB_PREC_FP32_DEV
     case default
-        error stop "error: FFB_PRECISION must be fp32, fp64, fp16dev or bf16dev"
+        error stop "error: FFB_PRECISION must be fp32, fp64, fp16dev, bf16dev or fp32dev"
     end select
 
     if (fp64) then
@@ -179,7 +181,8 @@ contains
 
     subroutine print_run_summary()
         real(real64) :: bytes_pop, gb
-        bytes_pop = merge(8.0_real64, merge(4.0_real64, 2.0_real64, s%precision == "fp32"), fp64)
+        bytes_pop = merge(8.0_real64, merge(4.0_real64, 2.0_real64, &
+                          s%precision == "fp32" .or. s%precision == "fp32dev"), fp64)
         gb = 2.0_real64 * 9.0_real64 * bytes_pop * real(s%n_x, real64) * real(s%n_y, real64) * 1.0e-9_real64
         print '(A)', ""
         print '(A)', "--- [ hardware ] ---------------------------------------------------------------"

@@ -35,10 +35,10 @@ def default_scale_log2(precision: str, u_ref: float = 0.1, rho_dev: float = 0.05
     The bound is |f_i - w_i| <= w_0 (rho_dev + 3 u + 6 u^2) with w_0 = 4/9 the
     largest weight, u the largest expected speed, rho_dev the largest density
     deviation."""
-    if precision == "bf16dev":
+    if precision in ("bf16dev", "fp32dev"):
         return 0
     if precision != "fp16dev":
-        raise ValueError(f"not a 16-bit form: {precision}")
+        raise ValueError(f"not a deviation form: {precision}")
     d_max = (4.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref)
     return int(math.floor(math.log2(2.0 ** 14 / d_max)))
 
@@ -86,7 +86,7 @@ def compare(ref: dict, run: dict) -> dict:
 
 
 def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
-                    forms=("fp16dev", "bf16dev", "fp32"), scales: dict | None = None,
+                    forms=("fp16dev", "bf16dev", "fp32", "fp32dev"), scales: dict | None = None,
                     log=print) -> dict:
     out = {"n": n, "re": re, "u_lid": u_lid, "steps": steps,
            "omega": omega_for_reynolds(re, u_lid, n), "forms": {}}
@@ -117,7 +117,7 @@ def main(argv=None) -> int:
     ap.add_argument("--re", type=float, default=5000.0)
     ap.add_argument("--u-lid", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=50000)
-    ap.add_argument("--forms", default="fp16dev,bf16dev,fp32")
+    ap.add_argument("--forms", default="fp16dev,bf16dev,fp32,fp32dev")
     ap.add_argument("--scale", action="append", default=[], help="form=s, e.g. fp16dev=16")
     args = ap.parse_args(argv)
     scales = {k: int(v) for k, v in (x.split("=") for x in args.scale)}

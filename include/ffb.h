@@ -51,13 +51,18 @@ extern "C" {
 #define FFB_BODY_FORCE_CHANNEL 5
 
 /* storage precision of the populations.  FP64 / FP32 are the reference's
- * FFB_FP64 on / off builds (source/settings.f90:7-13).  The 16-bit forms
- * store d_i = (f_i - w_i) * 2^storage_scale_log2 and collide in FP32; they
- * have no reference counterpart. */
+ * FFB_FP64 on / off builds (source/settings.f90:7-13).  The deviation forms
+ * store D_i = (f_i - w_i) * 2^storage_scale_log2 as FP16, BF16 or FP32 and run
+ * the step on D in FP32 (boundary rules and BGK collision in deviation space,
+ * FP32 arithmetic on values of the size of the deviations); they have no
+ * reference counterpart.  FFB_PREC_FP32_DEV has the FP32 form's traffic and
+ * an error against FP64 about ten times smaller for small velocities and
+ * forces. */
 #define FFB_PREC_FP32 0
 #define FFB_PREC_FP64 1
 #define FFB_PREC_FP16_DEV 2
 #define FFB_PREC_BF16_DEV 3
+#define FFB_PREC_FP32_DEV 4
 
 /* status codes */
 #define FFB_OK 0
@@ -145,13 +150,15 @@ int ffb_step_count(ffb_lattice_t *lat, int64_t *steps);
 int ffb_moments(ffb_lattice_t *lat, double *mass, double *mom_x, double *mom_y);
 
 /* Device pointers for zero-copy interop (torch, hipGraph capture):
- * *cur = populations the next step reads, *stream = the handle's hipStream_t,
- * *pitch = row pitch in elements, *plane = plane stride in elements. */
+ * *cur = populations the next step reads, *stream = the handle's hipStream_t;
+ * population d of cell (x, y) is element d * plane + y * pitch + x of *cur
+ * (*pitch = row stride, *plane = direction stride, in elements). */
 int ffb_device_info(ffb_lattice_t *lat, void **cur, void **stream, int64_t *pitch, int64_t *plane);
 
 /* Measurement helper with no reference counterpart: device-to-device copy of
  * `bytes` bytes by plain HIP copy kernels (16-byte loads and stores; four
- * launch shapes, the fastest counts), repeated `iters` times on `device`;
+ * variants -- grid-stride, one element per thread with plain, nontemporal-store
+ * and nontemporal load+store -- the fastest counts), repeated `iters` times on `device`;
  * *gbps = (read + written bytes) / time.  Gives the measured HBM ceiling the
  * step kernel is compared with. */
 int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);

@@ -38,12 +38,7 @@
 #undef NAME
 #undef SIN_FN
 
-/* ---- 16-bit scaled-deviation storage (the precision form; no reference
- * counterpart).  The GPU stores d_i = (f_i - w_i) * 2^s rounded to FP16 or
- * BF16 (round to nearest, ties to even) and decodes f_i = w_i + d_i * 2^-s in
- * FP32 when it loads.  The oracle runs the FP32 path and, after every step,
- * rounds each owned population through the same encode/decode, which gives
- * the values the GPU decodes at its next load. ---- */
+/* ---- 16-bit conversions of the deviation forms (below). ---- */
 
 /* IEEE binary16 from binary32, round to nearest even (subnormals, overflow to inf). */
 unsigned short ffbo_float_to_half(float f)
@@ -112,46 +107,252 @@ float ffbo_bf16_to_float(unsigned short b)
     return f;
 }
 
-/* Rounds the owned populations of a float halo array through the stored
- * form: kind 1 = FP16, 2 = BF16 deviations with scale 2^scale_log2.  The
- * float operations are the GPU's: (f - w) * 2^s on store, w + d * 2^-s on load. */
-void ffbo_s_quantize(int kind, int scale_log2, int nx, int ny, float *f)
+/* ---- Deviation forms (the precision form; no reference counterpart) ----
+ * The GPU stores D_i = (f_i - w_i) * 2^s -- as FP16 or BF16 (kind 1, 2;
+ * round to nearest even) or unrounded FP32 (kind 0) -- and works on D
+ * throughout a step: the halo rules act on D (copies and bounce-back are the
+ * same, since w_i = w_opp(i); the moving-wall and pressure increments are
+ * scaled by 2^s), and the collision runs in deviation space with the operation
+ * order below.  This restates that algorithm on the halo arrays of the
+ * reference form; the state array holds D (float). */
+
+static float q_round(int kind, float v)
 {
-    const size_t pitch = (size_t)(nx + 2), plane = pitch * (size_t)(ny + 2);
-    const float scale = ldexpf(1.0f, scale_log2), inv = ldexpf(1.0f, -scale_log2);
+    if (kind == 1) return ffbo_half_to_float(ffbo_float_to_half(v));
+    if (kind == 2) return ffbo_bf16_to_float(ffbo_float_to_bf16(v));
+    return v;
+}
+
+/* Stored form of the populations of a float halo array (initial state):
+ * D = round((f - w) * 2^s), every entry. */
+void ffbo_q_encode(int kind, int scale_log2, int nx, int ny, const float *f, float *D)
+{
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+    const float scale = ldexpf(1.0f, scale_log2);
     for (int i = 1; i <= 9; ++i) {
         const float w = ffbo_s_w(i);
-        for (int y = 1; y <= ny; ++y) {
-            float *row = f + (size_t)(i - 1) * plane + (size_t)y * pitch;
-            for (int x = 1; x <= nx; ++x) {
-                const float d = (row[x] - w) * scale;
-                const float back = kind == 1 ? ffbo_half_to_float(ffbo_float_to_half(d))
-                                             : ffbo_bf16_to_float(ffbo_float_to_bf16(d));
-                row[x] = w + back * inv;
-            }
+        for (size_t k = 0; k < plane; ++k) {
+            const size_t j = (size_t)(i - 1) * plane + k;
+            D[j] = q_round(kind, (f[j] - w) * scale);
         }
     }
 }
 
-/* n_steps of the FP32 loop with the 16-bit storage rounding after every step
- * (kind as in ffbo_s_quantize).  Final state in f_a. */
-void ffbo_s_run_quantized(int kind, int scale_log2, int mode, int nx, int ny, int n_steps,
-                          float omega, float rho_0, float u_wall, float u_lid, float rho_in,
-                          float rho_out, float *f_a, float *f_b, float *pf_state,
-                          int write_macro_last, float *rho, float *u_x, float *u_y, float force)
+/* Populations the GPU returns for stored D: f = w + D * 2^-s in FP32. */
+void ffbo_q_decode(int scale_log2, int nx, int ny, const float *D, float *f)
+{
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+    const float inv = ldexpf(1.0f, -scale_log2);
+    for (int i = 1; i <= 9; ++i) {
+        const float w = ffbo_s_w(i);
+        for (size_t k = 0; k < plane; ++k) {
+            const size_t j = (size_t)(i - 1) * plane + k;
+            f[j] = w + D[j] * inv;
+        }
+    }
+}
+
+#define FQ(f, x, y, i) (f)[(size_t)((i) - 1) * plane + (size_t)(y) * (size_t)pitch + (size_t)(x)]
+
+/* The pressure increment of pressure_periodic_distribution
+ * (modes/poiseuille_flow.f90:189-194) times 2^s, added to a stored D. */
+static float q_ppd(int i, float D_src, float rho_src, float ux_src, float uy_src, float rho_b, float scale)
+{
+    static const int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static const int cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    float usq = ux_src * ux_src + uy_src * uy_src;
+    float cu = (float)cx[i - 1] * ux_src + (float)cy[i - 1] * uy_src;
+    float pf = ((1.0f + 3.0f * cu) + 4.5f * cu * cu) - 1.5f * usq;
+    return D_src + (ffbo_s_w(i) * (rho_b - rho_src) * pf) * scale;
+}
+
+/* prepare_poiseuille_flow_halos_PF (modes/poiseuille_flow.f90:9-89) on D. */
+static void q_halos_pf(int nx, int ny, float rho_in, float rho_out, const float *ml, const float *mr,
+                       float scale, float *f)
+{
+    const long pitch = nx + 2;
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+#define QL(y, k) ml[(size_t)((k) - 1) * (size_t)ny + (size_t)((y) - 1)]
+#define QR(y, k) mr[(size_t)((k) - 1) * (size_t)ny + (size_t)((y) - 1)]
+    ffbo_s_bb_bottom(nx, ny, f);
+    ffbo_s_bb_top(nx, ny, 0, 0, 0, f);
+    for (int y = 1; y <= ny; ++y) {
+        FQ(f, 0, y, 2) = q_ppd(2, FQ(f, 0, y, 2), QL(y, 1), QL(y, 2), QL(y, 3), rho_in, scale);
+        FQ(f, 0, y, 6) = q_ppd(6, FQ(f, 0, y, 6), QL(y, 1), QL(y, 2), QL(y, 3), rho_in, scale);
+        FQ(f, 0, y, 9) = q_ppd(9, FQ(f, 0, y, 9), QL(y, 1), QL(y, 2), QL(y, 3), rho_in, scale);
+    }
+    FQ(f, 0, 0, 6) = q_ppd(6, FQ(f, 0, 0, 6), QL(1, 1), QL(1, 2), QL(1, 3), rho_in, scale);
+    FQ(f, 0, ny + 1, 9) = q_ppd(9, FQ(f, 0, ny + 1, 9), QL(ny, 1), QL(ny, 2), QL(ny, 3), rho_in, scale);
+    for (int y = 1; y <= ny; ++y) {
+        FQ(f, nx + 1, y, 4) = q_ppd(4, FQ(f, nx + 1, y, 4), QR(y, 1), QR(y, 2), QR(y, 3), rho_out, scale);
+        FQ(f, nx + 1, y, 7) = q_ppd(7, FQ(f, nx + 1, y, 7), QR(y, 1), QR(y, 2), QR(y, 3), rho_out, scale);
+        FQ(f, nx + 1, y, 8) = q_ppd(8, FQ(f, nx + 1, y, 8), QR(y, 1), QR(y, 2), QR(y, 3), rho_out, scale);
+    }
+    FQ(f, nx + 1, 0, 7) = q_ppd(7, FQ(f, nx + 1, 0, 7), QR(1, 1), QR(1, 2), QR(1, 3), rho_out, scale);
+    FQ(f, nx + 1, ny + 1, 8) = q_ppd(8, FQ(f, nx + 1, ny + 1, 8), QR(ny, 1), QR(ny, 2), QR(ny, 3), rho_out, scale);
+#undef QL
+#undef QR
+}
+
+/* Pulled D of cell (x, y), v[d] = D_{d+1}(x - c). */
+static void q_pull(const float *f, long pitch, size_t plane, int x, int y, float v[9])
+{
+    v[0] = FQ(f, x, y, 1);
+    v[1] = FQ(f, x - 1, y, 2);
+    v[2] = FQ(f, x, y - 1, 3);
+    v[3] = FQ(f, x + 1, y, 4);
+    v[4] = FQ(f, x, y + 1, 5);
+    v[5] = FQ(f, x - 1, y - 1, 6);
+    v[6] = FQ(f, x + 1, y - 1, 7);
+    v[7] = FQ(f, x + 1, y + 1, 8);
+    v[8] = FQ(f, x - 1, y + 1, 9);
+}
+
+/* update_poiseuille_flow_macro_strips_PF (modes/poiseuille_flow.f90:92-167)
+ * on D: rho = 1 + 2^-s Sum D, u = 2^-s Sum c D / rho, with the pair sums of
+ * the deviation collision. */
+static void q_macro_strip(int nx, int ny, int x, const float *f, float inv, float *strip)
+{
+    const long pitch = nx + 2;
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+    for (int y = 1; y <= ny; ++y) {
+        float v[9];
+        q_pull(f, pitch, plane, x, y, v);
+        float a13 = v[1] + v[3], a24 = v[2] + v[4], a57 = v[5] + v[7], a68 = v[6] + v[8];
+        float b13 = v[1] - v[3], b24 = v[2] - v[4], b57 = v[5] - v[7], b68 = v[6] - v[8];
+        float rho = 1.0f + (((v[0] + a13) + a24) + (a57 + a68)) * inv;
+        strip[(size_t)(y - 1)] = rho;
+        strip[(size_t)ny + (size_t)(y - 1)] = (((b13 + b57) - b68) * inv) / rho;
+        strip[2 * (size_t)ny + (size_t)(y - 1)] = (((b24 + b57) + b68) * inv) / rho;
+    }
+}
+
+/* Collision in deviation space.  With Sum w = 1 and Sum w c = 0:
+ * rho - 1 = 2^-s Sum D, rho u = 2^-s Sum c D (+ F/2), and
+ * 2^s (f_eq_i - w_i) = 2^s w_i ((rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)),
+ * so D'_i = (1 - omega) D_i + (omega w_i 2^s) inner_i, computed pairwise for
+ * opposite directions; body force: plus the Guo source
+ * 2^s w_i (1 - omega/2) [3 (c_i - u) + 9 (c_i.u) c_i] . F. */
+static void q_kernel(int kind, int guo, int nx, int ny, int write_macro, float omega, float force,
+                     float scale, float inv, const float *f, float *f_next,
+                     float *rho_o, float *ux_o, float *uy_o)
+{
+    const long pitch = nx + 2;
+    const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
+    float kw[9], kf[9];
+    const float om1 = 1.0f - omega;
+    const float pre = (1.0f - 0.5f * omega) * force;
+    const float half_force = 0.5f * force;
+    for (int d = 0; d < 9; ++d) {
+        kw[d] = (omega * ffbo_s_w(d + 1)) * scale;
+        kf[d] = (ffbo_s_w(d + 1) * pre) * scale;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int y = 1; y <= ny; ++y) {
+        for (int x = 1; x <= nx; ++x) {
+            float D[9];
+            q_pull(f, pitch, plane, x, y, D);
+            float a13 = D[1] + D[3], a24 = D[2] + D[4], a57 = D[5] + D[7], a68 = D[6] + D[8];
+            float b13 = D[1] - D[3], b24 = D[2] - D[4], b57 = D[5] - D[7], b68 = D[6] - D[8];
+            float m0 = ((D[0] + a13) + a24) + (a57 + a68);
+            float jx = (b13 + b57) - b68;
+            float jy = (b24 + b57) + b68;
+            float drho = m0 * inv;
+            float rho = 1.0f + drho;
+            float r = 1.0f / rho;
+            float ux = (guo ? fmaf(jx, inv, half_force) : jx * inv) * r;
+            float uy = (jy * inv) * r;
+            if (write_macro) {
+                size_t c = (size_t)(y - 1) * (size_t)nx + (size_t)(x - 1);
+                rho_o[c] = rho;
+                ux_o[c] = ux;
+                uy_o[c] = uy;
+            }
+            float usq = fmaf(ux, ux, uy * uy);
+            float base = fmaf(-(1.5f * usq), rho, drho);
+            float hr = 0.5f * rho;
+            float h1 = 3.0f * ux, h2 = 3.0f * uy;
+            float h5 = h1 + h2, h8 = h1 - h2;
+            static const int pd[4] = {1, 2, 5, 8}, pe[4] = {3, 4, 7, 6};
+            float hs[4] = {h1, h2, h5, h8};
+            D[0] = fmaf(om1, D[0], kw[0] * base);
+            for (int k = 0; k < 4; ++k) {
+                const int d = pd[k], e = pe[k];
+                float p = fmaf(hr, hs[k] * hs[k], base);
+                float q = rho * hs[k];
+                D[d] = fmaf(om1, D[d], kw[d] * (p + q));
+                D[e] = fmaf(om1, D[e], kw[e] * (p - q));
+            }
+            if (guo) {
+                float g = 3.0f * ux;
+                D[0] = fmaf(kf[0], -g, D[0]);
+                D[2] = fmaf(kf[2], -g, D[2]);
+                D[4] = fmaf(kf[4], -g, D[4]);
+                D[1] = fmaf(kf[1], (3.0f * h1 + 3.0f) - g, D[1]);
+                D[3] = fmaf(kf[3], (3.0f * h1 - 3.0f) - g, D[3]);
+                D[5] = fmaf(kf[5], (3.0f * h5 + 3.0f) - g, D[5]);
+                D[7] = fmaf(kf[7], (3.0f * h5 - 3.0f) - g, D[7]);
+                D[8] = fmaf(kf[8], (3.0f * h8 + 3.0f) - g, D[8]);
+                D[6] = fmaf(kf[6], (3.0f * h8 - 3.0f) - g, D[6]);
+            }
+            for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round(kind, D[d]);
+        }
+    }
+}
+
+/* One step of the deviation form: the driver order of ffbo_s_step (exchange,
+ * PF pre-pass and strips, mode halo prep, kernel) on D. */
+void ffbo_q_step(int kind, int scale_log2, int mode, int nx, int ny, float omega, float rho_0,
+                 float u_wall, float u_lid, float rho_in, float rho_out, float *f, float *f_next,
+                 float *pf_state, int write_macro, float *rho, float *u_x, float *u_y, float force)
+{
+    const float scale = ldexpf(1.0f, scale_log2), inv = ldexpf(1.0f, -scale_log2);
+    ffbo_s_exchange_1domain(mode, nx, ny, f);
+    if (mode == 3) {
+        size_t s = (size_t)ny * 3;
+        float *recv_l = pf_state, *recv_r = pf_state + s;
+        float *win_l = pf_state + 2 * s, *win_r = pf_state + 3 * s;
+        q_halos_pf(nx, ny, rho_in, rho_out, recv_l, recv_r, scale, f);
+        q_macro_strip(nx, ny, 1, f, inv, win_l);
+        q_macro_strip(nx, ny, nx, f, inv, win_r);
+        memcpy(recv_l, win_r, s * sizeof(float));
+        memcpy(recv_r, win_l, s * sizeof(float));
+    }
+    switch (mode) {
+    case 1: ffbo_s_halos_sw(nx, ny, f); break;
+    /* ((6 w_6) rho_0)(u 2^s) = ((6 w_6) rho_0 u) 2^s exactly: scaled moving-wall term */
+    case 2: ffbo_s_halos_cf(nx, ny, rho_0, u_wall * scale, f); break;
+    case 3: q_halos_pf(nx, ny, rho_in, rho_out, pf_state, pf_state + (size_t)ny * 3, scale, f); break;
+    case 5:
+        ffbo_s_bb_bottom(nx, ny, f);
+        ffbo_s_bb_top(nx, ny, 0, 0, 0, f);
+        break;
+    default: ffbo_s_halos_sl(nx, ny, rho_0, u_lid * scale, f); break;
+    }
+    q_kernel(kind, mode == 5, nx, ny, write_macro, omega, force, scale, inv, f, f_next, rho, u_x, u_y);
+}
+
+/* n_steps of the deviation form with ping-pong buffers; final D in f_a. */
+void ffbo_q_run(int kind, int scale_log2, int mode, int nx, int ny, int n_steps, float omega,
+                float rho_0, float u_wall, float u_lid, float rho_in, float rho_out, float *f_a,
+                float *f_b, float *pf_state, int write_macro_last, float *rho, float *u_x, float *u_y,
+                float force)
 {
     float *cur = f_a, *nxt = f_b;
     for (int step = 1; step <= n_steps; ++step) {
         int wm = write_macro_last && step == n_steps;
-        ffbo_s_step(mode, nx, ny, omega, rho_0, u_wall, u_lid, rho_in, rho_out, cur, nxt,
+        ffbo_q_step(kind, scale_log2, mode, nx, ny, omega, rho_0, u_wall, u_lid, rho_in, rho_out, cur, nxt,
                     pf_state, wm, rho, u_x, u_y, force);
-        ffbo_s_quantize(kind, scale_log2, nx, ny, nxt);
         float *t = cur;
         cur = nxt;
         nxt = t;
     }
     if (cur != f_a) memcpy(f_a, cur, sizeof(float) * (size_t)(nx + 2) * (size_t)(ny + 2) * 9);
 }
+
+#undef FQ
 
 /* Thread count for the kernel loop (1 = the reference's one thread per image). */
 void ffbo_set_threads(int n)

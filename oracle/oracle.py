@@ -53,12 +53,14 @@ def lib() -> ctypes.CDLL:
             f = getattr(_lib, f"ffbo_{p}_pf_seed")
             f.argtypes = [i, real, real, ptr]
             f.restype = None
-        f = _lib.ffbo_s_run_quantized
+        f = _lib.ffbo_q_run
         f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_int] * 4 + [ctypes.c_float] * 6 + \
             [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_float]
         f.restype = None
-        _lib.ffbo_s_quantize.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p]
-        _lib.ffbo_s_quantize.restype = None
+        _lib.ffbo_q_encode.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2
+        _lib.ffbo_q_encode.restype = None
+        _lib.ffbo_q_decode.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+        _lib.ffbo_q_decode.restype = None
         _lib.ffbo_float_to_half.argtypes = [ctypes.c_float]
         _lib.ffbo_float_to_half.restype = ctypes.c_ushort
         _lib.ffbo_half_to_float.argtypes = [ctypes.c_ushort]
@@ -92,15 +94,18 @@ def _p(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
-STORAGE_KINDS = {"fp16dev": 1, "bf16dev": 2}
+STORAGE_KINDS = {"fp32dev": 0, "fp16dev": 1, "bf16dev": 2}
 
 
 class OracleLattice:
     """One single-domain run of the reference loop on the CPU.
 
-    ``storage`` = "fp16dev" / "bf16dev" emulates the GPU's 16-bit scaled
-    deviation storage (FP32 arithmetic, populations rounded through
-    (f - w) * 2^scale_log2 after every step); it has no reference counterpart."""
+    ``storage`` = "fp32dev" / "fp16dev" / "bf16dev" restates the GPU's
+    deviation forms (no reference counterpart): the state is D = (f - w) *
+    2^scale_log2 stored as FP32 / FP16 / BF16, every step works on D in FP32
+    (deviation-space halo rules and collision, ffbo_q_step), and ``f`` holds
+    the populations decoded as the GPU returns them, w + D * 2^-scale_log2 in
+    FP32."""
 
     def __init__(self, mode: int, nx: int, ny: int, params: Params | None = None,
                  dtype=np.float64, storage: str | None = None, scale_log2: int = 0):
@@ -119,6 +124,9 @@ class OracleLattice:
         self.u_x = np.zeros_like(self.rho)
         self.u_y = np.zeros_like(self.rho)
         self.pf_state = np.zeros((4, 3, ny), self.dtype)
+        if storage is not None:
+            self.d = np.zeros_like(self.f)
+            self.d_b = np.zeros_like(self.f)
         self.steps = 0
 
     def init(self) -> "OracleLattice":
@@ -128,19 +136,21 @@ class OracleLattice:
             _p(self.f), _p(self.rho), _p(self.u_x), _p(self.u_y))
         getattr(lib(), f"ffbo_{self.p}_pf_seed")(self.ny, P.rho_in, P.rho_out, _p(self.pf_state))
         if self.storage is not None:
-            lib().ffbo_s_quantize(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
-                                  _p(self.f))
+            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
+                                _p(self.f), _p(self.d))
+            self._decode()
         self.steps = 0
         return self
 
     def run(self, n_steps: int, write_macro_last: bool = True) -> "OracleLattice":
         P = self.params
         if self.storage is not None:
-            lib().ffbo_s_run_quantized(
+            lib().ffbo_q_run(
                 STORAGE_KINDS[self.storage], self.scale_log2, self.mode, self.nx, self.ny,
                 int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
-                _p(self.f), _p(self.f_b), _p(self.pf_state), 1 if write_macro_last else 0,
+                _p(self.d), _p(self.d_b), _p(self.pf_state), 1 if write_macro_last else 0,
                 _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
+            self._decode()
             self.steps += int(n_steps)
             return self
         getattr(lib(), f"ffbo_{self.p}_run")(
@@ -149,6 +159,9 @@ class OracleLattice:
             1 if write_macro_last else 0, _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
         self.steps += int(n_steps)
         return self
+
+    def _decode(self) -> None:
+        lib().ffbo_q_decode(self.scale_log2, self.nx, self.ny, _p(self.d), _p(self.f))
 
     @property
     def interior(self) -> np.ndarray:

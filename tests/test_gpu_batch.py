@@ -30,17 +30,19 @@ def small_batch():
 def oracle_of(L, dtype, steps):
     p = Params(omega=L.omega, rho_0=L.rho_0, u_max=L.u_max, n_sin=L.n_sin, u_wall=L.u_wall,
                u_lid=L.u_wall, force=L.body_force)
+    if dtype == "fp32dev":  # FP32 deviations, scale 1
+        return OracleLattice(L.kind, L.n_x, L.n_y, p, storage="fp32dev").init().run(steps)
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev"])
 @pytest.mark.parametrize("tile_w,grid", [(256, 0), (128, 8), (512, 3), (1024, 0)])
 def test_batch_matches_oracle_per_lattice(precision, tile_w, grid):
     from fastfortranboltzmann_amd.batch import Batch
 
     print(f"[gpu] batch {precision} tile_w={tile_w} grid={grid}", flush=True)
     lats = small_batch()
-    dt = np.float64 if precision == "fp64" else np.float32
+    dt = {"fp64": np.float64, "fp32": np.float32, "fp32dev": "fp32dev"}[precision]
     with Batch(lats, precision) as b:
         b.set_tuning(tile_w, grid)
         b.init()

@@ -82,6 +82,40 @@ def test_gpu_matches_oracle_small_shapes(shape, mode, precision, vec, monkeypatc
         np.testing.assert_array_equal(uy, o.u_y)
 
 
+@pytest.mark.parametrize("precision,storage_scale", [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 16),
+                                                     ("bf16dev", 0)])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [(1, 1), (2, 7), (65, 17), (300, 7), (1030, 5)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("vec", [1, 4], ids=["scalar", "vec4"])
+def test_plane_layout_matches_oracle(shape, mode, precision, storage_scale, vec, monkeypatch):
+    """Layout 0 (nine planes; the default is the row-blocked layout 1) gives
+    the same results: bitwise equal to the oracle."""
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    monkeypatch.setenv("FFB_LAYOUT", "0")
+    monkeypatch.setenv("FFB_VEC", str(vec))
+    nx, ny = shape
+    print(f"[gpu] plane layout {nx}x{ny} mode {mode} {precision}", flush=True)
+    p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002,
+               rho_out=0.998, force=2e-4 if mode == 5 else 0.0)
+    steps = 13
+    if precision in ("fp64", "fp32"):
+        o = OracleLattice(mode, nx, ny, p, np.float64 if precision == "fp64" else np.float32)
+    else:
+        o = OracleLattice(mode, nx, ny, p, storage=precision, scale_log2=storage_scale)
+    o.init().run(steps)
+    s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=p.omega, U_MAX=p.u_max, N_SIN=p.n_sin,
+                 U_WALL=p.u_wall, U_LID=p.u_lid, RHO_IN=p.rho_in, RHO_OUT=p.rho_out, BODY_FORCE=p.force,
+                 precision=precision, storage_scale_log2=storage_scale)
+    with Lattice(s) as lat:
+        lat.init().step(steps, write_macro_last=True)
+        np.testing.assert_array_equal(lat.interior(), o.interior)
+        rho, ux, uy = lat.macros()
+        np.testing.assert_array_equal(rho, o.rho)
+        np.testing.assert_array_equal(ux, o.u_x)
+        np.testing.assert_array_equal(uy, o.u_y)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_gpu_matches_oracle_from_random_state(mode, precision):

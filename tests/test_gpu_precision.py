@@ -1,8 +1,9 @@
 """The 16-bit scaled-deviation storage form on the GPU.
 
-  * bitwise against the oracle's emulation of the same storage (FP32
-    arithmetic, every population rounded through (f - w) * 2^s to FP16/BF16
-    after each step; the conversions are pinned to numpy's float16 in
+  * bitwise against the oracle's restatement of the same form (state
+    D = (f - w) * 2^s rounded to FP16/BF16, deviation-space halo rules and
+    collision in FP32, ffbo_q_step; the conversions are pinned to numpy's
+    float16 and the deviation algebra to the FP64 oracle in
     tests/test_oracle_storage16.py), every mode, odd shapes;
   * error of the velocity field against an FP64 run of the same cavity,
     with the bound written in the test.
@@ -17,8 +18,9 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(13, 9), (65, 17), (300, 7), (1030, 5)]
 
 
-@pytest.mark.parametrize("storage,scale", [("fp16dev", 16), ("fp16dev", 11), ("bf16dev", 0)])
-@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("storage,scale", [("fp16dev", 16), ("fp16dev", 11), ("bf16dev", 0), ("fp32dev", 0),
+                                           ("fp32dev", 3)])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
 @pytest.mark.parametrize("vec", [1, 2, 8])
 def test_16bit_storage_matches_oracle_emulation(shape, mode, storage, scale, vec, monkeypatch):
@@ -27,11 +29,12 @@ def test_16bit_storage_matches_oracle_emulation(shape, mode, storage, scale, vec
     monkeypatch.setenv("FFB_VEC", str(vec))
     nx, ny = shape
     print(f"[gpu] storage {storage} s={scale} {nx}x{ny} mode {mode}", flush=True)
-    p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002, rho_out=0.998)
+    p = Params(omega=1.37, u_max=0.07, n_sin=1.0, u_wall=0.08, u_lid=0.09, rho_in=1.002, rho_out=0.998,
+               force=2e-4 if mode == 5 else 0.0)
     steps = 17
     o = OracleLattice(mode, nx, ny, p, storage=storage, scale_log2=scale).init().run(steps)
     s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=p.omega, U_MAX=p.u_max, N_SIN=p.n_sin,
-                 U_WALL=p.u_wall, U_LID=p.u_lid, RHO_IN=p.rho_in, RHO_OUT=p.rho_out,
+                 U_WALL=p.u_wall, U_LID=p.u_lid, RHO_IN=p.rho_in, RHO_OUT=p.rho_out, BODY_FORCE=p.force,
                  precision=storage, storage_scale_log2=scale)
     with Lattice(s) as lat:
         lat.init().step(steps, write_macro_last=True)

@@ -1,7 +1,10 @@
-"""Pins of the oracle's 16-bit storage emulation (the precision form): the
-binary16 conversions against numpy's float16 (an independent IEEE
-implementation, round to nearest even), the bfloat16 rounding against its
-definition, and the FP16 run against the FP32 run it perturbs."""
+"""Pins of the oracle's deviation forms (the precision form): the binary16
+conversions against numpy's float16 (an independent IEEE implementation,
+round to nearest even), the bfloat16 rounding against its definition, the
+deviation-space algebra (halo rules and collision on D = (f - w) 2^s) against
+the FP64 restatement of the reference in every mode -- unrounded FP32
+deviations must land within FP32 rounding of it, so a dropped term, a wrong
+sign or a swapped direction fails -- and the 16-bit runs against FP64."""
 import numpy as np
 import pytest
 
@@ -60,3 +63,55 @@ def test_16bit_run_tracks_fp32(storage, scale, tol):
     b = OracleLattice(4, 24, 20, p, storage=storage, scale_log2=scale).init().run(200)
     err = np.linalg.norm(b.u_x - a.u_x) / np.linalg.norm(a.u_x)
     assert 0 < err < tol
+
+
+MODE_PARAMS = {
+    1: Params(omega=1.3, u_max=0.05, n_sin=1.0),
+    2: Params(omega=0.9, u_wall=0.08),
+    3: Params(omega=1.2, rho_in=1.002, rho_out=0.998),
+    4: Params(omega=1.7, u_lid=0.1),
+    5: Params(omega=1.1, force=1e-4),
+}
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+def test_fp32_deviation_form_matches_fp64_within_rounding(mode):
+    """Unrounded FP32 deviations after 300 steps: max |f - f64| below 4e-7
+    (the plain FP32 path reaches 4e-7 .. 1.5e-6 here) and the velocity within
+    3e-5 relative L2; measured 2e-8 .. 3.4e-7."""
+    p = MODE_PARAMS[mode]
+    ref = OracleLattice(mode, 40, 30, p, np.float64).init().run(300)
+    dev = OracleLattice(mode, 40, 30, p, storage="fp32dev").init().run(300)
+    err = np.abs(dev.interior.astype(np.float64) - ref.interior).max()
+    assert err < 4e-7, err
+    uerr = np.linalg.norm(dev.u_x.astype(np.float64) - ref.u_x) / np.linalg.norm(ref.u_x)
+    assert uerr < 3e-5, uerr
+    rerr = np.abs(dev.rho.astype(np.float64) - ref.rho).max()
+    assert rerr < 2e-6, rerr
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("storage,scale,tol", [("fp16dev", 16, 5e-5), ("bf16dev", 0, 2e-3)])
+def test_16bit_deviation_forms_track_fp64(mode, storage, scale, tol):
+    """max |f - f64| after 300 steps within a few storage roundings of the
+    deviations (FP16 7e-6 .. 2.4e-5, BF16 2e-4 .. 9.5e-4 measured)."""
+    p = MODE_PARAMS[mode]
+    ref = OracleLattice(mode, 40, 30, p, np.float64).init().run(300)
+    q = OracleLattice(mode, 40, 30, p, storage=storage, scale_log2=scale).init().run(300)
+    err = np.abs(q.interior.astype(np.float64) - ref.interior).max()
+    assert 0 < err < tol, err
+
+
+def test_deviation_encode_decode_round_trip():
+    """encode/decode of the initial state: |f - decode(encode(f))| within half
+    a storage ulp of the deviation (times 2^-s), exact for FP32 at rest."""
+    p = Params(omega=1.0, u_max=0.08, n_sin=1.0)
+    f32 = OracleLattice(1, 16, 12, p, np.float32).init()
+    for storage, scale, rel in (("fp16dev", 12, 2.0 ** -11), ("bf16dev", 0, 2.0 ** -8)):
+        q = OracleLattice(1, 16, 12, p, storage=storage, scale_log2=scale).init()
+        w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, np.float32)[:, None, None]
+        dev = np.abs(f32.interior - w)
+        err = np.abs(q.interior - f32.interior)
+        assert np.all(err <= dev * rel + 1e-7)
+    rest = OracleLattice(4, 8, 8, Params(), storage="fp32dev").init()
+    assert np.all(rest.d[:, 1:-1, 1:-1] == 0)
