@@ -146,20 +146,23 @@ template <typename C> struct StepArgs {
     C half_force;          // BF: force / 2
     int n_tiles_x;         // tiles per row
     int n_blocks;          // grid size (for the XCD-aware tile order)
-    int xcd_order;         // 1: give each XCD a contiguous band of tiles
+    int xcd_group;         // G > 0: G consecutive tiles per XCD (tile_of_block)
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
-// with its own L2.  With xcd_order, XCD k gets the contiguous tile range
-// [k*q + min(k, r), ...) (q = n/8, r = n%8), so tiles that share cache lines
-// (x-neighbours) are fetched by one L2.  Measured on the 4096^2 cavity this
-// is 10 % slower than the plain order (the eight bands spread the concurrent
-// HBM accesses over distant rows; traffic is already 1.0x algorithmic), so it
-// is off by default.
-__device__ __forceinline__ int tile_of_block(int p, int n, int xcd_order) {
-    if (!xcd_order) return p;
-    const int q = n >> 3, r = n & 7, k = p & 7, j = p >> 3;
-    return k * q + (k < r ? k : r) + j;
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup p to XCD p mod 8), each XCD with its own L2.  With G > 0 the
+// workgroups p of one chunk of 8G map to tiles so that XCD j takes the G
+// consecutive tiles [jG, (j+1)G) of the chunk: x-neighbouring tiles, which
+// share the cache lines at their common edge, then hit one L2, while the eight
+// XCDs still work on neighbouring tiles.  The last, partial chunk keeps the
+// plain order.  G = 0: tile = workgroup index.
+__device__ __forceinline__ int tile_of_block(int p, int n, int G) {
+    if (G <= 0) return p;
+    const int chunk = 8 * G;
+    const int c = p / chunk;
+    if ((c + 1) * chunk > n) return p;
+    const int r = p - c * chunk;
+    return c * chunk + (r & 7) * G + (r >> 3);
 }
 
 template <typename S, typename C>
@@ -428,7 +431,7 @@ template <typename S, typename C, int MODE, int TW, int V>
 __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     using R = typename Store<S>::R;
     typedef R RV __attribute__((ext_vector_type(V)));
-    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_order);
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
     const int y = t / a.n_tiles_x;
     const int x0 = ((t - y * a.n_tiles_x) * TW + (int)threadIdx.x) * V;
     if (x0 >= a.nx || y >= a.ny) return;

@@ -115,7 +115,7 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     }
     a.half_force = (C)0.5 * a.force;
     a.n_blocks = a.n_tiles_x * c.n_y;
-    a.xcd_order = L->xcd_order;
+    a.xcd_group = L->xcd_group;
     return a;
 }
 
@@ -368,12 +368,16 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     //     few neighbouring DRAM regions instead of 18 distant ones --
     //     FP32 4096^2 193 against 206, FP64 393 against 428, FP16 8192^2 424
     //     against 473, BF16 451 against 491;
-    //   one cell per thread: FP32/FP64 with 256-/512-thread tiles (FP32 197,
-    //     FP64 410 with the other width); 16-bit forms with 256-thread tiles
-    //     (2 or 4 cells per thread in 4- or 8-byte vectors 440-489);
-    //   dispatch-order tiles: round-robin over the XCDs keeps all of them on
-    //     neighbouring rows (202 against 222 with XCD-banded tiles, planes).
-    // FFB_LAYOUT, FFB_TILE_W, FFB_VEC (cells per thread), FFB_XCD_ORDER,
+    //   one cell per thread, 256-thread tiles (FP32 197 with 512; 16-bit forms
+    //     440-489 with 2 or 4 cells per thread in 4- or 8-byte vectors);
+    //   XCD groups (tile_of_block): consecutive tiles of two rows per XCD for
+    //     the 16-bit forms (FP16 390 against 424 in dispatch order, BF16 411
+    //     against 452: a wave reads one 128-byte line per direction, so the
+    //     line shared with the neighbouring tile is a large share of the
+    //     traffic when that tile runs on another XCD's L2), half a row for
+    //     FP64 (389 against 393-411), dispatch order for FP32 (193 against
+    //     194-199 with groups of 16-32 tiles).
+    // FFB_LAYOUT, FFB_TILE_W, FFB_VEC (cells per thread), FFB_XCD_GROUP,
     // FFB_PLANE_PAD, FFB_BUF_PAD override them.
     auto env_ll = [](const char *name, long long dflt) {
         const char *v = std::getenv(name);
@@ -398,13 +402,17 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     const bool half = L->elem == 2;
     const bool rows = L->layout == 1;
     L->vec = (int)env_ll("FFB_VEC", half && !rows ? 4 : 1);
-    L->tile_w = (int)env_ll("FFB_TILE_W", cfg->precision == FFB_PREC_FP64 ? (rows ? 512 : 256) : (rows ? 256 : 512));
-    L->xcd_order = (int)env_ll("FFB_XCD_ORDER", 0);
+    L->tile_w = (int)env_ll("FFB_TILE_W", rows || cfg->precision == FFB_PREC_FP64 ? 256 : 512);
     if (L->tile_w != 64 && L->tile_w != 128 && L->tile_w != 256 && L->tile_w != 512) L->tile_w = 256;
     // cells per thread must be 1, 2, 4 or 8 with at most 16 bytes per vector and
     // a 256- or 512-thread tile; anything else falls back to one cell per thread
     if (!(L->vec == 2 || L->vec == 4 || L->vec == 8) || (size_t)L->vec * L->elem > 16 || L->tile_w < 256)
         L->vec = 1;
+    {
+        const int tiles_x = (cfg->n_x + L->tile_w * L->vec - 1) / (L->tile_w * L->vec);
+        const int g = rows ? (half ? 2 * tiles_x : (cfg->precision == FFB_PREC_FP64 ? std::max(1, tiles_x / 2) : 0)) : 0;
+        L->xcd_group = (int)std::max(0LL, env_ll("FFB_XCD_GROUP", g));
+    }
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);

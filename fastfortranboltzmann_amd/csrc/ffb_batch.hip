@@ -104,7 +104,7 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     a.inv_scale = 1;
     a.n_tiles_x = 0;
     a.n_blocks = 0;
-    a.xcd_order = 0;
+    a.xcd_group = 0;
     a.force = L.force;
     if constexpr (Store<S>::dev) {  // deviation-space constants of collide_dev (2^s = 1)
         a.om1 = (C)1 - a.omega;
