@@ -145,10 +145,12 @@ def channel_settings(n: int, omega: float, force: float, precision: str) -> Sett
 
 
 def inverse_channel(n: int = 512, steps: int = 5000, interval: int = 100, seed: int = 0,
-                    iters: int = 50, lr: float = 0.05, precision: str = "fp32", log=print) -> dict:
+                    iters: int = 50, lr: float = 0.08, precision: str = "fp32", log=print,
+                    lr_decay: float = 0.85, decay_start: int = 30) -> dict:
     """Inverse body-force channel: the target u field is the forward solution at
     seeded true (omega, g) with omega in [0.8, 1.8], g in [1e-7, 1e-5]; Adam on
-    (omega, g / 1e-6) from a perturbed start recovers them."""
+    (omega, g / 1e-6) from a perturbed start recovers them.  Learning rate lr
+    for the first decay_start iterations, then times lr_decay per iteration."""
     rng = np.random.Generator(np.random.PCG64(seed))
     om_true = float(rng.uniform(0.8, 1.8))
     g_true = float(10 ** rng.uniform(-7, -5))
@@ -158,6 +160,8 @@ def inverse_channel(n: int = 512, steps: int = 5000, interval: int = 100, seed: 
         om0, g0 = om_true * 0.85 + 0.1, g_true * 1.6
         theta = torch.tensor([om0, g0 / 1e-6], dtype=torch.float64, requires_grad=True)
         opt = torch.optim.Adam([theta], lr=lr)
+        sched = torch.optim.lr_scheduler.LambdaLR(
+            opt, lambda k: 1.0 if k < decay_start else lr_decay ** (k - decay_start + 1))
         hist = []
         for it in range(iters):
             opt.zero_grad()
@@ -165,25 +169,28 @@ def inverse_channel(n: int = 512, steps: int = 5000, interval: int = 100, seed: 
             loss = torch.sum((ux - torch.from_numpy(tx)) ** 2 + (uy - torch.from_numpy(ty)) ** 2)
             loss.backward()
             opt.step()
+            sched.step()
             with torch.no_grad():
                 theta[0].clamp_(0.55, 1.95)
                 theta[1].clamp_(1e-3, 100.0)
             hist.append((float(loss), float(theta[0]), float(theta[1]) * 1e-6))
             if it % 10 == 0 or it == iters - 1:
                 log(f"adam {it}: loss {hist[-1][0]:.3e} omega {hist[-1][1]:.6f} g {hist[-1][2]:.4e}")
-    return {"omega_true": om_true, "g_true": g_true, "omega_start": om0, "g_start": g0,
+    return {"precision": precision, "omega_true": om_true, "g_true": g_true, "omega_start": om0, "g_start": g0,
             "omega_final": hist[-1][1], "g_final": hist[-1][2], "loss_start": hist[0][0],
-            "loss_final": hist[-1][0], "history": hist}
+            "loss_final": hist[-1][0], "rel_err_omega": abs(hist[-1][1] - om_true) / om_true,
+            "rel_err_g": abs(hist[-1][2] - g_true) / g_true, "history": hist}
 
 
 def gradient_check(n: int, steps: int, interval: int, omega: float, force: float,
                    t_omega: float, t_force: float, rel_h: float = 1e-4) -> dict:
-    """Adjoint gradients of ||u - u_target||^2 in FP32 and FP64 against central
-    differences of the FP64 forward solver (h = rel_h relative)."""
+    """Adjoint gradients of ||u - u_target||^2 in FP64, FP32 and FP32
+    deviations against central differences of the FP64 forward solver
+    (h = rel_h relative) and against each other."""
     out = {}
     with Adjoint(channel_settings(n, t_omega, t_force, "fp64"), steps, interval) as a:
         _, tx, ty = a.forward()
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32dev"):
         with Adjoint(channel_settings(n, omega, force, prec), steps, interval) as a:
             loss, g = loss_and_gradient(a, tx, ty)
             out[prec] = {"loss": loss, "d_omega": g[0], "d_force": g[1]}
@@ -193,7 +200,8 @@ def gradient_check(n: int, steps: int, interval: int, omega: float, force: float
     for k in ("d_omega", "d_force"):
         fd = out["fd"][k]
         out[f"rel_err_fp64_vs_fd_{k}"] = abs(out["fp64"][k] - fd) / abs(fd)
-        out[f"rel_err_fp32_vs_fp64_{k}"] = abs(out["fp32"][k] - out["fp64"][k]) / abs(out["fp64"][k])
+        for prec in ("fp32", "fp32dev"):
+            out[f"rel_err_{prec}_vs_fp64_{k}"] = abs(out[prec][k] - out["fp64"][k]) / abs(out["fp64"][k])
     return out
 
 
@@ -208,7 +216,10 @@ def main(argv=None) -> int:
     ap.add_argument("--interval", type=int, default=100)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--lr", type=float, default=0.05)
+    ap.add_argument("--lr", type=float, default=0.08)
+    ap.add_argument("--lr-decay", type=float, default=0.85)
+    ap.add_argument("--decay-start", type=int, default=30)
+    ap.add_argument("--precisions", default="fp32,fp32dev", help="forward storage of the Adam runs")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     rng = np.random.Generator(np.random.PCG64(args.seed))
@@ -216,8 +227,12 @@ def main(argv=None) -> int:
     log(f"gradient check at ({t_om * 0.85 + 0.1:.4f}, {t_g * 1.6:.3e}) against target ({t_om:.4f}, {t_g:.3e})")
     gc = gradient_check(args.n, args.steps, args.interval, t_om * 0.85 + 0.1, t_g * 1.6, t_om, t_g)
     log(json.dumps(gc))
-    inv = inverse_channel(args.n, args.steps, args.interval, args.seed, args.iters, args.lr, "fp32", log)
-    inv.pop("history")
+    inv = {}
+    for prec in args.precisions.split(","):
+        inv[prec] = inverse_channel(args.n, args.steps, args.interval, args.seed, args.iters, args.lr, prec, log,
+                                    args.lr_decay, args.decay_start)
+        inv[prec].pop("history")
+        log(json.dumps(inv[prec]))
     with Adjoint(channel_settings(args.n, t_om, t_g, "fp32"), args.steps, args.interval) as a:
         for _ in range(2):
             _, ux, uy = a.forward()

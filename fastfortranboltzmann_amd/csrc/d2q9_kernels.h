@@ -105,10 +105,10 @@ template <> struct Store<__hip_bfloat16> {
     static constexpr bool dev = true;
     static __device__ __forceinline__ C in(R r) { return __uint_as_float((unsigned)r << 16); }
     static __device__ __forceinline__ R out(C v) {
-        unsigned u = __float_as_uint(v);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return (R)((u >> 16) | 0x40u);  // quiet NaN
-        u += 0x7fffu + ((u >> 16) & 1u);                                       // nearest even
-        return (R)(u >> 16);
+        // v_cvt_pk_bf16_f32 (gfx950): round to nearest even, as the FP32 value
+        // v itself (the empty asm keeps the conversion on that value)
+        asm("" : "+v"(v));
+        return __builtin_bit_cast(R, (__bf16)v);
     }
     static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };

@@ -191,7 +191,8 @@ typedef struct ffb_batch_tuning {
 
 typedef struct ffb_batch ffb_batch_t;
 
-/* precision: FFB_PREC_FP32 or FFB_PREC_FP64. */
+/* precision: FFB_PREC_FP32, FFB_PREC_FP64 or FFB_PREC_FP32_DEV (FP32
+ * deviations with scale 1; populations read back as f = w + D). */
 int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t precision, int32_t device,
                      ffb_batch_t **out);
 int ffb_batch_destroy(ffb_batch_t *batch);
@@ -217,7 +218,7 @@ int ffb_batch_read_populations(ffb_batch_t *batch, int32_t index, void *f_halo);
 /* ------------------------------------------------------------------------
  * Differentiation form (no reference counterpart): discrete adjoint of n_steps
  * steps of a handle (shear wave, Couette, sliding lid or body-force channel;
- * FP32 or FP64), with a checkpoint every checkpoint_interval steps and
+ * FP32, FP32 deviations or FP64), with a checkpoint every checkpoint_interval steps and
  * recomputation of each segment during the reverse sweep.  The loss is any
  * function of the velocity field u written by the last step; the caller
  * passes dL/du and gets dL/domega, dL/d(body force) and dL/d(wall or lid
