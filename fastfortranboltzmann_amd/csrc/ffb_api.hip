@@ -387,8 +387,8 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->buf_pad = ((env_ll("FFB_BUF_PAD", 0) + 63) / 64) * 64;
     L->layout = (int)env_ll("FFB_LAYOUT", 1) == 0 ? 0 : 1;
     if (L->layout == 1) {
-        L->plane = px;
-        L->pitch = 9 * px;
+        L->plane = px + L->plane_pad;
+        L->pitch = (int)(9 * L->plane);
         L->set_elems = (long long)L->pitch * cfg->n_y;
     } else {
         L->pitch = px;

@@ -21,7 +21,7 @@ struct ffb_lattice {
     //   layout 0 (planes): 9 planes of n_y rows, pitch = n_x rounded up to 64,
     //     plane = pitch * n_y (+ plane_pad);
     //   layout 1 (rows): per row y the 9 direction rows one after another,
-    //     plane = n_x rounded up to 64, pitch = 9 * plane.
+    //     plane = n_x rounded up to 64 (+ plane_pad), pitch = 9 * plane.
     int pitch = 0;           // row stride in elements
     long long plane = 0;     // direction stride in elements
     long long set_elems = 0; // elements of one population set
