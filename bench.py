@@ -13,7 +13,10 @@ the HBM roofline of the step kernel (algorithmic 18*s bytes per cell update,
 HIP events on the library's stream), an FP64 run of the same workload, a
 copy-kernel ceiling, and the CPU baseline: the reference's own kernel
 modules compiled by flang -O3 (oracle/_ref) on one host core, or, where that
-binary is absent, the CPU oracle on all host cores.
+binary is absent, the CPU oracle on all host cores.  Unless --no-extras, the
+line also carries the deviation storage forms (8192^2 speed, 1024^2 error
+against FP64), the 512-lattice batched form, the adjoint's backward/forward
+cost and the 256^2 shear-wave case with its fitted viscosity.
 """
 from __future__ import annotations
 
@@ -91,6 +94,36 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
     return out
+
+
+def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
+    """The reference's CPU-runnable case on the GPU: shear-wave decay 256^2,
+    FP64, omega 1, u_max 0.05, n_sin 1, 2000 steps.  MLUPS (cache-resident at
+    this size, not graded against HBM) and the decay-rate viscosity fitted
+    from the amplitude every `chunk` steps against nu = (1/omega - 1/2)/3."""
+    import numpy as np
+
+    from fastfortranboltzmann_amd import Lattice, Settings
+    from fastfortranboltzmann_amd.analysis import decay_amplitude, fit_viscosity, lattice_viscosity
+
+    n = 256
+    s = Settings(N_X=n, N_Y=n, SIM_MODE=1, OMEGA=1.0, U_MAX=0.05, N_SIN=1.0, RHO_0=1.0, precision="fp64")
+    amps, times, kernel_ms = [], [], 0.0
+    with Lattice(s) as lat:
+        lat.init()
+        for t in range(chunk, steps + 1, chunk):
+            lat.reset_timer()
+            lat.step(chunk, write_macro_last=True)
+            lat.sync()
+            kernel_ms += lat.elapsed_ms()
+            _, ux, _ = lat.macros()
+            amps.append(decay_amplitude(ux, 1.0))
+            times.append(t)
+    nu = lattice_viscosity(1.0)
+    nu_fit = fit_viscosity(np.array(times), np.array(amps), n, 1.0)
+    return {"size": n, "steps": steps, "mlups": round(n * n * steps / (kernel_ms * 1e-3) / 1e6, 1),
+            "nu_fit": nu_fit, "nu": nu, "nu_rel_err": abs(nu_fit - nu) / nu,
+            "amplitude_ratio": amps[-1] / 0.05}
 
 
 def batched(single_fp32_mlups: float, steps: int = 200) -> dict:
@@ -229,6 +262,8 @@ def main() -> int:
         extra["batched"] = batched(main_run["mlups"])
         print("bench: adjoint", file=sys.stderr, flush=True)
         extra["adjoint"] = adjoint_cost()
+        print("bench: shear wave 256^2", file=sys.stderr, flush=True)
+        extra["shear_wave_256_fp64"] = shear_wave_256()
     cc = copy_ceiling()
     workload = f"sl{n}_{args.precision}"
     traffic = traffic_from_profiles(workload)

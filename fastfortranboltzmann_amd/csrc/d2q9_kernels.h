@@ -49,9 +49,10 @@ static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) ==
 
 // ---------------------------------------------------------------------------
 // storage of one population value.  FP64 / FP32 store f itself and the step
-// works on f.  The 16-bit forms store D = (f - w) * 2^s; the step reads D as
-// FP32, works on D throughout (boundary rules and collision in deviation
-// space, collide_dev) and rounds the new D to 16 bits.  `in` gives the value
+// works on f.  The deviation forms store D = (f - w) * 2^s (FP16, BF16 or
+// FP32); the step reads D as FP32, works on D throughout (boundary rules and
+// collision in deviation space, collide_dev) and rounds the new D to the
+// storage type.  `in` gives the value
 // the step works on, `out` the stored form of it, `from_f` turns a population
 // f into that value (initial state).
 // ---------------------------------------------------------------------------
@@ -130,16 +131,16 @@ template <typename C> struct StepArgs {
     long long plane;
     C omega;
     C corr8, corr9;        // moving-wall terms ((6*w_6)*rho_0)*u_wall, couette_flow.f90:118-119
-                           // (times 2^s in the 16-bit forms)
+                           // (times 2^s in the deviation forms)
     C rho_in, rho_out;     // Poiseuille boundary densities
     const C *m_old;        // Poiseuille strips of the previous step [side][rho,ux,uy][ny]
     C *m_new;              // strips of this step (written by the strip pass, read by the step)
     C *rho_o, *ux_o, *uy_o;// macro fields (nx*ny, x fastest), written when write_macro
     int write_macro;
     C w[9];                // lattice weights in C precision
-    C scale, inv_scale;    // 2^s and 2^-s of the 16-bit forms
+    C scale, inv_scale;    // 2^s and 2^-s of the deviation forms
     C force;               // BF: body-force density along x (lattice units)
-    // deviation-space collision of the 16-bit forms (collide_dev):
+    // deviation-space collision of the deviation forms (collide_dev):
     C om1;                 // 1 - omega
     C kw[9];               // (omega * w_d) * 2^s
     C kf[9];               // BF: (w_d * ((1 - omega/2) * force)) * 2^s
@@ -184,7 +185,7 @@ __device__ __forceinline__ C pf_delta(const StepArgs<C> &a, const C *m, int d, i
     const C fac = (((C)1 + (C)3 * cu) + (C)4.5 * cu * cu) - (C)1.5 * usq;
     return a.w[d] * (rho_b - rho_s) * fac;
 }
-// The increment in the units the step works on (times 2^s in the 16-bit forms).
+// The increment in the units the step works on (times 2^s in the deviation forms).
 template <typename S, typename C>
 __device__ __forceinline__ C pf_inc(const StepArgs<C> &a, const C *m, int d, int side, int row) {
     const C v = pf_delta<C>(a, m, d, side, row);
@@ -317,9 +318,9 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
     }
 }
 
-// BGK collision of the 16-bit forms in deviation space.  D[] holds the
-// scaled deviations D_d = (f_d - w_d) * 2^s on entry and the new ones on exit
-// (not yet rounded to 16 bits).  With Sum w_d = 1 and Sum w_d c_d = 0,
+// BGK collision of the deviation forms.  D[] holds the scaled deviations
+// D_d = (f_d - w_d) * 2^s on entry and the new ones on exit (not yet rounded
+// to the storage type).  With Sum w_d = 1 and Sum w_d c_d = 0,
 //   rho - 1 = 2^-s Sum D_d,   rho u = 2^-s Sum c_d D_d (+ F/2 for BF),
 // and the equilibrium deviation is
 //   2^s (f_eq_d - w_d) = 2^s w_d ((rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)),
@@ -330,7 +331,7 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
 // directions share the even part of the equilibrium:
 //   inner_{d,opp} = (base + (rho/2) h^2) +- rho h,  h = 3 c_d.u,
 //   base = (rho - 1) - 1.5 rho u^2.
-// The operation order here is the definition of the 16-bit forms (the oracle
+// The operation order here is the definition of the deviation forms (the oracle
 // restates it, ffbo_q_step).  No reference counterpart.
 template <typename C, int MODE>
 __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int x, int y) {

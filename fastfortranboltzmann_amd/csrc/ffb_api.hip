@@ -103,7 +103,7 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     const int cells_per_tile = L->tile_w * cells_per_thread(L);
     a.n_tiles_x = (c.n_x + cells_per_tile - 1) / cells_per_tile;
     a.force = (C)c.body_force;
-    if (dev) {  // the 16-bit forms work on deviations times 2^s (collide_dev)
+    if (dev) {  // the deviation forms work on deviations times 2^s (collide_dev)
         a.corr8 = a.corr8 * a.scale;
         a.corr9 = a.corr9 * a.scale;
     }

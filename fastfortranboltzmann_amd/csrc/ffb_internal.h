@@ -52,6 +52,6 @@ struct ffb_lattice {
 namespace ffb {
 // Enqueues one step of L's configuration reading populations at src and
 // writing dst (both in L's layout), on L's stream; Poiseuille flow and the
-// 16-bit forms are not supported.
+// 16-bit forms are not supported (FP32 deviations are).
 int step_between(ffb_lattice *L, const void *src, void *dst, int write_macro);
 }  // namespace ffb
