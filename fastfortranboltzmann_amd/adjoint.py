@@ -7,7 +7,9 @@ steps, recomputation inside each segment).  No reference counterpart.
 solver for n_steps through the C ABI and returns (u_x, u_y); its backward
 hands dL/du to the adjoint and returns dL/domega, dL/dF, dL/du_wall.  Torch is
 only the autograd and tensor plumbing here; every step and every adjoint
-step runs in the HIP kernels.
+step runs in the HIP kernels.  With parameters on a GPU device the velocity
+fields and their gradients stay in device memory: the library copies the
+macro fields into torch's tensors and reads the seed from them directly.
 """
 from __future__ import annotations
 
@@ -71,6 +73,28 @@ class Adjoint:
                                         ctypes.c_void_p(ux.ctypes.data), ctypes.c_void_p(uy.ctypes.data)))
         return rho, ux, uy
 
+    def forward_device(self, device):
+        """As forward, into torch tensors on a GPU device (no host copy); returns
+        (u_x, u_y) in the lattice's element type."""
+        dt = torch.float64 if self.dtype == np.float64 else torch.float32
+        ux = torch.empty((self.ny, self.nx), dtype=dt, device=device)
+        uy = torch.empty_like(ux)
+        torch.cuda.current_stream(device).synchronize()  # the library works on its own stream
+        check(lib().ffb_adjoint_forward(self._h, None, ctypes.c_void_p(ux.data_ptr()),
+                                        ctypes.c_void_p(uy.data_ptr())))
+        return ux, uy
+
+    def backward_device(self, grad_u_x, grad_u_y):
+        """As backward, for seeds held in torch tensors on a GPU device."""
+        dt = torch.float64 if self.dtype == np.float64 else torch.float32
+        gx = grad_u_x.detach().to(dt).contiguous()
+        gy = grad_u_y.detach().to(dt).contiguous()
+        torch.cuda.current_stream(gx.device).synchronize()  # seeds written by torch's stream
+        d = [ctypes.c_double() for _ in range(3)]
+        check(lib().ffb_adjoint_backward(self._h, ctypes.c_void_p(gx.data_ptr()), ctypes.c_void_p(gy.data_ptr()),
+                                         *[ctypes.byref(x) for x in d]))
+        return d[0].value, d[1].value, d[2].value
+
     def backward(self, grad_u_x: np.ndarray, grad_u_y: np.ndarray):
         """dL/domega, dL/d(body force), dL/d(wall speed) for the seed dL/du."""
         gx = np.ascontiguousarray(grad_u_x, dtype=self.dtype)
@@ -122,16 +146,22 @@ try:  # torch is plumbing only; the Function exists where torch is importable
         @staticmethod
         def forward(ctx, omega, body_force, u_wall, adj: Adjoint):
             adj.set_parameters(float(omega), float(body_force), float(u_wall))
-            _, ux, uy = adj.forward()
             ctx.adj = adj
             dev = omega.device
+            if dev.type == "cuda":  # device-resident fields, no host round trip
+                ux, uy = adj.forward_device(dev)
+                return ux.to(torch.float64), uy.to(torch.float64)
+            _, ux, uy = adj.forward()
             return (torch.from_numpy(ux.astype(np.float64)).to(dev),
                     torch.from_numpy(uy.astype(np.float64)).to(dev))
 
         @staticmethod
         def backward(ctx, grad_ux, grad_uy):
             adj = ctx.adj
-            d_om, d_f, d_w = adj.backward(grad_ux.detach().cpu().numpy(), grad_uy.detach().cpu().numpy())
+            if grad_ux.device.type == "cuda":
+                d_om, d_f, d_w = adj.backward_device(grad_ux, grad_uy)
+            else:
+                d_om, d_f, d_w = adj.backward(grad_ux.detach().cpu().numpy(), grad_uy.detach().cpu().numpy())
             dev = grad_ux.device
             t = lambda v: torch.tensor(v, dtype=torch.float64, device=dev)  # noqa: E731
             return t(d_om), t(d_f), t(d_w), None

@@ -102,6 +102,33 @@ def test_torch_function_gradients_equal_adjoint():
     assert float(fo.grad) == pytest.approx(g[1], rel=1e-12)
 
 
+def test_torch_function_on_device_equals_host_path():
+    """Parameters on the GPU: fields and seeds stay in device memory and the
+    gradients equal those of the host path bit for bit (same seed values)."""
+    torch = pytest.importorskip("torch")
+    from fastfortranboltzmann_amd.adjoint import Adjoint, LatticeVelocity
+
+    print("[gpu] torch autograd function on device", flush=True)
+    mode, true_kw, start_kw, _ = CASES[0]
+    steps = 100
+    tx, ty = _target(mode, true_kw, steps)
+    grads = {}
+    for dev in ("cpu", "cuda"):
+        with Adjoint(_settings(mode, start_kw, "fp64"), steps, 25) as a:
+            om = torch.tensor(start_kw["OMEGA"], dtype=torch.float64, device=dev, requires_grad=True)
+            fo = torch.tensor(start_kw["BODY_FORCE"], dtype=torch.float64, device=dev, requires_grad=True)
+            uw = torch.tensor(0.0, dtype=torch.float64, device=dev, requires_grad=True)
+            ux, uy = LatticeVelocity.apply(om, fo, uw, a)
+            assert ux.device.type == dev
+            t_x = torch.from_numpy(tx).to(dev)
+            t_y = torch.from_numpy(ty).to(dev)
+            loss = torch.sum((ux - t_x) ** 2 + (uy - t_y) ** 2)
+            loss.backward()
+            grads[dev] = (float(om.grad), float(fo.grad), float(loss))
+    assert grads["cuda"][:2] == grads["cpu"][:2], grads
+    assert grads["cuda"][2] == pytest.approx(grads["cpu"][2], rel=1e-12)
+
+
 def test_adjoint_cost_against_forward():
     """Reverse sweep (recomputation + adjoint steps) against the forward pass
     with checkpoints, 512^2 channel, FP32, 2000 steps, checkpoint every 100."""
