@@ -160,6 +160,16 @@ class OracleLattice:
         self.steps += int(n_steps)
         return self
 
+    def set_populations(self, f_halo: np.ndarray) -> "OracleLattice":
+        """Replaces the state by f_halo (shape (9, ny+2, nx+2)); a deviation form
+        stores it encoded (D = round((f - w) 2^s)), as ffb_write_populations does."""
+        self.f[:] = f_halo.astype(self.dtype)
+        if self.storage is not None:
+            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
+                                _p(self.f), _p(self.d))
+            self._decode()
+        return self
+
     def _decode(self) -> None:
         lib().ffbo_q_decode(self.scale_log2, self.nx, self.ny, _p(self.d), _p(self.f))
 

@@ -1,0 +1,113 @@
+"""C ABI entry points around the step, on the GPU: population write/read
+round trips in every storage form, random states in the deviation forms (every
+boundary rule carries a distinct value), FP64 moment sums, and parameter
+changes between steps (ffb_set_parameters), each against the oracle."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleLattice, Params
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 12), ("bf16dev", 0)]
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _oracle(mode, nx, ny, p, precision, scale):
+    if precision in ("fp64", "fp32"):
+        return OracleLattice(mode, nx, ny, p, np.float64 if precision == "fp64" else np.float32)
+    return OracleLattice(mode, nx, ny, p, storage=precision, scale_log2=scale)
+
+
+def _lattice(mode, nx, ny, p, precision, scale):
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    return Lattice(Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=p.omega, U_MAX=p.u_max, N_SIN=p.n_sin,
+                            U_WALL=p.u_wall, U_LID=p.u_lid, RHO_IN=p.rho_in, RHO_OUT=p.rho_out,
+                            BODY_FORCE=p.force, precision=precision, storage_scale_log2=scale))
+
+
+def _random_state(nx, ny, seed, dtype):
+    rng = np.random.default_rng(seed)
+    return (W[:, None, None] * (1 + 0.2 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dtype)
+
+
+@pytest.mark.parametrize("precision,scale", FORMS)
+def test_population_round_trip(precision, scale):
+    nx, ny = 37, 11
+    print(f"[gpu] population round trip {precision}", flush=True)
+    p = Params(omega=1.1)
+    dt = np.float64 if precision == "fp64" else np.float32
+    f0 = _random_state(nx, ny, 7, dt)
+    o = _oracle(4, nx, ny, p, precision, scale).init().set_populations(f0)
+    with _lattice(4, nx, ny, p, precision, scale) as lat:
+        lat.init()
+        lat.set_populations(f0)
+        np.testing.assert_array_equal(lat.interior(), o.interior)
+        if precision in ("fp64", "fp32"):
+            np.testing.assert_array_equal(lat.interior(), f0[:, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("precision,scale", FORMS[2:])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+def test_deviation_forms_from_random_state(mode, precision, scale):
+    nx, ny = 37, 29
+    print(f"[gpu] random state mode {mode} {precision}", flush=True)
+    p = Params(omega=0.83, u_wall=0.05, u_lid=0.12, rho_in=1.01, rho_out=0.99,
+               force=3e-4 if mode == 5 else 0.0)
+    f0 = _random_state(nx, ny, 1234 + mode, np.float32)
+    o = _oracle(mode, nx, ny, p, precision, scale).init().set_populations(f0).run(11)
+    with _lattice(mode, nx, ny, p, precision, scale) as lat:
+        lat.init()
+        lat.set_populations(f0)
+        lat.step(11, write_macro_last=True)
+        np.testing.assert_array_equal(lat.interior(), o.interior)
+        rho, ux, uy = lat.macros()
+        np.testing.assert_array_equal(ux, o.u_x)
+        np.testing.assert_array_equal(uy, o.u_y)
+        if mode == 3:
+            left, right = lat.pf_strips()
+            np.testing.assert_array_equal(left, o.recv_macro_left)
+            np.testing.assert_array_equal(right, o.recv_macro_right)
+
+
+@pytest.mark.parametrize("precision,scale", FORMS)
+def test_moments_are_fp64_sums_of_the_state(precision, scale):
+    """mass and momentum from ffb_moments against float64 sums of the state
+    read back (the read-back rounds a deviation form's w + D 2^-s to FP32,
+    hence the tolerance)."""
+    nx, ny = 64, 48
+    print(f"[gpu] moments {precision}", flush=True)
+    p = Params(omega=1.2, u_lid=0.1)
+    with _lattice(4, nx, ny, p, precision, scale) as lat:
+        lat.init().step(40)
+        m, mx, my = lat.moments()
+        f = lat.interior().astype(np.float64)
+    cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], float)[:, None, None]
+    cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], float)[:, None, None]
+    tol = 0 if precision in ("fp64", "fp32") else 1e-7
+    assert m == pytest.approx(f.sum(), rel=1e-13 + tol)
+    assert mx == pytest.approx((cx * f).sum(), abs=1e-10 + tol * nx * ny)
+    assert my == pytest.approx((cy * f).sum(), abs=1e-10 + tol * nx * ny)
+
+
+@pytest.mark.parametrize("precision,scale", FORMS)
+@pytest.mark.parametrize("mode", [2, 4, 5])
+def test_parameters_changed_between_steps(mode, precision, scale):
+    """ffb_set_parameters between two runs of steps: the same as the oracle
+    continuing with the new omega, wall speed and force."""
+    from fastfortranboltzmann_amd import _lib
+
+    nx, ny = 33, 21
+    print(f"[gpu] set_parameters mode {mode} {precision}", flush=True)
+    p = Params(omega=1.3, u_wall=0.06, u_lid=0.06, force=2e-4 if mode == 5 else 0.0)
+    o = _oracle(mode, nx, ny, dataclasses.replace(p), precision, scale).init().run(7)
+    o.params = dataclasses.replace(p, omega=0.9, u_wall=0.09, u_lid=0.09, force=5e-4 if mode == 5 else 0.0)
+    o.run(6)
+    with _lattice(mode, nx, ny, p, precision, scale) as lat:
+        lat.init().step(7)
+        _lib.check(_lib.lib().ffb_set_parameters(lat.handle, 0.9, o.params.force, 0.09, 0.09))
+        lat.step(6, write_macro_last=True)
+        np.testing.assert_array_equal(lat.interior(), o.interior)
