@@ -73,9 +73,11 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
     """Deviation storage forms: MLUPS at n^2 (cavity, Re 5000) against FP32 at
     the same size, and the relative L2 velocity error and mass drift against
     FP64 on a smaller cavity run (err_n^2, Re 1000, err_steps steps), for the
-    16-bit forms, FP32 deviations and plain FP32."""
+    16-bit forms, FP32 deviations and plain FP32; for each form also the density
+    error and the relative error of the shear-wave viscosity fitted in that form
+    (256^2, omega 1, 2000 steps) against nu = (1/omega - 1/2)/3."""
     from fastfortranboltzmann_amd.precision import (cavity_settings, compare, default_scale_log2,
-                                                    run_form)
+                                                    run_form, shear_wave_viscosity)
 
     forms = ("fp32", "fp32dev", "fp16dev", "bf16dev")
     out = {"size": n, "steps": steps, "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
@@ -87,10 +89,13 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
     for form in forms:
         s = default_scale_log2(form) if form.endswith("dev") else 0
         r = run_form(cavity_settings(err_n, 1000.0, 0.1, form, s), err_steps, chunk=err_steps)
+        c = compare(ref, r)
+        sw = shear_wave_viscosity(form)
         out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
                      "achieved_gbs": round(rates[form] * 1e6 * 18 * BYTES_PER_POP[form] / 1e9, 1),
-                     "scale_log2": s, "rel_l2_u_vs_fp64": compare(ref, r)["rel_l2_u"],
-                     "mass_drift": r["mass_drift"]}
+                     "scale_log2": s, "rel_l2_u_vs_fp64": c["rel_l2_u"], "max_abs_rho_vs_fp64": c["max_abs_rho"],
+                     "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"], "mass_drift": r["mass_drift"],
+                     "shear_wave_nu_rel_err": sw["nu_rel_err"]}
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
     return out

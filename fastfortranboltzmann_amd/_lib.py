@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(PKG, "libffb_hip.so")
 
 SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID, BODY_FORCE_CHANNEL = 1, 2, 3, 4, 5
 PREC_FP32, PREC_FP64, PREC_FP16_DEV, PREC_BF16_DEV, PREC_FP32_DEV = 0, 1, 2, 3, 4
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 
 # every entry point include/ffb.h declares
 EXPORTS = (
@@ -57,6 +58,7 @@ class FfbConfig(ctypes.Structure):
         ("rho_in", ctypes.c_double), ("rho_out", ctypes.c_double),
         ("storage_scale_log2", ctypes.c_int32), ("device", ctypes.c_int32),
         ("body_force", ctypes.c_double),
+        ("storage_rounding", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

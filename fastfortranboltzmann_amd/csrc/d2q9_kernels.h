@@ -58,6 +58,34 @@ static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) ==
 // ---------------------------------------------------------------------------
 template <typename S> struct Store;
 
+// Stochastic rounding of the 16-bit forms.  Each stored value takes 7 random
+// bits from a hash of (cell, step): the MurmurHash3 32-bit finaliser
+// (Appleby, public domain) of cell ^ (step * 0x9e3779b9) gives 32 bits, a
+// second finaliser call 32 more; population d uses bits 7d.. of the first
+// word (d < 4), 7(d-4).. of the second (4 <= d < 8), and the top nibbles of
+// both (d = 8).  The same (cell, step) always gives the same bits, so runs
+// are reproducible and the oracle restates them exactly.
+__host__ __device__ __forceinline__ unsigned fmix32(unsigned h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+struct CellBits {
+    unsigned h0, h1;
+};
+__device__ __forceinline__ CellBits cell_bits(unsigned long long cell, unsigned step) {
+    const unsigned c = (unsigned)cell ^ (unsigned)(cell >> 32);
+    const unsigned h0 = fmix32(c ^ (step * 0x9e3779b9u));
+    return {h0, fmix32(h0 + 0x632be5abu)};
+}
+__device__ __forceinline__ unsigned pop_bits(const CellBits &b, int d) {
+    return d < 4 ? (b.h0 >> (7 * d)) & 127u
+                 : (d < 8 ? (b.h1 >> (7 * (d - 4))) & 127u : ((b.h0 >> 28) | ((b.h1 >> 28) << 4)) & 127u);
+}
+
 template <> struct Store<double> {
     using C = double;  // compute type
     using R = double;  // raw stored type
@@ -98,6 +126,16 @@ template <> struct Store<__half> {
         asm("" : "+v"(v));
         return __half_as_ushort(__float2half_rn(v));  // round to nearest even
     }
+    // Stochastic rounding with 7 random bits r: gfx950's v_cvt_sr_f16_f32 with
+    // the random operand (r << 25) | (1 << 24), i.e. r and a half in the top
+    // bits of the bits the conversion discards (13 for FP16 normals, more
+    // below 2^-14): the magnitude bits plus those are truncated.  Up with
+    // probability (discarded fraction + 1/256) rounded to 1/128.
+    static __device__ __forceinline__ R out_sr(C v, unsigned r) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 h = __builtin_amdgcn_cvt_sr_f16_f32(h2{}, v, (r << 25) | (1u << 24), false);
+        return __builtin_bit_cast(R, h[0]);
+    }
     static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
 template <> struct Store<__hip_bfloat16> {
@@ -110,6 +148,14 @@ template <> struct Store<__hip_bfloat16> {
         // v itself (the empty asm keeps the conversion on that value)
         asm("" : "+v"(v));
         return __builtin_bit_cast(R, (__bf16)v);
+    }
+    // Stochastic rounding with 7 random bits r: gfx950's v_cvt_sr_bf16_f32 with
+    // the random operand (r << 25) | (1 << 24), which adds (r << 9) | 256 to the
+    // 16 bits below the BF16 mantissa and truncates them.
+    static __device__ __forceinline__ R out_sr(C v, unsigned r) {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        const b2 h = __builtin_amdgcn_cvt_sr_bf16_f32(b2{}, v, (r << 25) | (1u << 24), false);
+        return __builtin_bit_cast(R, h[0]);
     }
     static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
@@ -142,10 +188,13 @@ template <typename C> struct StepArgs {
     C force;               // BF: body-force density along x (lattice units)
     // deviation-space collision of the deviation forms (collide_dev):
     C om1;                 // 1 - omega
-    C kw[9];               // (omega * w_d) * 2^s
+    C kw[9], kw_lo[9];     // omega * w_d * 2^s with the exact weights 4/9, 1/9, 1/36 as a
+                           // pair of floats (kw + kw_lo, from a double product)
     C kf[9];               // BF: (w_d * ((1 - omega/2) * force)) * 2^s
     C half_force;          // BF: force / 2
     int n_tiles_x;         // tiles per row
+    int stochastic;        // 16-bit forms: 1 = stochastic rounding of the stored values
+    unsigned step_seed;    // step index for the stochastic-rounding hash
     int n_blocks;          // grid size (for the XCD-aware tile order)
     int xcd_group;         // G > 0: G consecutive tiles per XCD (tile_of_block)
 };
@@ -324,7 +373,8 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
 //   rho - 1 = 2^-s Sum D_d,   rho u = 2^-s Sum c_d D_d (+ F/2 for BF),
 // and the equilibrium deviation is
 //   2^s (f_eq_d - w_d) = 2^s w_d ((rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)),
-// so D'_d = (1 - omega) D_d + (omega w_d 2^s) * inner_d (+ Guo source for BF).
+// so D'_d = (1 - omega) D_d + (omega w_d 2^s) * inner_d (+ Guo source for BF),
+// the weight factor carried to double precision (kw + kw_lo).
 // Working on deviations keeps the FP32 arithmetic on values of the size of
 // the deviations instead of f ~ w (rounding errors ~1e3 times smaller for
 // the cavity), and needs one division (1/rho) instead of two.  Opposite
@@ -358,13 +408,18 @@ __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int
     const C om1 = a.om1;
     const C h1 = (C)3 * ux, h2 = (C)3 * uy;
     const C h5 = h1 + h2, h8 = h1 - h2;
-    D[0] = __builtin_fmaf(om1, D[0], a.kw[0] * base);
+    // (omega w_d 2^s) * v with the weight to double precision: the float
+    // weights 1/9, 1/36 are off by ~1e-8, which would break
+    // Sum w c_x^2 = 1/3 and grow the momentum by ~7e-9 per step (a
+    // viscosity bias of ~1e-4 for slowly decaying flows)
+    auto kmul = [&](int d, C v) { return __builtin_fmaf(a.kw[d], v, a.kw_lo[d] * v); };
+    D[0] = __builtin_fmaf(om1, D[0], kmul(0, base));
     // pair (d, e) with c_e = -c_d and h = 3 c_d.u
     auto pair = [&](int d, int e, C h) {
         const C p = __builtin_fmaf(hr, h * h, base);
         const C q = rho * h;
-        D[d] = __builtin_fmaf(om1, D[d], a.kw[d] * (p + q));
-        D[e] = __builtin_fmaf(om1, D[e], a.kw[e] * (p - q));
+        D[d] = __builtin_fmaf(om1, D[d], kmul(d, p + q));
+        D[e] = __builtin_fmaf(om1, D[e], kmul(e, p - q));
     };
     pair(1, 3, h1);
     pair(2, 4, h2);
@@ -416,6 +471,14 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
         for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
     }
     collide_mode<S, C, MODE>(a, v, x, y);
+    if constexpr (Store<S>::dev && sizeof(typename Store<S>::R) == 2) {
+        if (a.stochastic) {
+            const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
+#pragma unroll
+            for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(Store<S>::out_sr(v[d], pop_bits(b, d)), g + (size_t)d * P + c);
+            return;
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop<S>(g, (size_t)d * P + c, v[d]);
 }
@@ -487,6 +550,15 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
 #pragma unroll
         for (int d = 0; d < 9; ++d) v[d] = Store<S>::in(in[d][k]);
         collide_mode<S, C, MODE>(a, v, x0 + k, y);
+        if constexpr (Store<S>::dev && sizeof(R) == 2) {
+            if (a.stochastic) {
+                const CellBits b =
+                    cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)(x0 + k), a.step_seed);
+#pragma unroll
+                for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out_sr(v[d], pop_bits(b, d));
+                continue;
+            }
+        }
 #pragma unroll
         for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d]);
     }

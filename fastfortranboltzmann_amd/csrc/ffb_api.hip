@@ -68,6 +68,9 @@ __global__ void __launch_bounds__(256) moments_final_kernel(const double *partia
 }
 
 
+// D2Q9 weight to double precision (the deviation forms' collision factors)
+inline double exact_weight(int d) { return d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+
 template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macro) {
     StepArgs<C> a{};
     const ffb_config_t &c = L->cfg;
@@ -110,10 +113,15 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     a.om1 = (C)1 - a.omega;
     const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
     for (int d = 0; d < 9; ++d) {
-        a.kw[d] = (a.omega * a.w[d]) * a.scale;
+        const double kd = (double)a.omega * exact_weight(d) * (double)a.scale;
+        a.kw[d] = (C)kd;
+        a.kw_lo[d] = (C)(kd - (double)a.kw[d]);
         a.kf[d] = (a.w[d] * pre) * a.scale;
     }
     a.half_force = (C)0.5 * a.force;
+    a.stochastic = (c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV) &&
+                   c.storage_rounding == FFB_ROUND_STOCHASTIC;
+    a.step_seed = (unsigned)(L->steps + 1);  // the step this launch computes
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_group = L->xcd_group;
     return a;
@@ -336,6 +344,8 @@ void ffb_config_default(ffb_config_t *cfg) {
     cfg->storage_scale_log2 = 0;
     cfg->device = 0;
     cfg->body_force = 0.0;
+    cfg->storage_rounding = FFB_ROUND_STOCHASTIC;
+    cfg->reserved = 0;
 }
 
 int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
@@ -352,6 +362,8 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
          cfg->precision == FFB_PREC_FP32_DEV) &&
         (cfg->storage_scale_log2 < -30 || cfg->storage_scale_log2 > 30))
         return fail(FFB_E_INVALID, "ffb_create: storage_scale_log2 outside [-30, 30]");
+    if (cfg->storage_rounding != FFB_ROUND_NEAREST && cfg->storage_rounding != FFB_ROUND_STOCHASTIC)
+        return fail(FFB_E_INVALID, "ffb_create: storage_rounding must be FFB_ROUND_NEAREST or FFB_ROUND_STOCHASTIC");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev)

@@ -111,7 +111,10 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
         const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
-            a.kw[d] = a.omega * a.w[d];
+            const double wd = d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
+            const double kd = (double)a.omega * wd;
+            a.kw[d] = (C)kd;
+            a.kw_lo[d] = (C)(kd - (double)a.kw[d]);
             a.kf[d] = a.w[d] * pre;
         }
         a.half_force = (C)0.5 * a.force;

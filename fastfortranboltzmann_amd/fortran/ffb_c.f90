@@ -14,11 +14,13 @@ module ffb_c
         ffb_moments, ffb_check, ffb_error_message
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
     public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV
+    public :: FFB_ROUND_NEAREST, FFB_ROUND_STOCHASTIC
 
     integer(c_int32_t), parameter :: FFB_SHEAR_WAVE = 1, FFB_COUETTE_FLOW = 2, &
         FFB_POISEUILLE_FLOW = 3, FFB_SLIDING_LID = 4
     integer(c_int32_t), parameter :: FFB_PREC_FP32 = 0, FFB_PREC_FP64 = 1, &
         FFB_PREC_FP16_DEV = 2, FFB_PREC_BF16_DEV = 3, FFB_PREC_FP32_DEV = 4
+    integer(c_int32_t), parameter :: FFB_ROUND_NEAREST = 0, FFB_ROUND_STOCHASTIC = 1
 
     ! ffb_config_t, field for field
     type, bind(c) :: ffb_config_t
@@ -37,6 +39,8 @@ module ffb_c
         integer(c_int32_t) :: storage_scale_log2
         integer(c_int32_t) :: device
         real(c_double) :: body_force
+        integer(c_int32_t) :: storage_rounding
+        integer(c_int32_t) :: reserved
     end type ffb_config_t
 
     interface

@@ -46,6 +46,14 @@ program ffb_host
     cfg%rho_in = s%rho_in
     cfg%rho_out = s%rho_out
     cfg%storage_scale_log2 = s%storage_scale_log2
+    select case (s%storage_rounding)
+    case ("stochastic")
+        cfg%storage_rounding = FFB_ROUND_STOCHASTIC
+    case ("nearest")
+        cfg%storage_rounding = FFB_ROUND_NEAREST
+    case default
+        error stop "error: FFB_STORAGE_ROUNDING must be stochastic or nearest"
+    end select
     select case (s%precision)
     case ("fp64")
         cfg%precision = FFB_PREC_FP64

@@ -9,7 +9,8 @@
 ! Because the solver is a library, one binary serves every configuration:
 ! an environment variable of the same name (e.g. FFB_N_X=4096) overrides the
 ! compiled value at run time.  FFB_PRECISION=fp32|fp64|fp16dev|bf16dev|fp32dev and
-! FFB_STORAGE_SCALE_LOG2 select the storage forms the reference lacks.
+! FFB_STORAGE_SCALE_LOG2 (and FFB_STORAGE_ROUNDING=stochastic|nearest for the
+! 16-bit forms) select the storage forms the reference lacks.
 module ffb_settings
     use iso_fortran_env, only: int32, int64, real64
     implicit none
@@ -28,6 +29,7 @@ module ffb_settings
         character(len=:), allocatable :: export_num
         character(len=:), allocatable :: precision
         integer(int32) :: storage_scale_log2
+        character(len=:), allocatable :: storage_rounding
     end type run_settings_t
 
 contains
@@ -84,6 +86,7 @@ contains
         s%precision = "fp32"
 #endif
         s%storage_scale_log2 = 0
+        s%storage_rounding = "stochastic"
 
         call env_int("FFB_N_X", s%n_x)
         call env_int("FFB_N_Y", s%n_y)
@@ -105,6 +108,7 @@ contains
         call env_string("FFB_EXPORT_NUM", s%export_num)
         call env_string("FFB_PRECISION", s%precision)
         call env_int("FFB_STORAGE_SCALE_LOG2", s%storage_scale_log2)
+        call env_string("FFB_STORAGE_ROUNDING", s%storage_rounding)
 
         if (s%n_x < 1 .or. s%n_y < 1) error stop "error: N_X and N_Y must be positive"
         if (s%n_steps < 0) error stop "error: N_STEPS must not be negative"

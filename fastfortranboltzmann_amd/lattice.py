@@ -31,6 +31,7 @@ from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, POISEUILLE_FLOW, PREC_BF16_
 
 PRECISIONS = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp16dev": PREC_FP16_DEV,
               "bf16dev": PREC_BF16_DEV, "fp32dev": PREC_FP32_DEV}
+ROUNDINGS = {"nearest": _lib.ROUND_NEAREST, "stochastic": _lib.ROUND_STOCHASTIC}
 MODES = {"shear_wave": SHEAR_WAVE, "couette_flow": COUETTE_FLOW,
          "poiseuille_flow": POISEUILLE_FLOW, "sliding_lid": SLIDING_LID,
          "body_force_channel": BODY_FORCE_CHANNEL}
@@ -54,21 +55,25 @@ class Settings:
     RHO_IN: float = 1.00005
     RHO_OUT: float = 0.99995
     precision: str = "fp32"          # FFB_FP64 off -> "fp32", on -> "fp64"
-    storage_scale_log2: int = 0      # s of the 16-bit deviation forms
+    storage_scale_log2: int = 0      # s of the deviation forms
     device: int = 0
     BODY_FORCE: float = 0.0          # body-force channel (mode 5) only
+    storage_rounding: str = "stochastic"  # 16-bit forms: "stochastic" or "nearest"
     extra: dict = field(default_factory=dict)
 
     def config(self) -> FfbConfig:
         if self.precision not in PRECISIONS:
             raise ValueError(f"unknown precision {self.precision!r}")
+        if self.storage_rounding not in ROUNDINGS:
+            raise ValueError(f"unknown storage rounding {self.storage_rounding!r}")
         return FfbConfig(
             n_x=int(self.N_X), n_y=int(self.N_Y), sim_mode=int(self.SIM_MODE),
             precision=PRECISIONS[self.precision], omega=float(self.OMEGA),
             rho_0=float(self.RHO_0), u_max=float(self.U_MAX), n_sin=float(self.N_SIN),
             u_wall=float(self.U_WALL), u_lid=float(self.U_LID), rho_in=float(self.RHO_IN),
             rho_out=float(self.RHO_OUT), storage_scale_log2=int(self.storage_scale_log2),
-            device=int(self.device), body_force=float(self.BODY_FORCE))
+            device=int(self.device), body_force=float(self.BODY_FORCE),
+            storage_rounding=ROUNDINGS[self.storage_rounding])
 
 
 def reynolds(settings: Settings) -> float:

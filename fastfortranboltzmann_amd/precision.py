@@ -1,13 +1,17 @@
-"""Reduced-storage (16-bit) form of the solver: error against FP64 and speed.
+"""Deviation storage forms of the solver: error against FP64 and speed.
 
-The populations are stored as d_i = (f_i - w_i) * 2^s in FP16 or BF16, the
-collision runs in FP32, and the conservation sums (mass, momentum) are taken in
-FP64 (``Lattice.moments``).  This form has no counterpart in the reference;
-its results are stated against an FP64 run of the same initial state:
+The populations are stored as D_i = (f_i - w_i) * 2^s in FP16, BF16 or FP32,
+the step (boundary rules and collision) runs on D in FP32, and the
+conservation sums (mass, momentum) are taken in FP64 (``Lattice.moments``).
+These forms have no counterpart in the reference; their results are stated
+against an FP64 run of the same initial state:
 
   * relative L2 error of the velocity field, ||u - u64|| / ||u64||
   * maximum absolute velocity error
+  * density error: max |rho - rho64| and ||rho - rho64|| / ||rho64 - 1||
   * total-mass drift (m(t) - m(0)) / m(0), FP64 sums
+  * the shear-wave viscosity fitted from the decay of a sine wave in the same
+    form, against nu = (1/omega - 1/2)/3
 
     python -m fastfortranboltzmann_amd.precision --n 8192 --re 5000 --steps 50000
 """
@@ -66,10 +70,10 @@ def run_form(settings: Settings, steps: int, chunk: int = 5000):
             done += k
             masses.append(lat.moments()[0])
         wall = time.perf_counter() - t0
-        _, ux, uy = lat.macros()
+        rho, ux, uy = lat.macros()
     cells = settings.N_X * settings.N_Y
     return {
-        "u_x": ux.astype(np.float64), "u_y": uy.astype(np.float64),
+        "rho": rho.astype(np.float64), "u_x": ux.astype(np.float64), "u_y": uy.astype(np.float64),
         "mlups": cells * steps / (kernel_ms * 1e-3) / 1e6,
         "wall_s": wall,
         "mass_drift": (masses[-1] - m0) / m0,
@@ -80,9 +84,38 @@ def run_form(settings: Settings, steps: int, chunk: int = 5000):
 def compare(ref: dict, run: dict) -> dict:
     du = np.sqrt((run["u_x"] - ref["u_x"]) ** 2 + (run["u_y"] - ref["u_y"]) ** 2)
     norm = math.sqrt(float(np.sum(ref["u_x"] ** 2 + ref["u_y"] ** 2)))
+    drho = run["rho"] - ref["rho"]
+    dev = math.sqrt(float(np.sum((ref["rho"] - 1.0) ** 2)))
     return {"rel_l2_u": float(math.sqrt(float(np.sum(du ** 2))) / norm),
             "max_abs_u": float(du.max()),
-            "u_max_ref": float(np.sqrt(ref["u_x"] ** 2 + ref["u_y"] ** 2).max())}
+            "u_max_ref": float(np.sqrt(ref["u_x"] ** 2 + ref["u_y"] ** 2).max()),
+            "max_abs_rho": float(np.abs(drho).max()),
+            "rel_l2_rho_dev": float(math.sqrt(float(np.sum(drho ** 2))) / dev) if dev > 0 else 0.0}
+
+
+def shear_wave_viscosity(precision: str, n: int = 256, omega: float = 1.0, u_max: float = 0.05,
+                         steps: int = 2000, chunk: int = 200, scale_log2: int | None = None) -> dict:
+    """Shear-wave decay u_x = u_max sin(2 pi y / n) in one storage form: the
+    amplitude every `chunk` steps, the viscosity fitted from its decay, and its
+    relative error against nu = (1/omega - 1/2)/3."""
+    from .analysis import decay_amplitude, fit_viscosity, lattice_viscosity
+
+    if scale_log2 is None:
+        scale_log2 = default_scale_log2(precision, u_max) if precision.endswith("dev") else 0
+    s = Settings(N_X=n, N_Y=n, SIM_MODE=1, OMEGA=omega, U_MAX=u_max, N_SIN=1.0, RHO_0=1.0,
+                 precision=precision, storage_scale_log2=scale_log2)
+    amps, times = [], []
+    with Lattice(s) as lat:
+        lat.init()
+        for t in range(chunk, steps + 1, chunk):
+            lat.step(chunk, write_macro_last=True)
+            _, ux, _ = lat.macros()
+            amps.append(decay_amplitude(ux.astype(np.float64), 1.0))
+            times.append(t)
+    nu = lattice_viscosity(omega)
+    nu_fit = fit_viscosity(np.array(times), np.array(amps), n, 1.0)
+    return {"nu_fit": nu_fit, "nu": nu, "nu_rel_err": abs(nu_fit - nu) / nu, "n": n, "omega": omega,
+            "steps": steps}
 
 
 def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
@@ -101,7 +134,8 @@ def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
         log(f"precision: {form} (s = {s})")
         r = run_form(cavity_settings(n, re, u_lid, form, s), steps)
         rec = {"scale_log2": s, "mlups": r["mlups"], "mass_drift": r["mass_drift"],
-               "mass_drift_max": r["mass_drift_max"], **compare(ref, r)}
+               "mass_drift_max": r["mass_drift_max"], **compare(ref, r),
+               "shear_wave": shear_wave_viscosity(form)}
         out["forms"][form] = rec
         log(f"precision: {form}: " + json.dumps({k: v for k, v in rec.items()}))
     if "fp32" in out["forms"]:
@@ -112,7 +146,7 @@ def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
 
 
 def main(argv=None) -> int:
-    ap = argparse.ArgumentParser(description="16-bit storage error and speed against FP64")
+    ap = argparse.ArgumentParser(description="deviation storage forms: error and speed against FP64")
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--re", type=float, default=5000.0)
     ap.add_argument("--u-lid", type=float, default=0.1)

@@ -64,6 +64,14 @@ extern "C" {
 #define FFB_PREC_BF16_DEV 3
 #define FFB_PREC_FP32_DEV 4
 
+/* rounding of the 16-bit deviation forms' stored values.  Nearest-even loses
+ * every update smaller than half an FP16/BF16 step, so slow flows stagnate
+ * (a 256^2 shear wave at omega 1 does not decay at all); stochastic rounding,
+ * with random bits from a hash of (cell, step), keeps those updates on
+ * average and stays reproducible. */
+#define FFB_ROUND_NEAREST 0
+#define FFB_ROUND_STOCHASTIC 1
+
 /* status codes */
 #define FFB_OK 0
 #define FFB_E_INVALID 1   /* bad argument or configuration */
@@ -85,9 +93,11 @@ typedef struct ffb_config {
     double u_lid;         /* U_LID, sliding lid (settings.f90:41) */
     double rho_in;        /* RHO_IN, Poiseuille inlet density (settings.f90:42) */
     double rho_out;       /* RHO_OUT (settings.f90:43) */
-    int32_t storage_scale_log2; /* s of the 16-bit deviation forms; ignored otherwise */
+    int32_t storage_scale_log2; /* s of the deviation forms; ignored otherwise */
     int32_t device;       /* HIP device ordinal */
     double body_force;    /* FFB_BODY_FORCE_CHANNEL: force density along x; ignored otherwise */
+    int32_t storage_rounding; /* 16-bit forms: FFB_ROUND_STOCHASTIC or FFB_ROUND_NEAREST */
+    int32_t reserved;
 } ffb_config_t;
 
 typedef struct ffb_lattice ffb_lattice_t;

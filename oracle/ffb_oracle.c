@@ -123,6 +123,66 @@ static float q_round(int kind, float v)
     return v;
 }
 
+/* Stochastic rounding of the 16-bit forms (the GPU's out_sr): 7 random bits
+ * per stored value from the MurmurHash3 32-bit finaliser (Appleby, public
+ * domain) of cell ^ (step * 0x9e3779b9), cell = y * nx + x over the owned cells
+ * (0-based), step = index of the step being computed (1, 2, ...).  The GPU
+ * rounds with gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32 and the random
+ * operand R = (r << 25) | (1 << 24); restated here as measured on MI355X for
+ * 2^20 random (value, R) pairs: BF16 = (bits + (R >> 16)) >> 16; FP16 normal
+ * = magnitude bits + (R >> 19) with the low 13 bits cleared; FP16 below 2^-14
+ * (k = 126 - exponent discarded bits of the significand) = (significand +
+ * R aligned to the top of the k bits) >> k. */
+static unsigned int q_fmix32(unsigned int h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+static unsigned int q_pop_bits(unsigned int h0, unsigned int h1, int d)
+{
+    if (d < 4) return (h0 >> (7 * d)) & 127u;
+    if (d < 8) return (h1 >> (7 * (d - 4))) & 127u;
+    return ((h0 >> 28) | ((h1 >> 28) << 4)) & 127u;
+}
+
+static float q_round_sr(int kind, float v, unsigned int r)
+{
+    unsigned int u;
+    memcpy(&u, &v, 4);
+    if (kind == 2) { /* BF16: add r to the 16 bits below the mantissa, truncate */
+        unsigned int b = (u + ((r << 9) | 256u)) & 0xffff0000u;
+        float o;
+        memcpy(&o, &b, 4);
+        return o;
+    }
+    /* FP16 */
+    unsigned int sgn = u & 0x80000000u, a = u & 0x7fffffffu;
+    unsigned int R = (r << 25) | (1u << 24);
+    if (a >= 0x38800000u) {
+        unsigned int b = sgn | ((a + (R >> 19)) & ~0x1fffu);
+        float q;
+        memcpy(&q, &b, 4);
+        return ffbo_half_to_float(ffbo_float_to_half(q)); /* exact (or overflow to inf) */
+    }
+    int e = (int)(a >> 23);
+    int k = 126 - e;                                     /* discarded bits, >= 14 */
+    unsigned long long m = (a & 0x7fffffu) | (e ? 0x800000u : 0u);
+    unsigned int q;
+    if (k <= 32) q = (unsigned int)((m + (R >> (32 - k))) >> k);
+    else if (k < 56) q = (unsigned int)((m + ((unsigned long long)R << (k - 32))) >> k);
+    else q = 0;
+    return ffbo_half_to_float((unsigned short)((sgn >> 16) | q));
+}
+
+/* The stochastic rounding of one value with random bits r (0..127): the
+ * stored value decoded to float (kind 1 FP16, 2 BF16). */
+float ffbo_round_sr(int kind, float v, unsigned int r) { return q_round_sr(kind, v, r & 127u); }
+
 /* Stored form of the populations of a float halo array (initial state):
  * D = round((f - w) * 2^s), every entry. */
 void ffbo_q_encode(int kind, int scale_log2, int nx, int ny, const float *f, float *D)
@@ -227,24 +287,29 @@ static void q_macro_strip(int nx, int ny, int x, const float *f, float inv, floa
     }
 }
 
-/* Collision in deviation space.  With Sum w = 1 and Sum w c = 0:
+/* Collision in deviation space (the factor omega w_i 2^s carried as a float
+ * pair kw + kw_lo from the exact weight).  With Sum w = 1 and Sum w c = 0:
  * rho - 1 = 2^-s Sum D, rho u = 2^-s Sum c D (+ F/2), and
  * 2^s (f_eq_i - w_i) = 2^s w_i ((rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)),
  * so D'_i = (1 - omega) D_i + (omega w_i 2^s) inner_i, computed pairwise for
  * opposite directions; body force: plus the Guo source
  * 2^s w_i (1 - omega/2) [3 (c_i - u) + 9 (c_i.u) c_i] . F. */
-static void q_kernel(int kind, int guo, int nx, int ny, int write_macro, float omega, float force,
-                     float scale, float inv, const float *f, float *f_next,
+static void q_kernel(int kind, int stochastic, unsigned int step, int guo, int nx, int ny, int write_macro,
+                     float omega, float force, float scale, float inv, const float *f, float *f_next,
                      float *rho_o, float *ux_o, float *uy_o)
 {
     const long pitch = nx + 2;
     const size_t plane = (size_t)(nx + 2) * (size_t)(ny + 2);
-    float kw[9], kf[9];
+    float kw[9], kw_lo[9], kf[9];
     const float om1 = 1.0f - omega;
     const float pre = (1.0f - 0.5f * omega) * force;
     const float half_force = 0.5f * force;
     for (int d = 0; d < 9; ++d) {
-        kw[d] = (omega * ffbo_s_w(d + 1)) * scale;
+        /* omega w_d 2^s with the exact weight, as a float pair from a double product */
+        const double wd = d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
+        const double kd = (double)omega * wd * (double)scale;
+        kw[d] = (float)kd;
+        kw_lo[d] = (float)(kd - (double)kw[d]);
         kf[d] = (ffbo_s_w(d + 1) * pre) * scale;
     }
 #ifdef _OPENMP
@@ -277,13 +342,14 @@ static void q_kernel(int kind, int guo, int nx, int ny, int write_macro, float o
             float h5 = h1 + h2, h8 = h1 - h2;
             static const int pd[4] = {1, 2, 5, 8}, pe[4] = {3, 4, 7, 6};
             float hs[4] = {h1, h2, h5, h8};
-            D[0] = fmaf(om1, D[0], kw[0] * base);
+            D[0] = fmaf(om1, D[0], fmaf(kw[0], base, kw_lo[0] * base));
             for (int k = 0; k < 4; ++k) {
                 const int d = pd[k], e = pe[k];
                 float p = fmaf(hr, hs[k] * hs[k], base);
                 float q = rho * hs[k];
-                D[d] = fmaf(om1, D[d], kw[d] * (p + q));
-                D[e] = fmaf(om1, D[e], kw[e] * (p - q));
+                const float pq = p + q, pm = p - q;
+                D[d] = fmaf(om1, D[d], fmaf(kw[d], pq, kw_lo[d] * pq));
+                D[e] = fmaf(om1, D[e], fmaf(kw[e], pm, kw_lo[e] * pm));
             }
             if (guo) {
                 float g = 3.0f * ux;
@@ -297,16 +363,24 @@ static void q_kernel(int kind, int guo, int nx, int ny, int write_macro, float o
                 D[8] = fmaf(kf[8], (3.0f * h8 + 3.0f) - g, D[8]);
                 D[6] = fmaf(kf[6], (3.0f * h8 - 3.0f) - g, D[6]);
             }
-            for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round(kind, D[d]);
+            if (stochastic && (kind == 1 || kind == 2)) {
+                unsigned long long cell = (unsigned long long)(y - 1) * (unsigned long long)nx + (unsigned)(x - 1);
+                unsigned int c = (unsigned int)cell ^ (unsigned int)(cell >> 32);
+                unsigned int h0 = q_fmix32(c ^ (step * 0x9e3779b9u));
+                unsigned int h1 = q_fmix32(h0 + 0x632be5abu);
+                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round_sr(kind, D[d], q_pop_bits(h0, h1, d));
+            } else {
+                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round(kind, D[d]);
+            }
         }
     }
 }
 
 /* One step of the deviation form: the driver order of ffbo_s_step (exchange,
  * PF pre-pass and strips, mode halo prep, kernel) on D. */
-void ffbo_q_step(int kind, int scale_log2, int mode, int nx, int ny, float omega, float rho_0,
-                 float u_wall, float u_lid, float rho_in, float rho_out, float *f, float *f_next,
-                 float *pf_state, int write_macro, float *rho, float *u_x, float *u_y, float force)
+void ffbo_q_step(int kind, int scale_log2, int stochastic, unsigned int step, int mode, int nx, int ny,
+                 float omega, float rho_0, float u_wall, float u_lid, float rho_in, float rho_out, float *f,
+                 float *f_next, float *pf_state, int write_macro, float *rho, float *u_x, float *u_y, float force)
 {
     const float scale = ldexpf(1.0f, scale_log2), inv = ldexpf(1.0f, -scale_log2);
     ffbo_s_exchange_1domain(mode, nx, ny, f);
@@ -331,20 +405,22 @@ void ffbo_q_step(int kind, int scale_log2, int mode, int nx, int ny, float omega
         break;
     default: ffbo_s_halos_sl(nx, ny, rho_0, u_lid * scale, f); break;
     }
-    q_kernel(kind, mode == 5, nx, ny, write_macro, omega, force, scale, inv, f, f_next, rho, u_x, u_y);
+    q_kernel(kind, stochastic, step, mode == 5, nx, ny, write_macro, omega, force, scale, inv, f, f_next, rho,
+             u_x, u_y);
 }
 
-/* n_steps of the deviation form with ping-pong buffers; final D in f_a. */
-void ffbo_q_run(int kind, int scale_log2, int mode, int nx, int ny, int n_steps, float omega,
-                float rho_0, float u_wall, float u_lid, float rho_in, float rho_out, float *f_a,
-                float *f_b, float *pf_state, int write_macro_last, float *rho, float *u_x, float *u_y,
-                float force)
+/* n_steps of the deviation form with ping-pong buffers; final D in f_a.
+ * steps_done = steps taken before (the stochastic-rounding step index). */
+void ffbo_q_run(int kind, int scale_log2, int stochastic, int steps_done, int mode, int nx, int ny,
+                int n_steps, float omega, float rho_0, float u_wall, float u_lid, float rho_in,
+                float rho_out, float *f_a, float *f_b, float *pf_state, int write_macro_last, float *rho,
+                float *u_x, float *u_y, float force)
 {
     float *cur = f_a, *nxt = f_b;
     for (int step = 1; step <= n_steps; ++step) {
         int wm = write_macro_last && step == n_steps;
-        ffbo_q_step(kind, scale_log2, mode, nx, ny, omega, rho_0, u_wall, u_lid, rho_in, rho_out, cur, nxt,
-                    pf_state, wm, rho, u_x, u_y, force);
+        ffbo_q_step(kind, scale_log2, stochastic, (unsigned int)(steps_done + step), mode, nx, ny, omega, rho_0,
+                    u_wall, u_lid, rho_in, rho_out, cur, nxt, pf_state, wm, rho, u_x, u_y, force);
         float *t = cur;
         cur = nxt;
         nxt = t;

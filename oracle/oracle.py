@@ -54,7 +54,7 @@ def lib() -> ctypes.CDLL:
             f.argtypes = [i, real, real, ptr]
             f.restype = None
         f = _lib.ffbo_q_run
-        f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_int] * 4 + [ctypes.c_float] * 6 + \
+        f.argtypes = [ctypes.c_int] * 4 + [ctypes.c_int] * 4 + [ctypes.c_float] * 6 + \
             [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_float]
         f.restype = None
         _lib.ffbo_q_encode.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2
@@ -69,6 +69,8 @@ def lib() -> ctypes.CDLL:
         _lib.ffbo_float_to_bf16.restype = ctypes.c_ushort
         _lib.ffbo_bf16_to_float.argtypes = [ctypes.c_ushort]
         _lib.ffbo_bf16_to_float.restype = ctypes.c_float
+        _lib.ffbo_round_sr.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_uint]
+        _lib.ffbo_round_sr.restype = ctypes.c_float
         _lib.ffbo_set_threads.argtypes = [ctypes.c_int]
         _lib.ffbo_max_threads.restype = ctypes.c_int
     return _lib
@@ -105,10 +107,12 @@ class OracleLattice:
     2^scale_log2 stored as FP32 / FP16 / BF16, every step works on D in FP32
     (deviation-space halo rules and collision, ffbo_q_step), and ``f`` holds
     the populations decoded as the GPU returns them, w + D * 2^-scale_log2 in
-    FP32."""
+    FP32.  ``rounding`` ("stochastic", the GPU default, or "nearest") applies
+    to the 16-bit forms."""
 
     def __init__(self, mode: int, nx: int, ny: int, params: Params | None = None,
-                 dtype=np.float64, storage: str | None = None, scale_log2: int = 0):
+                 dtype=np.float64, storage: str | None = None, scale_log2: int = 0,
+                 rounding: str = "stochastic"):
         self.mode, self.nx, self.ny = int(mode), int(nx), int(ny)
         self.params = params or Params()
         if storage is not None:
@@ -116,6 +120,9 @@ class OracleLattice:
                 raise ValueError(f"unknown storage {storage!r}")
             dtype = np.float32
         self.storage, self.scale_log2 = storage, int(scale_log2)
+        if rounding not in ("stochastic", "nearest"):
+            raise ValueError(f"unknown rounding {rounding!r}")
+        self.stochastic = 1 if rounding == "stochastic" else 0
         self.dtype = np.dtype(dtype)
         self.p = "d" if self.dtype == np.float64 else "s"
         self.f = np.zeros((9, ny + 2, nx + 2), self.dtype)
@@ -146,8 +153,8 @@ class OracleLattice:
         P = self.params
         if self.storage is not None:
             lib().ffbo_q_run(
-                STORAGE_KINDS[self.storage], self.scale_log2, self.mode, self.nx, self.ny,
-                int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
+                STORAGE_KINDS[self.storage], self.scale_log2, self.stochastic, self.steps, self.mode,
+                self.nx, self.ny, int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
                 _p(self.d), _p(self.d_b), _p(self.pf_state), 1 if write_macro_last else 0,
                 _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
             self._decode()
@@ -201,6 +208,11 @@ def bf16_bits(x: float) -> int:
 
 def bf16_value(bits: int) -> float:
     return float(lib().ffbo_bf16_to_float(int(bits)))
+
+
+def round_sr(kind: str, x: float, r: int) -> float:
+    """Stochastic rounding of x to "fp16" or "bf16" with random bits r (0..127)."""
+    return float(lib().ffbo_round_sr({"fp16": 1, "bf16": 2}[kind], float(x), int(r)))
 
 
 def set_threads(n: int) -> None:

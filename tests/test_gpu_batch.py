@@ -66,7 +66,7 @@ def test_batch_autotune_reinitialises_and_keeps_results():
     lats = small_batch()
     with Batch(lats, "fp32") as b:
         best = b.autotune(4)
-        assert best["tile_w"] in (128, 256, 512, 1024) and best["ms_per_step"] > 0
+        assert best["tile_w"] in (64, 128, 256, 512, 1024) and best["ms_per_step"] > 0
         b.step(9, write_macro_last=True)
         for i, L in enumerate(lats):
             np.testing.assert_array_equal(b.populations(i)[:, 1:-1, 1:-1],

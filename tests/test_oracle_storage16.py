@@ -1,6 +1,8 @@
 """Pins of the oracle's deviation forms (the precision form): the binary16
 conversions against numpy's float16 (an independent IEEE implementation,
 round to nearest even), the bfloat16 rounding against its definition, the
+stochastic rounding against its definition (two neighbours, unbiased over its
+random bits) and its effect on a slowly decaying wave, the
 deviation-space algebra (halo rules and collision on D = (f - w) 2^s) against
 the FP64 restatement of the reference in every mode -- unrounded FP32
 deviations must land within FP32 rounding of it, so a dropped term, a wrong
@@ -115,3 +117,62 @@ def test_deviation_encode_decode_round_trip():
         assert np.all(err <= dev * rel + 1e-7)
     rest = OracleLattice(4, 8, 8, Params(), storage="fp32dev").init()
     assert np.all(rest.d[:, 1:-1, 1:-1] == 0)
+
+
+def _neighbours16(x, kind):
+    """The two representable FP16 / BF16 values around x (float32 in)."""
+    if kind == "fp16":
+        h = np.float16(x)
+        lo = h if float(h) <= x else np.nextafter(h, np.float16(-np.inf))
+        hi = np.nextafter(lo, np.float16(np.inf)) if float(lo) < x else lo
+        return float(lo), float(hi)
+    b = O.bf16_bits(x)
+    v = O.bf16_value(b)
+    if v == x:
+        return v, v
+    if (v < x) == (x > 0):
+        lo, hi = sorted((v, O.bf16_value(b + 1)))
+    else:
+        lo, hi = sorted((v, O.bf16_value(b - 1)))
+    return lo, hi
+
+
+@pytest.mark.parametrize("kind", ["fp16", "bf16"])
+def test_stochastic_rounding_is_unbiased_over_its_random_bits(kind):
+    """For every random value r = 0..127 the result is one of the two
+    neighbours of x, and the mean over all r lies within 1/256 of the
+    neighbour spacing of x (7 random bits): an unbiased rounding, unlike
+    nearest-even, which always returns the same neighbour."""
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-3000, 3000, 300), rng.uniform(-1, 1, 300) * 1e-3,
+                         rng.uniform(-1, 1, 200) * 5e-6, rng.uniform(-1, 1, 100) * 5e-8]).astype(np.float32)
+    for x in xs:
+        x = float(x)
+        lo, hi = _neighbours16(x, kind)
+        outs = np.array([O.round_sr(kind, x, r) for r in range(128)])
+        assert np.all((outs == lo) | (outs == hi)), (x, lo, hi, set(outs))
+        if hi > lo:
+            assert abs(outs.mean() - x) <= (hi - lo) / 256 + 1e-12 * abs(x), (x, outs.mean(), lo, hi)
+
+
+def test_stochastic_rounding_keeps_slow_decay_that_nearest_even_freezes():
+    """Shear wave 1x256, omega 1, u_max 0.05: an update of ~1e-4 of the
+    amplitude per step is a fifth of an FP16 step.  Nearest-even returns the
+    old value every step (the wave never decays); stochastic rounding keeps
+    the decay, nu within 1e-3 of (1/omega - 1/2)/3."""
+    from fastfortranboltzmann_amd.analysis import decay_amplitude, fit_viscosity
+
+    p = Params(omega=1.0, u_max=0.05, n_sin=1.0)
+    res = {}
+    for rounding in ("nearest", "stochastic"):
+        o = OracleLattice(1, 1, 256, p, storage="fp16dev", scale_log2=17, rounding=rounding).init()
+        amps, times = [], []
+        for t in range(200, 2001, 200):
+            o.run(200)
+            amps.append(decay_amplitude(o.u_x.astype(np.float64), 1.0))
+            times.append(t)
+        res[rounding] = (fit_viscosity(np.array(times), np.array(amps), 256, 1.0), amps[-1] / 0.05)
+    nu = 1.0 / 6.0
+    assert abs(res["nearest"][0]) < 1e-9 and res["nearest"][1] > 0.999
+    assert abs(res["stochastic"][0] - nu) / nu < 1e-3
+    assert abs(res["stochastic"][1] - np.exp(-nu * (2 * np.pi / 256) ** 2 * 2000)) < 2e-3
