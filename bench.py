@@ -95,7 +95,8 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
                      "achieved_gbs": round(rates[form] * 1e6 * 18 * BYTES_PER_POP[form] / 1e9, 1),
                      "scale_log2": s, "rel_l2_u_vs_fp64": c["rel_l2_u"], "max_abs_rho_vs_fp64": c["max_abs_rho"],
                      "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"], "mass_drift": r["mass_drift"],
-                     "shear_wave_nu_rel_err": sw["nu_rel_err"]}
+                     "shear_wave_nu_rel_err": sw["nu_rel_err"],
+                     "rounding": "stochastic" if form in ("fp16dev", "bf16dev") else "nearest"}
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
     return out
