@@ -32,7 +32,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2, "fp32dev": 4}
+BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2, "fp32dev": 4, "e3m12dev": 2}
 
 
 def cavity_omega(re: float = 1000.0, u: float = 0.1, n: int = 4096) -> float:
@@ -73,13 +73,14 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
     """Deviation storage forms: MLUPS at n^2 (cavity, Re 5000) against FP32 at
     the same size, and the relative L2 velocity error and mass drift against
     FP64 on a smaller cavity run (err_n^2, Re 1000, err_steps steps), for the
-    16-bit forms, FP32 deviations and plain FP32; for each form also the density
+    16-bit forms (FP16, BF16, the 1-3-12 layout), FP32 deviations and plain
+    FP32; for each form also the density
     error and the relative error of the shear-wave viscosity fitted in that form
     (256^2, omega 1, 2000 steps) against nu = (1/omega - 1/2)/3."""
     from fastfortranboltzmann_amd.precision import (cavity_settings, compare, default_scale_log2,
                                                     run_form, shear_wave_viscosity)
 
-    forms = ("fp32", "fp32dev", "fp16dev", "bf16dev")
+    forms = ("fp32", "fp32dev", "fp16dev", "bf16dev", "e3m12dev")
     out = {"size": n, "steps": steps, "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
     rates = {}
     for form in forms:
@@ -96,7 +97,7 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
                      "scale_log2": s, "rel_l2_u_vs_fp64": c["rel_l2_u"], "max_abs_rho_vs_fp64": c["max_abs_rho"],
                      "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"], "mass_drift": r["mass_drift"],
                      "shear_wave_nu_rel_err": sw["nu_rel_err"],
-                     "rounding": "stochastic" if form in ("fp16dev", "bf16dev") else "nearest"}
+                     "rounding": "stochastic" if form in ("fp16dev", "bf16dev", "e3m12dev") else "nearest"}
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
     return out
@@ -289,7 +290,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp64": "f64", "fp16dev": "f16-storage/f32", "bf16dev": "bf16-storage/f32",
-                  "fp32dev": "f32-deviation"}[args.precision],
+                  "fp32dev": "f32-deviation", "e3m12dev": "e3m12-storage/f32"}[args.precision],
         "data": "synthetic (fluid at rest, deterministic initial state)",
         "config": {"workload": f"lid-driven cavity {n}x{n}, bounce-back walls + moving lid u=0.1, "
                                f"Re=1000 (omega={cavity_omega(n=n):.6f}), {args.precision} populations",

@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libffb_hip.so")
 
 SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID, BODY_FORCE_CHANNEL = 1, 2, 3, 4, 5
-PREC_FP32, PREC_FP64, PREC_FP16_DEV, PREC_BF16_DEV, PREC_FP32_DEV = 0, 1, 2, 3, 4
+PREC_FP32, PREC_FP64, PREC_FP16_DEV, PREC_BF16_DEV, PREC_FP32_DEV, PREC_E3M12_DEV = 0, 1, 2, 3, 4, 5
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 
 # every entry point include/ffb.h declares

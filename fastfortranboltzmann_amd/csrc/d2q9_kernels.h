@@ -2,10 +2,11 @@
 // for gfx950 (MI355X), shared by the single-domain library (ffb_api.hip) and
 // the batched launcher.
 //
-// Data layout in HBM (one domain):
-//   populations: 9 planes ("SoA"), plane d holds f_d(x, y) at d*plane + y*pitch + x,
-//   x fastest; pitch is n_x rounded up to 64 elements so every row starts on a
-//   256-byte boundary; no halo.  Two such sets ping-pong between steps.
+// Data layout in HBM (one domain): f_d(x, y) at d*plane + y*pitch + x, x
+// fastest, no halo; two such sets ping-pong between steps.  Row-blocked
+// (default): plane = n_x rounded up to 64 elements and pitch = 9*plane, so
+// row y of all nine directions is one contiguous block.  Planes: pitch = n_x
+// rounded up to 64 elements, plane = pitch*n_y (nine separate planes).
 // Direction index d = 0..8 is the reference's i = 1..9 minus one
 // (source/settings.f90:96-113): 0 rest, 1 (1,0), 2 (0,1), 3 (-1,0), 4 (0,-1),
 // 5 (1,1), 6 (-1,1), 7 (-1,-1), 8 (1,-1).
@@ -49,10 +50,10 @@ static_assert(opp_of(5) == 7 && opp_of(6) == 8 && opp_of(7) == 5 && opp_of(8) ==
 
 // ---------------------------------------------------------------------------
 // storage of one population value.  FP64 / FP32 store f itself and the step
-// works on f.  The deviation forms store D = (f - w) * 2^s (FP16, BF16 or
-// FP32); the step reads D as FP32, works on D throughout (boundary rules and
-// collision in deviation space, collide_dev) and rounds the new D to the
-// storage type.  `in` gives the value
+// works on f.  The deviation forms store D = (f - w) * 2^s (FP16, BF16, FP32
+// or the 1-3-12 layout); the step reads D as FP32, works on D throughout
+// (boundary rules and collision in deviation space, collide_dev) and rounds
+// the new D to the storage type.  `in` gives the value
 // the step works on, `out` the stored form of it, `from_f` turns a population
 // f into that value (initial state).
 // ---------------------------------------------------------------------------
@@ -156,6 +157,51 @@ template <> struct Store<__hip_bfloat16> {
         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
         const b2 h = __builtin_amdgcn_cvt_sr_bf16_f32(b2{}, v, (r << 25) | (1u << 24), false);
         return __builtin_bit_cast(R, h[0]);
+    }
+    static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
+};
+
+// 16-bit deviations in a 1-3-12 float layout (FFB_PREC_E3M12_DEV): sign bit 15,
+// a 3-bit exponent and a 12-bit significand with subnormals, i.e. the FP32
+// bit pattern of (D * 2^-120) with the exponent field limited to 0..7 and the
+// low 11 significand bits dropped.  D * 2^-120 spans [2^-138, 2^-119): seven
+// binades of 12-bit precision below the maximum (2 - 2^-12) * 2^-120 and a
+// uniform step of 2^-138 under them; scaled by 2^(120-s) the stored deviation
+// f - w covers |f - w| < 2^(1-s) with 12-bit precision down to 2^(-5-s).
+// The step works on the raw FP32 value D * 2^-120 (the deviation form with
+// scale 2^(s-120), so values under 2^-126 are FP32 subnormals): decoding is a
+// shift and a mask, encoding clamps to the largest magnitude and drops 11
+// bits (rounded).  Two more significand bits than FP16 where deviations are
+// small and bounded, as in a flow with known largest speed.
+struct e3m12 {};
+template <> struct Store<e3m12> {
+    using C = float;
+    using R = unsigned short;
+    static constexpr bool dev = true;
+    static constexpr unsigned kMaxBits = 0x03fff800u;  // (2 - 2^-12) * 2^-120
+    static __device__ __forceinline__ C in(R r) {
+        // sign-extended: bits 11..25 get the magnitude, bit 31 the sign
+        return __uint_as_float(((unsigned)(int)(short)r << 11) & 0x83ffffffu);
+    }
+    static __device__ __forceinline__ unsigned clamped(C v) {
+        const float m = __uint_as_float(kMaxBits);
+        return __float_as_uint(__builtin_amdgcn_fmed3f(v, -m, m));
+    }
+    // magnitude bits 11..25 -> 0..14, sign bit 31 -> 15
+    static __device__ __forceinline__ R pack(unsigned u) {
+        return (R)(((u >> 11) & 0x7fffu) | ((u >> 16) & 0x8000u));
+    }
+    // to nearest, ties to even (on the magnitude bits; the clamp leaves room
+    // for the carry)
+    static __device__ __forceinline__ R out(C v) {
+        const unsigned u = clamped(v);
+        return pack(u + 0x3ffu + ((u >> 11) & 1u));
+    }
+    // Stochastic rounding with 7 random bits r: (r << 4) | 8 is added to the
+    // 11 dropped bits before they are cut, as the FP16/BF16 hardware
+    // conversions add r and a half in the top discarded bits.
+    static __device__ __forceinline__ R out_sr(C v, unsigned r) {
+        return pack(clamped(v) + ((r << 4) | 8u));
     }
     static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
@@ -453,34 +499,85 @@ __device__ __forceinline__ void collide_mode(const StepArgs<C> &a, C (&f)[9], in
 __host__ __device__ constexpr int pull_dx(int d) { return -cx_of(d); }
 __host__ __device__ constexpr int pull_dy(int d) { return -cy_of(d); }
 
+// Row-relative addressing.  A tile (or a batched row segment) lies in one row,
+// so the address of that row in every direction set is the same for the whole
+// wave (scalar registers) and each lane adds one 32-bit byte offset: the
+// global loads and stores take the scalar-base form, with no 64-bit address
+// arithmetic per access.  `row` may point before the array when the access
+// itself stays inside it (a pull from row y-1 of an interior cell).
+template <typename R>
+__device__ __forceinline__ const R *at(const R *row, unsigned xb) {
+    return reinterpret_cast<const R *>(reinterpret_cast<const char *>(row) + xb);
+}
+template <typename R>
+__device__ __forceinline__ R *at(R *row, unsigned xb) {
+    return reinterpret_cast<R *>(reinterpret_cast<char *>(row) + xb);
+}
+
+// Raw buffer access of one direction row: a buffer descriptor whose base is the
+// row address (wave-uniform, built in scalar registers; word 3 0x00020000 is
+// the raw 32-bit data format of gfx9 parts) and one 32-bit byte offset per
+// lane.  The x shift of a pull (+-1 element) is folded into the base, so the
+// accesses of a cell share one offset register.  num_records = 2^32-1
+// disables the range check: every access made this way lies inside its array.
+// Measured on MI355X (8192^2 cavity): buffer stores beat global stores for
+// every element size; buffer loads beat global loads for 4- and 8-byte
+// elements (FP32 192.5 against 193.6 us at 4096^2) but are 12 % slower for
+// 2-byte ones (FP16 450 against 394 us per step), which keep global loads.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *row) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), 0, -1, 0x00020000);
+}
+template <typename R>
+__device__ __forceinline__ R buf_ld(__amdgpu_buffer_rsrc_t r, unsigned xb) {
+    if constexpr (sizeof(R) == 2) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b16(r, xb, 0, 0));
+    else if constexpr (sizeof(R) == 4) return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, xb, 0, 0));
+    else return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, xb, 0, 0));
+}
+// Nontemporal store (cache policy bit 1, "nt").
+template <typename R>
+__device__ __forceinline__ void buf_st_nt(R v, __amdgpu_buffer_rsrc_t r, unsigned xb) {
+    if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, xb, 0, 2);
+    else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, xb, 0, 2);
+    else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, 0, 2);
+    }
+}
+
 // One cell, any position: plain pull in the interior, boundary rule on the edge.
 template <typename S, typename C, int MODE>
 __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
                                           typename Store<S>::R *__restrict__ g, int x, int y) {
-    const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
-    const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
+    using R = typename Store<S>::R;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    const R *fr = f + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
+    R *gr = g + (ptrdiff_t)y * W;
+    const unsigned xb = (unsigned)x * (unsigned)sizeof(R);
     const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
     C v[9];
     if (!edge) {
 #pragma unroll
-        for (int d = 0; d < 9; ++d)
-            v[d] = ld_pop<S>(f, (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W + pull_dx(d));
+        for (int d = 0; d < 9; ++d) {
+            const R *row = fr + d * P + pull_dy(d) * W + pull_dx(d);
+            if constexpr (sizeof(R) >= 4) v[d] = Store<S>::in(buf_ld<R>(row_rsrc(row), xb));
+            else v[d] = Store<S>::in(*at(row, xb));
+        }
     } else {
         v[0] = LD<S, C>(a, f, 0, x, y);
 #pragma unroll
         for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
     }
     collide_mode<S, C, MODE>(a, v, x, y);
-    if constexpr (Store<S>::dev && sizeof(typename Store<S>::R) == 2) {
+    if constexpr (Store<S>::dev && sizeof(R) == 2) {
         if (a.stochastic) {
             const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
 #pragma unroll
-            for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(Store<S>::out_sr(v[d], pop_bits(b, d)), g + (size_t)d * P + c);
+            for (int d = 0; d < 9; ++d) buf_st_nt<R>(Store<S>::out_sr(v[d], pop_bits(b, d)), row_rsrc(gr + d * P), xb);
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) st_pop<S>(g, (size_t)d * P + c, v[d]);
+    for (int d = 0; d < 9; ++d) buf_st_nt<R>(Store<S>::out(v[d]), row_rsrc(gr + d * P), xb);
 }
 
 // One fused pull-stream-collide step over one domain.  Each thread updates V
@@ -508,8 +605,10 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
             if (x0 + k < a.nx) step_cell<S, C, MODE>(a, f, g, x0 + k, y);
         return;
     }
-    const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x0;
-    const size_t P = (size_t)a.plane, W = (size_t)a.pitch;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    const R *fr = f + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
+    R *gr = g + (ptrdiff_t)y * W;
+    const unsigned xb = (unsigned)x0 * (unsigned)sizeof(R);
     // neighbour elements first, then a compiler-only memory barrier (emits no
     // instruction): it keeps these element loads separate from the aligned
     // vector loads below, which the load combiner would otherwise fuse into
@@ -517,7 +616,7 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     R side[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-        const R *row = f + (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W;
+        const R *row = at(fr + d * P + pull_dy(d) * W, xb);
         if (pull_dx(d) < 0) side[d] = row[-1];
         else if (pull_dx(d) > 0) side[d] = row[V];
     }
@@ -525,7 +624,7 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     RV in[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-        const R *row = f + (size_t)d * P + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)W;  // 16-byte aligned
+        const R *row = at(fr + d * P + pull_dy(d) * W, xb);  // 16-byte aligned
         const RV own = *reinterpret_cast<const RV *>(row);
         if (pull_dx(d) == 0) {
             in[d] = own;
@@ -563,7 +662,7 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
         for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d]);
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(g + (size_t)d * P + c));
+    for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(at(gr + d * P, xb)));
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,

@@ -71,6 +71,16 @@ __global__ void __launch_bounds__(256) moments_final_kernel(const double *partia
 // D2Q9 weight to double precision (the deviation forms' collision factors)
 inline double exact_weight(int d) { return d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
 
+inline bool is_16bit_form(int p) {
+    return p == FFB_PREC_FP16_DEV || p == FFB_PREC_BF16_DEV || p == FFB_PREC_E3M12_DEV;
+}
+inline bool is_deviation_form(int p) { return is_16bit_form(p) || p == FFB_PREC_FP32_DEV; }
+// Exponent of the scale the step works in: 2^s, except the 1-3-12 form,
+// whose kernels work on D * 2^-120 (the raw FP32 pattern of the stored value).
+inline int work_scale_log2(const ffb_config_t &c) {
+    return c.precision == FFB_PREC_E3M12_DEV ? c.storage_scale_log2 - 120 : c.storage_scale_log2;
+}
+
 template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macro) {
     StepArgs<C> a{};
     const ffb_config_t &c = L->cfg;
@@ -98,17 +108,17 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     a.w[0] = (C)4 / (C)9;
     for (int d = 1; d <= 4; ++d) a.w[d] = (C)1 / (C)9;
     for (int d = 5; d <= 8; ++d) a.w[d] = (C)1 / (C)36;
-    const bool dev = c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV ||
-                     c.precision == FFB_PREC_FP32_DEV;
-    const int s = dev ? c.storage_scale_log2 : 0;
+    const bool dev = is_deviation_form(c.precision);
+    const int s = dev ? work_scale_log2(c) : 0;
     a.scale = (C)std::ldexp(1.0, s);
     a.inv_scale = (C)std::ldexp(1.0, -s);
     const int cells_per_tile = L->tile_w * cells_per_thread(L);
     a.n_tiles_x = (c.n_x + cells_per_tile - 1) / cells_per_tile;
     a.force = (C)c.body_force;
-    if (dev) {  // the deviation forms work on deviations times 2^s (collide_dev)
-        a.corr8 = a.corr8 * a.scale;
-        a.corr9 = a.corr9 * a.scale;
+    if (dev) {  // the deviation forms work on deviations times 2^s (collide_dev):
+                // ((6*w_6)*rho_0)*(u*2^s), exact wherever u*2^s is an FP32 normal
+        a.corr8 = (C)6 * w6 * rho0 * (u * a.scale);
+        a.corr9 = a.corr8;
     }
     a.om1 = (C)1 - a.omega;
     const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
@@ -119,8 +129,7 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
         a.kf[d] = (a.w[d] * pre) * a.scale;
     }
     a.half_force = (C)0.5 * a.force;
-    a.stochastic = (c.precision == FFB_PREC_FP16_DEV || c.precision == FFB_PREC_BF16_DEV) &&
-                   c.storage_rounding == FFB_ROUND_STOCHASTIC;
+    a.stochastic = is_16bit_form(c.precision) && c.storage_rounding == FFB_ROUND_STOCHASTIC;
     a.step_seed = (unsigned)(L->steps + 1);  // the step this launch computes
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_group = L->xcd_group;
@@ -198,6 +207,7 @@ int dispatch_step(ffb_lattice *L, int write_macro) {
     case FFB_PREC_FP16_DEV: return launch_step<__half, float>(L, write_macro);
     case FFB_PREC_BF16_DEV: return launch_step<__hip_bfloat16, float>(L, write_macro);
     case FFB_PREC_FP32_DEV: return launch_step<f32dev, float>(L, write_macro);
+    case FFB_PREC_E3M12_DEV: return launch_step<e3m12, float>(L, write_macro);
     default: return launch_step<float, float>(L, write_macro);
     }
 }
@@ -243,6 +253,7 @@ int dispatch_init(ffb_lattice *L) {
     case FFB_PREC_FP16_DEV: return launch_init<__half, float>(L);
     case FFB_PREC_BF16_DEV: return launch_init<__hip_bfloat16, float>(L);
     case FFB_PREC_FP32_DEV: return launch_init<f32dev, float>(L);
+    case FFB_PREC_E3M12_DEV: return launch_init<e3m12, float>(L);
     default: return launch_init<float, float>(L);
     }
 }
@@ -275,6 +286,23 @@ float bf16_to_float(uint16_t b) {
     float f;
     std::memcpy(&f, &u, 4);
     return f;
+}
+// 1-3-12 form: stored bits <-> the FP32 value D * 2^-120 (as Store<e3m12>)
+float e3m12_to_float(uint16_t h) {
+    const uint32_t u = ((uint32_t)(int32_t)(int16_t)h << 11) & 0x83ffffffu;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+uint16_t float_to_e3m12(float v) {  // to nearest, ties to even; clamped to the largest magnitude
+    const uint32_t max_bits = 0x03fff800u;
+    float m;
+    std::memcpy(&m, &max_bits, 4);
+    v = v > m ? m : (v < -m ? -m : v);
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x3ffu + ((u >> 11) & 1u);
+    return (uint16_t)(((u >> 11) & 0x7fffu) | ((u >> 16) & 0x8000u));
 }
 uint16_t float_to_bf16(float f) {
     uint32_t u;
@@ -356,11 +384,9 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         return fail(FFB_E_INVALID, "ffb_create: domain larger than 2^34 cells");
     if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_BODY_FORCE_CHANNEL)
         return fail(FFB_E_INVALID, "ffb_create: unknown sim mode " + std::to_string(cfg->sim_mode));
-    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_FP32_DEV)
+    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_E3M12_DEV)
         return fail(FFB_E_INVALID, "ffb_create: unknown precision " + std::to_string(cfg->precision));
-    if ((cfg->precision == FFB_PREC_FP16_DEV || cfg->precision == FFB_PREC_BF16_DEV ||
-         cfg->precision == FFB_PREC_FP32_DEV) &&
-        (cfg->storage_scale_log2 < -30 || cfg->storage_scale_log2 > 30))
+    if (is_deviation_form(cfg->precision) && (cfg->storage_scale_log2 < -30 || cfg->storage_scale_log2 > 30))
         return fail(FFB_E_INVALID, "ffb_create: storage_scale_log2 outside [-30, 30]");
     if (cfg->storage_rounding != FFB_ROUND_NEAREST && cfg->storage_rounding != FFB_ROUND_STOCHASTIC)
         return fail(FFB_E_INVALID, "ffb_create: storage_rounding must be FFB_ROUND_NEAREST or FFB_ROUND_STOCHASTIC");
@@ -543,7 +569,7 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
     HIPCHK(hipMemcpyAsync(stage.data(), L->buf[L->cur], stage.size(), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
-    const int s = L->cfg.storage_scale_log2;
+    const int s = work_scale_log2(L->cfg);
     for (int d = 0; d < 9; ++d) {
         for (int y = 0; y < ny; ++y) {
             for (int x = 0; x < nx; ++x) {
@@ -557,6 +583,10 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
                     break;
                 case FFB_PREC_FP16_DEV:
                     ((float *)f_halo)[dst] = weightf(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
+                                                                    (float)std::ldexp(1.0, -s);
+                    break;
+                case FFB_PREC_E3M12_DEV:
+                    ((float *)f_halo)[dst] = weightf(d) + e3m12_to_float(((const uint16_t *)stage.data())[src]) *
                                                                     (float)std::ldexp(1.0, -s);
                     break;
                 default:
@@ -587,7 +617,7 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
     const int nx = L->cfg.n_x, ny = L->cfg.n_y;
     std::vector<unsigned char> stage((size_t)L->set_elems * L->elem, 0);
     const size_t hp = (size_t)(nx + 2), hplane = hp * (size_t)(ny + 2);
-    const int s = L->cfg.storage_scale_log2;
+    const int s = work_scale_log2(L->cfg);
     for (int d = 0; d < 9; ++d) {
         for (int y = 0; y < ny; ++y) {
             for (int x = 0; x < nx; ++x) {
@@ -602,6 +632,10 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
                 case FFB_PREC_FP16_DEV:
                     ((uint16_t *)stage.data())[dst] =
                         float_to_half((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
+                    break;
+                case FFB_PREC_E3M12_DEV:
+                    ((uint16_t *)stage.data())[dst] =
+                        float_to_e3m12((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
                     break;
                 default:
                     ((uint16_t *)stage.data())[dst] =
@@ -659,6 +693,7 @@ int ffb_moments(ffb_lattice_t *L, double *mass, double *mom_x, double *mom_y) {
     case FFB_PREC_FP16_DEV: rc = launch_moments<__half, float>(L); break;
     case FFB_PREC_BF16_DEV: rc = launch_moments<__hip_bfloat16, float>(L); break;
     case FFB_PREC_FP32_DEV: rc = launch_moments<f32dev, float>(L); break;
+    case FFB_PREC_E3M12_DEV: rc = launch_moments<e3m12, float>(L); break;
     default: rc = launch_moments<float, float>(L); break;
     }
     if (rc) return rc;

@@ -13,13 +13,14 @@ module ffb_c
         ffb_sync, ffb_read_macros, ffb_read_populations, ffb_elapsed_ms, ffb_reset_timer, &
         ffb_moments, ffb_check, ffb_error_message
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
-    public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV
+    public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV, &
+        FFB_PREC_E3M12_DEV
     public :: FFB_ROUND_NEAREST, FFB_ROUND_STOCHASTIC
 
     integer(c_int32_t), parameter :: FFB_SHEAR_WAVE = 1, FFB_COUETTE_FLOW = 2, &
         FFB_POISEUILLE_FLOW = 3, FFB_SLIDING_LID = 4
     integer(c_int32_t), parameter :: FFB_PREC_FP32 = 0, FFB_PREC_FP64 = 1, &
-        FFB_PREC_FP16_DEV = 2, FFB_PREC_BF16_DEV = 3, FFB_PREC_FP32_DEV = 4
+        FFB_PREC_FP16_DEV = 2, FFB_PREC_BF16_DEV = 3, FFB_PREC_FP32_DEV = 4, FFB_PREC_E3M12_DEV = 5
     integer(c_int32_t), parameter :: FFB_ROUND_NEAREST = 0, FFB_ROUND_STOCHASTIC = 1
 
     ! ffb_config_t, field for field

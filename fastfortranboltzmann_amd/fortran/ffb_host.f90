@@ -65,8 +65,10 @@ program ffb_host
         cfg%precision = FFB_PREC_BF16_DEV
     case ("fp32dev")
         cfg%precision = FFB_PREC_FP32_DEV
+    case ("e3m12dev")
+        cfg%precision = FFB_PREC_E3M12_DEV
     case default
-        error stop "error: FFB_PRECISION must be fp32, fp64, fp16dev, bf16dev or fp32dev"
+        error stop "error: FFB_PRECISION must be fp32, fp64, fp16dev, bf16dev, fp32dev or e3m12dev"
     end select
 
     if (fp64) then

@@ -8,7 +8,7 @@
 ! precision) set the defaults, with the reference's values when absent.
 ! Because the solver is a library, one binary serves every configuration:
 ! an environment variable of the same name (e.g. FFB_N_X=4096) overrides the
-! compiled value at run time.  FFB_PRECISION=fp32|fp64|fp16dev|bf16dev|fp32dev and
+! compiled value at run time.  FFB_PRECISION=fp32|fp64|fp16dev|bf16dev|fp32dev|e3m12dev and
 ! FFB_STORAGE_SCALE_LOG2 (and FFB_STORAGE_ROUNDING=stochastic|nearest for the
 ! 16-bit forms) select the storage forms the reference lacks.
 module ffb_settings

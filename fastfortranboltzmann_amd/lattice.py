@@ -25,12 +25,12 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, POISEUILLE_FLOW, PREC_BF16_DEV,
+from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, POISEUILLE_FLOW, PREC_BF16_DEV, PREC_E3M12_DEV,
                    PREC_FP16_DEV, PREC_FP32, PREC_FP32_DEV, PREC_FP64, SHEAR_WAVE, SLIDING_LID, FfbConfig, check,
                    lib)
 
 PRECISIONS = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp16dev": PREC_FP16_DEV,
-              "bf16dev": PREC_BF16_DEV, "fp32dev": PREC_FP32_DEV}
+              "bf16dev": PREC_BF16_DEV, "fp32dev": PREC_FP32_DEV, "e3m12dev": PREC_E3M12_DEV}
 ROUNDINGS = {"nearest": _lib.ROUND_NEAREST, "stochastic": _lib.ROUND_STOCHASTIC}
 MODES = {"shear_wave": SHEAR_WAVE, "couette_flow": COUETTE_FLOW,
          "poiseuille_flow": POISEUILLE_FLOW, "sliding_lid": SLIDING_LID,

@@ -41,6 +41,15 @@ def default_scale_log2(precision: str, u_ref: float = 0.1, rho_dev: float = 0.05
     deviation."""
     if precision in ("bf16dev", "fp32dev"):
         return 0
+    if precision == "e3m12dev":
+        # 1-3-12 layout: |f - w| < 2^(1-s), 12-bit precision down to 2^(-5-s).
+        # Largest expected deviation per direction class, w (rho_dev + 3 |c| u
+        # + 4.5 |c|^2 u^2 + 1.5 u^2), put at 3/4 of the largest magnitude
+        # (values beyond it are clamped, so the headroom matters).
+        d_max = max((4.0 / 9.0) * (rho_dev + 1.5 * u_ref * u_ref),
+                    (1.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref),
+                    (1.0 / 36.0) * (rho_dev + 6.0 * u_ref + 19.5 * u_ref * u_ref))
+        return int(math.floor(math.log2(1.5 / d_max)))
     if precision != "fp16dev":
         raise ValueError(f"not a deviation form: {precision}")
     d_max = (4.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref)
@@ -119,7 +128,7 @@ def shear_wave_viscosity(precision: str, n: int = 256, omega: float = 1.0, u_max
 
 
 def precision_study(n: int, re: float, steps: int, u_lid: float = 0.1,
-                    forms=("fp16dev", "bf16dev", "fp32", "fp32dev"), scales: dict | None = None,
+                    forms=("fp16dev", "bf16dev", "fp32", "fp32dev", "e3m12dev"), scales: dict | None = None,
                     log=print) -> dict:
     out = {"n": n, "re": re, "u_lid": u_lid, "steps": steps,
            "omega": omega_for_reynolds(re, u_lid, n), "forms": {}}
@@ -151,7 +160,7 @@ def main(argv=None) -> int:
     ap.add_argument("--re", type=float, default=5000.0)
     ap.add_argument("--u-lid", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=50000)
-    ap.add_argument("--forms", default="fp16dev,bf16dev,fp32,fp32dev")
+    ap.add_argument("--forms", default="fp16dev,bf16dev,fp32,fp32dev,e3m12dev")
     ap.add_argument("--scale", action="append", default=[], help="form=s, e.g. fp16dev=16")
     args = ap.parse_args(argv)
     scales = {k: int(v) for k, v in (x.split("=") for x in args.scale)}

@@ -57,12 +57,18 @@ extern "C" {
  * FP32 arithmetic on values of the size of the deviations); they have no
  * reference counterpart.  FFB_PREC_FP32_DEV has the FP32 form's traffic and
  * an error against FP64 about ten times smaller for small velocities and
- * forces. */
+ * forces.  FFB_PREC_E3M12_DEV stores D in 16 bits as sign, 3-bit exponent and
+ * 12-bit significand with subnormals: |f - w| < 2^(1 - storage_scale_log2)
+ * with 12-bit precision down to 2^(-5 - s) and a uniform step of 2^(-17 - s)
+ * below that (two significand bits more than FP16 over the range a flow with
+ * a known largest speed uses); larger deviations are clamped to the largest
+ * magnitude, so s must leave room for the flow's largest |f - w|. */
 #define FFB_PREC_FP32 0
 #define FFB_PREC_FP64 1
 #define FFB_PREC_FP16_DEV 2
 #define FFB_PREC_BF16_DEV 3
 #define FFB_PREC_FP32_DEV 4
+#define FFB_PREC_E3M12_DEV 5
 
 /* rounding of the 16-bit deviation forms' stored values.  Nearest-even loses
  * every update smaller than half an FP16/BF16 step, so slow flows stagnate
@@ -96,7 +102,7 @@ typedef struct ffb_config {
     int32_t storage_scale_log2; /* s of the deviation forms; ignored otherwise */
     int32_t device;       /* HIP device ordinal */
     double body_force;    /* FFB_BODY_FORCE_CHANNEL: force density along x; ignored otherwise */
-    int32_t storage_rounding; /* 16-bit forms: FFB_ROUND_STOCHASTIC or FFB_ROUND_NEAREST */
+    int32_t storage_rounding; /* 16-bit forms (FP16, BF16, 1-3-12): FFB_ROUND_STOCHASTIC or FFB_ROUND_NEAREST */
     int32_t reserved;
 } ffb_config_t;
 

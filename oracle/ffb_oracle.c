@@ -114,10 +114,50 @@ float ffbo_bf16_to_float(unsigned short b)
  * same, since w_i = w_opp(i); the moving-wall and pressure increments are
  * scaled by 2^s), and the collision runs in deviation space with the operation
  * order below.  This restates that algorithm on the halo arrays of the
- * reference form; the state array holds D (float). */
+ * reference form; the state array holds D (float).  Kind 3, the 1-3-12 form,
+ * works on D * 2^-120 (scale_log2 = s - 120 here), whose small values are
+ * FP32 subnormals: the oracle is built without flush-to-zero, as the GPU
+ * kernels run with FP32 denormals. */
+
+/* 1-3-12 form (kind 3): the stored 16 bits are the sign, a 3-bit exponent and
+ * a 12-bit significand of the FP32 value v = D * 2^-120 the step works on
+ * (the oracle's state holds v): FP32 bits 11..25 and 31, i.e. magnitudes
+ * k * 2^-138 (k < 2^12, subnormal) and (1 + k 2^-12) 2^(e-127), e = 1..7.
+ * Encoding clamps |v| to the largest magnitude (bits 0x03fff800), then adds
+ * `add` to the FP32 bits and cuts the low 11 (the sign bit is untouched: the
+ * clamp leaves room for the carry).  add = 0x3ff + (bit 11) rounds to nearest
+ * even; (r << 4) | 8 with 7 random bits r rounds stochastically. */
+static unsigned short q_e3m12_bits(float v, unsigned int add)
+{
+    const unsigned int max_bits = 0x03fff800u;
+    float m;
+    memcpy(&m, &max_bits, 4);
+    if (v > m) v = m;
+    if (v < -m) v = -m;
+    unsigned int u;
+    memcpy(&u, &v, 4);
+    if (add == 0xffffffffu) add = 0x3ffu + ((u >> 11) & 1u);
+    u += add;
+    return (unsigned short)(((u >> 11) & 0x7fffu) | ((u >> 16) & 0x8000u));
+}
+
+float ffbo_e3m12_to_float(unsigned short h)
+{
+    unsigned int u = ((unsigned int)(h & 0x7fffu) << 11) | ((unsigned int)(h & 0x8000u) << 16);
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+/* Stored bits of v: nearest even (sr = 0) or stochastic with random bits r. */
+unsigned short ffbo_float_to_e3m12(float v, int sr, unsigned int r)
+{
+    return q_e3m12_bits(v, sr ? (((r & 127u) << 4) | 8u) : 0xffffffffu);
+}
 
 static float q_round(int kind, float v)
 {
+    if (kind == 3) return ffbo_e3m12_to_float(q_e3m12_bits(v, 0xffffffffu));
     if (kind == 1) return ffbo_half_to_float(ffbo_float_to_half(v));
     if (kind == 2) return ffbo_bf16_to_float(ffbo_float_to_bf16(v));
     return v;
@@ -152,6 +192,7 @@ static unsigned int q_pop_bits(unsigned int h0, unsigned int h1, int d)
 
 static float q_round_sr(int kind, float v, unsigned int r)
 {
+    if (kind == 3) return ffbo_e3m12_to_float(q_e3m12_bits(v, (r << 4) | 8u));
     unsigned int u;
     memcpy(&u, &v, 4);
     if (kind == 2) { /* BF16: add r to the 16 bits below the mantissa, truncate */
@@ -180,7 +221,7 @@ static float q_round_sr(int kind, float v, unsigned int r)
 }
 
 /* The stochastic rounding of one value with random bits r (0..127): the
- * stored value decoded to float (kind 1 FP16, 2 BF16). */
+ * stored value decoded to float (kind 1 FP16, 2 BF16, 3 the 1-3-12 form). */
 float ffbo_round_sr(int kind, float v, unsigned int r) { return q_round_sr(kind, v, r & 127u); }
 
 /* Stored form of the populations of a float halo array (initial state):
@@ -363,7 +404,7 @@ static void q_kernel(int kind, int stochastic, unsigned int step, int guo, int n
                 D[8] = fmaf(kf[8], (3.0f * h8 + 3.0f) - g, D[8]);
                 D[6] = fmaf(kf[6], (3.0f * h8 - 3.0f) - g, D[6]);
             }
-            if (stochastic && (kind == 1 || kind == 2)) {
+            if (stochastic && kind >= 1) {
                 unsigned long long cell = (unsigned long long)(y - 1) * (unsigned long long)nx + (unsigned)(x - 1);
                 unsigned int c = (unsigned int)cell ^ (unsigned int)(cell >> 32);
                 unsigned int h0 = q_fmix32(c ^ (step * 0x9e3779b9u));

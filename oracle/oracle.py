@@ -71,6 +71,10 @@ def lib() -> ctypes.CDLL:
         _lib.ffbo_bf16_to_float.restype = ctypes.c_float
         _lib.ffbo_round_sr.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_uint]
         _lib.ffbo_round_sr.restype = ctypes.c_float
+        _lib.ffbo_e3m12_to_float.argtypes = [ctypes.c_ushort]
+        _lib.ffbo_e3m12_to_float.restype = ctypes.c_float
+        _lib.ffbo_float_to_e3m12.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_uint]
+        _lib.ffbo_float_to_e3m12.restype = ctypes.c_ushort
         _lib.ffbo_set_threads.argtypes = [ctypes.c_int]
         _lib.ffbo_max_threads.restype = ctypes.c_int
     return _lib
@@ -96,17 +100,18 @@ def _p(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
-STORAGE_KINDS = {"fp32dev": 0, "fp16dev": 1, "bf16dev": 2}
+STORAGE_KINDS = {"fp32dev": 0, "fp16dev": 1, "bf16dev": 2, "e3m12dev": 3}
 
 
 class OracleLattice:
     """One single-domain run of the reference loop on the CPU.
 
-    ``storage`` = "fp32dev" / "fp16dev" / "bf16dev" restates the GPU's
-    deviation forms (no reference counterpart): the state is D = (f - w) *
-    2^scale_log2 stored as FP32 / FP16 / BF16, every step works on D in FP32
-    (deviation-space halo rules and collision, ffbo_q_step), and ``f`` holds
-    the populations decoded as the GPU returns them, w + D * 2^-scale_log2 in
+    ``storage`` = "fp32dev" / "fp16dev" / "bf16dev" / "e3m12dev" restates the
+    GPU's deviation forms (no reference counterpart): the state is D = (f - w)
+    * 2^scale_log2 stored as FP32 / FP16 / BF16 / the 1-3-12 layout, every
+    step works on D in FP32 (deviation-space halo rules and collision,
+    ffbo_q_step; the 1-3-12 form on D * 2^-120), and ``f`` holds the
+    populations decoded as the GPU returns them, w + D * 2^-scale_log2 in
     FP32.  ``rounding`` ("stochastic", the GPU default, or "nearest") applies
     to the 16-bit forms."""
 
@@ -120,6 +125,8 @@ class OracleLattice:
                 raise ValueError(f"unknown storage {storage!r}")
             dtype = np.float32
         self.storage, self.scale_log2 = storage, int(scale_log2)
+        # exponent of the scale the step works in (the 1-3-12 form: D * 2^-120)
+        self.work_log2 = self.scale_log2 - (120 if storage == "e3m12dev" else 0)
         if rounding not in ("stochastic", "nearest"):
             raise ValueError(f"unknown rounding {rounding!r}")
         self.stochastic = 1 if rounding == "stochastic" else 0
@@ -143,7 +150,7 @@ class OracleLattice:
             _p(self.f), _p(self.rho), _p(self.u_x), _p(self.u_y))
         getattr(lib(), f"ffbo_{self.p}_pf_seed")(self.ny, P.rho_in, P.rho_out, _p(self.pf_state))
         if self.storage is not None:
-            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
+            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.work_log2, self.nx, self.ny,
                                 _p(self.f), _p(self.d))
             self._decode()
         self.steps = 0
@@ -153,7 +160,7 @@ class OracleLattice:
         P = self.params
         if self.storage is not None:
             lib().ffbo_q_run(
-                STORAGE_KINDS[self.storage], self.scale_log2, self.stochastic, self.steps, self.mode,
+                STORAGE_KINDS[self.storage], self.work_log2, self.stochastic, self.steps, self.mode,
                 self.nx, self.ny, int(n_steps), P.omega, P.rho_0, P.u_wall, P.u_lid, P.rho_in, P.rho_out,
                 _p(self.d), _p(self.d_b), _p(self.pf_state), 1 if write_macro_last else 0,
                 _p(self.rho), _p(self.u_x), _p(self.u_y), P.force)
@@ -172,13 +179,13 @@ class OracleLattice:
         stores it encoded (D = round((f - w) 2^s)), as ffb_write_populations does."""
         self.f[:] = f_halo.astype(self.dtype)
         if self.storage is not None:
-            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.scale_log2, self.nx, self.ny,
+            lib().ffbo_q_encode(STORAGE_KINDS[self.storage], self.work_log2, self.nx, self.ny,
                                 _p(self.f), _p(self.d))
             self._decode()
         return self
 
     def _decode(self) -> None:
-        lib().ffbo_q_decode(self.scale_log2, self.nx, self.ny, _p(self.d), _p(self.f))
+        lib().ffbo_q_decode(self.work_log2, self.nx, self.ny, _p(self.d), _p(self.f))
 
     @property
     def interior(self) -> np.ndarray:
@@ -211,8 +218,18 @@ def bf16_value(bits: int) -> float:
 
 
 def round_sr(kind: str, x: float, r: int) -> float:
-    """Stochastic rounding of x to "fp16" or "bf16" with random bits r (0..127)."""
-    return float(lib().ffbo_round_sr({"fp16": 1, "bf16": 2}[kind], float(x), int(r)))
+    """Stochastic rounding of x to "fp16", "bf16" or "e3m12" with random bits r (0..127)."""
+    return float(lib().ffbo_round_sr({"fp16": 1, "bf16": 2, "e3m12": 3}[kind], float(x), int(r)))
+
+
+def e3m12_bits(v: float, r: int | None = None) -> int:
+    """Stored bits of the FP32 value v (= D * 2^-120) in the 1-3-12 layout:
+    nearest even, or stochastic with random bits r."""
+    return int(lib().ffbo_float_to_e3m12(float(v), 0 if r is None else 1, 0 if r is None else int(r)))
+
+
+def e3m12_value(bits: int) -> float:
+    return float(lib().ffbo_e3m12_to_float(int(bits)))
 
 
 def set_threads(n: int) -> None:
