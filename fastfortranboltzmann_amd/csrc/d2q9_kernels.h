@@ -27,6 +27,8 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace ffb {
@@ -166,8 +168,9 @@ template <> struct Store<__hip_bfloat16> {
 // bit pattern of (D * 2^-120) with the exponent field limited to 0..7 and the
 // low 11 significand bits dropped.  D * 2^-120 spans [2^-138, 2^-119): seven
 // binades of 12-bit precision below the maximum (2 - 2^-12) * 2^-120 and a
-// uniform step of 2^-138 under them; scaled by 2^(120-s) the stored deviation
-// f - w covers |f - w| < 2^(1-s) with 12-bit precision down to 2^(-5-s).
+// uniform step of 2^-138 under them; scaled by 2^(120-s) (and by the
+// direction's offset 2^off, store_off) the stored deviation f - w covers
+// |f - w| < 2^(1-s-off) with 12-bit precision down to 2^(-5-s-off).
 // The step works on the raw FP32 value D * 2^-120 (the deviation form with
 // scale 2^(s-120), so values under 2^-126 are FP32 subnormals): decoding is a
 // shift and a mask, encoding clamps to the largest magnitude and drops 11
@@ -206,13 +209,41 @@ template <> struct Store<e3m12> {
     static __device__ __forceinline__ C from_f(C f, C w, C scale) { return (f - w) * scale; }
 };
 
+// Exponent offset of direction d in the stored form: direction d stores the
+// value the step works on times 2^off.  Only the 1-3-12 form, whose range is
+// eight binades, uses offsets, one per direction class: the largest |f - w| of
+// a class is its weight times (rho - 1) plus, for the moving directions,
+// 3 |c.u| and smaller terms.  Rest (w = 4/9, no velocity term, but the density
+// peak at a sliding lid's corners, rho - 1 ~ 0.14 for u = 0.1): -1, one binade
+// more range.  Axis (1/9): 0.  Diagonal (1/36): +1, one bit more precision.
+// Scaling by a power of two is exact except where the product is an FP32
+// subnormal (then rounded as FP32 arithmetic rounds it).
 template <typename S>
-__device__ __forceinline__ typename Store<S>::C ld_pop(const typename Store<S>::R *p, size_t k) {
-    return Store<S>::in(p[k]);
+__host__ __device__ constexpr int store_off(int d) {
+    return std::is_same_v<S, e3m12> ? (d == 0 ? -1 : (d <= 4 ? 0 : 1)) : 0;
+}
+__host__ __device__ constexpr float pow2f(int e) { return e >= 0 ? (float)(1u << e) : 1.0f / (float)(1u << -e); }
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C pop_in(typename Store<S>::R r, int d) {
+    const typename Store<S>::C v = Store<S>::in(r);
+    return store_off<S>(d) == 0 ? v : v * pow2f(-store_off<S>(d));
 }
 template <typename S>
-__device__ __forceinline__ void st_pop(typename Store<S>::R *p, size_t k, typename Store<S>::C v) {
-    __builtin_nontemporal_store(Store<S>::out(v), p + k);
+__device__ __forceinline__ typename Store<S>::R pop_out(typename Store<S>::C v, int d) {
+    return Store<S>::out(store_off<S>(d) == 0 ? v : v * pow2f(store_off<S>(d)));
+}
+template <typename S>
+__device__ __forceinline__ typename Store<S>::R pop_out_sr(typename Store<S>::C v, unsigned r, int d) {
+    return Store<S>::out_sr(store_off<S>(d) == 0 ? v : v * pow2f(store_off<S>(d)), r);
+}
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C ld_pop(const typename Store<S>::R *p, size_t k, int d) {
+    return pop_in<S>(p[k], d);
+}
+template <typename S>
+__device__ __forceinline__ void st_pop(typename Store<S>::R *p, size_t k, typename Store<S>::C v, int d) {
+    __builtin_nontemporal_store(pop_out<S>(v, d), p + k);
 }
 
 // Parameters of one step over one domain.
@@ -263,7 +294,7 @@ __device__ __forceinline__ int tile_of_block(int p, int n, int G) {
 
 template <typename S, typename C>
 __device__ __forceinline__ C LD(const StepArgs<C> &a, const typename Store<S>::R *f, int d, int x, int y) {
-    return ld_pop<S>(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x);
+    return ld_pop<S>(f, (size_t)d * (size_t)a.plane + (size_t)y * (size_t)a.pitch + (size_t)x, d);
 }
 
 // Pressure-periodic increment of pressure_periodic_distribution
@@ -559,8 +590,8 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
             const R *row = fr + d * P + pull_dy(d) * W + pull_dx(d);
-            if constexpr (sizeof(R) >= 4) v[d] = Store<S>::in(buf_ld<R>(row_rsrc(row), xb));
-            else v[d] = Store<S>::in(*at(row, xb));
+            if constexpr (sizeof(R) >= 4) v[d] = pop_in<S>(buf_ld<R>(row_rsrc(row), xb), d);
+            else v[d] = pop_in<S>(*at(row, xb), d);
         }
     } else {
         v[0] = LD<S, C>(a, f, 0, x, y);
@@ -572,12 +603,12 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
         if (a.stochastic) {
             const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
 #pragma unroll
-            for (int d = 0; d < 9; ++d) buf_st_nt<R>(Store<S>::out_sr(v[d], pop_bits(b, d)), row_rsrc(gr + d * P), xb);
+            for (int d = 0; d < 9; ++d) buf_st_nt<R>(pop_out_sr<S>(v[d], pop_bits(b, d), d), row_rsrc(gr + d * P), xb);
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) buf_st_nt<R>(Store<S>::out(v[d]), row_rsrc(gr + d * P), xb);
+    for (int d = 0; d < 9; ++d) buf_st_nt<R>(pop_out<S>(v[d], d), row_rsrc(gr + d * P), xb);
 }
 
 // One fused pull-stream-collide step over one domain.  Each thread updates V
@@ -647,19 +678,19 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     for (int k = 0; k < V; ++k) {
         C v[9];
 #pragma unroll
-        for (int d = 0; d < 9; ++d) v[d] = Store<S>::in(in[d][k]);
+        for (int d = 0; d < 9; ++d) v[d] = pop_in<S>(in[d][k], d);
         collide_mode<S, C, MODE>(a, v, x0 + k, y);
         if constexpr (Store<S>::dev && sizeof(R) == 2) {
             if (a.stochastic) {
                 const CellBits b =
                     cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)(x0 + k), a.step_seed);
 #pragma unroll
-                for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out_sr(v[d], pop_bits(b, d));
+                for (int d = 0; d < 9; ++d) out[d][k] = pop_out_sr<S>(v[d], pop_bits(b, d), d);
                 continue;
             }
         }
 #pragma unroll
-        for (int d = 0; d < 9; ++d) out[d][k] = Store<S>::out(v[d]);
+        for (int d = 0; d < 9; ++d) out[d][k] = pop_out<S>(v[d], d);
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(at(gr + d * P, xb)));
@@ -724,7 +755,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepArgs<C> a, C rho_0, const
         } else {
             feq = a.w[d] * rho_0;
         }
-        st_pop<S>(g, (size_t)d * (size_t)a.plane + c, Store<S>::from_f(feq, a.w[d], a.scale));
+        st_pop<S>(g, (size_t)d * (size_t)a.plane + c, Store<S>::from_f(feq, a.w[d], a.scale), d);
     }
     if (a.rho_o) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
@@ -749,9 +780,9 @@ __global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, dou
         for (int d = 0; d < 9; ++d) {
             double v;
             if (Store<S>::dev)
-                v = (double)a.w[d] + (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c) * (double)a.inv_scale;
+                v = (double)a.w[d] + (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, d) * (double)a.inv_scale;
             else
-                v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c);
+                v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, d);
             m += v;
             mx += (double)cx_of(d) * v;
             my += (double)cy_of(d) * v;

@@ -84,7 +84,7 @@ __device__ __forceinline__ void pull_cell(const StepArgs<C> &s, int x, int y, C 
         const size_t c = (size_t)y * (size_t)s.pitch + (size_t)x;
 #pragma unroll
         for (int d = 0; d < 9; ++d)
-            p[d] = ld_pop<S>(f, (size_t)d * (size_t)s.plane + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)s.pitch + pull_dx(d));
+            p[d] = ld_pop<S>(f, (size_t)d * (size_t)s.plane + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)s.pitch + pull_dx(d), d);
     } else {
         p[0] = LD<S, C>(s, f, 0, x, y);
 #pragma unroll

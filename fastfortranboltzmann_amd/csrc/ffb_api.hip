@@ -585,9 +585,10 @@ int ffb_read_populations(ffb_lattice_t *L, void *f_halo) {
                     ((float *)f_halo)[dst] = weightf(d) + half_to_float(((const uint16_t *)stage.data())[src]) *
                                                                     (float)std::ldexp(1.0, -s);
                     break;
-                case FFB_PREC_E3M12_DEV:
-                    ((float *)f_halo)[dst] = weightf(d) + e3m12_to_float(((const uint16_t *)stage.data())[src]) *
-                                                                    (float)std::ldexp(1.0, -s);
+                case FFB_PREC_E3M12_DEV:  // stored value times 2^-off of its direction, as the kernels read it
+                    ((float *)f_halo)[dst] =
+                        weightf(d) + (e3m12_to_float(((const uint16_t *)stage.data())[src]) *
+                                      pow2f(-store_off<e3m12>(d))) * (float)std::ldexp(1.0, -s);
                     break;
                 default:
                     ((float *)f_halo)[dst] = weightf(d) + bf16_to_float(((const uint16_t *)stage.data())[src]) *
@@ -634,8 +635,9 @@ int ffb_write_populations(ffb_lattice_t *L, const void *f_halo) {
                         float_to_half((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
                     break;
                 case FFB_PREC_E3M12_DEV:
-                    ((uint16_t *)stage.data())[dst] =
-                        float_to_e3m12((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s));
+                    ((uint16_t *)stage.data())[dst] = float_to_e3m12(
+                        ((((const float *)f_halo)[src] - weightf(d)) * (float)std::ldexp(1.0, s)) *
+                        pow2f(store_off<e3m12>(d)));
                     break;
                 default:
                     ((uint16_t *)stage.data())[dst] =

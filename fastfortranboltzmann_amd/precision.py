@@ -42,13 +42,14 @@ def default_scale_log2(precision: str, u_ref: float = 0.1, rho_dev: float = 0.05
     if precision in ("bf16dev", "fp32dev"):
         return 0
     if precision == "e3m12dev":
-        # 1-3-12 layout: |f - w| < 2^(1-s), 12-bit precision down to 2^(-5-s).
-        # Largest expected deviation per direction class, w (rho_dev + 3 |c| u
-        # + 4.5 |c|^2 u^2 + 1.5 u^2), put at 3/4 of the largest magnitude
-        # (values beyond it are clamped, so the headroom matters).
-        d_max = max((4.0 / 9.0) * (rho_dev + 1.5 * u_ref * u_ref),
+        # 1-3-12 layout: |f - w| < 2^(1-s-off) for a direction with exponent
+        # offset off (rest -1, axis 0, diagonal +1).  Largest expected
+        # deviation per class, w (rho_dev + 3 |c.u| + 4.5 (c.u)^2 + 1.5 u^2)
+        # with |c.u| <= |c| u, times 2^off, put at 3/4 of the largest
+        # magnitude (values beyond it are clamped, so the headroom matters).
+        d_max = max((4.0 / 9.0) * (rho_dev + 1.5 * u_ref * u_ref) / 2.0,
                     (1.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref),
-                    (1.0 / 36.0) * (rho_dev + 6.0 * u_ref + 19.5 * u_ref * u_ref))
+                    (1.0 / 36.0) * (rho_dev + 3.0 * math.sqrt(2.0) * u_ref + 10.5 * u_ref * u_ref) * 2.0)
         return int(math.floor(math.log2(1.5 / d_max)))
     if precision != "fp16dev":
         raise ValueError(f"not a deviation form: {precision}")

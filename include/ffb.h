@@ -58,11 +58,15 @@ extern "C" {
  * reference counterpart.  FFB_PREC_FP32_DEV has the FP32 form's traffic and
  * an error against FP64 about ten times smaller for small velocities and
  * forces.  FFB_PREC_E3M12_DEV stores D in 16 bits as sign, 3-bit exponent and
- * 12-bit significand with subnormals: |f - w| < 2^(1 - storage_scale_log2)
- * with 12-bit precision down to 2^(-5 - s) and a uniform step of 2^(-17 - s)
- * below that (two significand bits more than FP16 over the range a flow with
- * a known largest speed uses); larger deviations are clamped to the largest
- * magnitude, so s must leave room for the flow's largest |f - w|. */
+ * 12-bit significand with subnormals, with one more binade of range for the
+ * rest population and one more bit of precision for the diagonal ones:
+ * |f - w| < 2^(1 - s) (axis), 2^(2 - s) (rest), 2^-s (diagonal), s =
+ * storage_scale_log2, with 12-bit precision over the top seven binades and a
+ * uniform step below (two significand bits more than FP16 over the range a
+ * flow with a known largest speed uses); larger deviations are clamped to the
+ * largest magnitude, so s must leave room for the flow's largest |f - w|
+ * (s = 5 for speeds up to 0.1, including the density peak at a sliding lid's
+ * corners). */
 #define FFB_PREC_FP32 0
 #define FFB_PREC_FP64 1
 #define FFB_PREC_FP16_DEV 2

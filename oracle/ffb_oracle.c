@@ -155,9 +155,22 @@ unsigned short ffbo_float_to_e3m12(float v, int sr, unsigned int r)
     return q_e3m12_bits(v, sr ? (((r & 127u) << 4) | 8u) : 0xffffffffu);
 }
 
-static float q_round(int kind, float v)
+/* Exponent offset of direction d (0-based) in the 1-3-12 form: the stored
+ * value is v * 2^off (rest -1, axis 0, diagonal +1), read back as
+ * stored * 2^-off; FP32 multiplications, exact except on subnormal products. */
+static int q_off(int kind, int d) { return kind == 3 ? (d == 0 ? -1 : (d <= 4 ? 0 : 1)) : 0; }
+
+static float q_e3m12_round(float v, int d, unsigned int add)
 {
-    if (kind == 3) return ffbo_e3m12_to_float(q_e3m12_bits(v, 0xffffffffu));
+    const int o = q_off(3, d);
+    const float x = o == 0 ? v : v * ldexpf(1.0f, o);
+    const float y = ffbo_e3m12_to_float(q_e3m12_bits(x, add));
+    return o == 0 ? y : y * ldexpf(1.0f, -o);
+}
+
+static float q_round(int kind, float v, int d)
+{
+    if (kind == 3) return q_e3m12_round(v, d, 0xffffffffu);
     if (kind == 1) return ffbo_half_to_float(ffbo_float_to_half(v));
     if (kind == 2) return ffbo_bf16_to_float(ffbo_float_to_bf16(v));
     return v;
@@ -190,9 +203,9 @@ static unsigned int q_pop_bits(unsigned int h0, unsigned int h1, int d)
     return ((h0 >> 28) | ((h1 >> 28) << 4)) & 127u;
 }
 
-static float q_round_sr(int kind, float v, unsigned int r)
+static float q_round_sr(int kind, float v, unsigned int r, int d)
 {
-    if (kind == 3) return ffbo_e3m12_to_float(q_e3m12_bits(v, (r << 4) | 8u));
+    if (kind == 3) return q_e3m12_round(v, d, (r << 4) | 8u);
     unsigned int u;
     memcpy(&u, &v, 4);
     if (kind == 2) { /* BF16: add r to the 16 bits below the mantissa, truncate */
@@ -222,7 +235,7 @@ static float q_round_sr(int kind, float v, unsigned int r)
 
 /* The stochastic rounding of one value with random bits r (0..127): the
  * stored value decoded to float (kind 1 FP16, 2 BF16, 3 the 1-3-12 form). */
-float ffbo_round_sr(int kind, float v, unsigned int r) { return q_round_sr(kind, v, r & 127u); }
+float ffbo_round_sr(int kind, float v, unsigned int r) { return q_round_sr(kind, v, r & 127u, 1); }
 
 /* Stored form of the populations of a float halo array (initial state):
  * D = round((f - w) * 2^s), every entry. */
@@ -234,7 +247,7 @@ void ffbo_q_encode(int kind, int scale_log2, int nx, int ny, const float *f, flo
         const float w = ffbo_s_w(i);
         for (size_t k = 0; k < plane; ++k) {
             const size_t j = (size_t)(i - 1) * plane + k;
-            D[j] = q_round(kind, (f[j] - w) * scale);
+            D[j] = q_round(kind, (f[j] - w) * scale, i - 1);
         }
     }
 }
@@ -409,9 +422,9 @@ static void q_kernel(int kind, int stochastic, unsigned int step, int guo, int n
                 unsigned int c = (unsigned int)cell ^ (unsigned int)(cell >> 32);
                 unsigned int h0 = q_fmix32(c ^ (step * 0x9e3779b9u));
                 unsigned int h1 = q_fmix32(h0 + 0x632be5abu);
-                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round_sr(kind, D[d], q_pop_bits(h0, h1, d));
+                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round_sr(kind, D[d], q_pop_bits(h0, h1, d), d);
             } else {
-                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round(kind, D[d]);
+                for (int d = 0; d < 9; ++d) FQ(f_next, x, y, d + 1) = q_round(kind, D[d], d);
             }
         }
     }

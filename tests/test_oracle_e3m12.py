@@ -120,8 +120,9 @@ def test_e3m12_encode_decode_round_trip_and_rest():
     f32 = OracleLattice(1, 16, 12, p, np.float32).init()
     q = OracleLattice(1, 16, 12, p, storage="e3m12dev", scale_log2=5).init()
     w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, np.float32)[:, None, None]
+    off = np.array([-1] + [0] * 4 + [1] * 4, np.float64)[:, None, None]   # direction offsets
     dev = np.abs(f32.interior.astype(np.float64) - w)
-    step = np.maximum(dev * 2.0 ** -12, 2.0 ** -17 * 2.0 ** -5)   # 12-bit relative, 2^(-17-s) below
+    step = np.maximum(dev * 2.0 ** -12, 2.0 ** (-17 - 5 - off))   # 12-bit relative, 2^(-17-s-off) below
     err = np.abs(q.interior.astype(np.float64) - f32.interior)
     # plus the FP32 rounding of the decoded w + D 2^-s (half an ulp of f < 1)
     assert np.all(err <= step * 0.5 + 2.0 ** -25), float((err - step * 0.5).max())
@@ -149,3 +150,33 @@ def test_stochastic_rounding_keeps_slow_decay():
     nu = 1.0 / 6.0
     assert abs(res["nearest"] - nu) / nu > 0.02, res
     assert abs(res["stochastic"] - nu) / nu < 1e-3, res
+
+
+def test_rest_population_keeps_range_at_the_lid_corner():
+    """Sliding lid u = 0.1 at omega 1.0085 (the 8192^2 Re 5000 cavity's
+    omega): the density peak at the lid's corners, rho - 1 = 0.14, puts the
+    rest deviation at |f_0 - 4/9| = 0.062, at the limit 2^(1-s) = 0.0625 of an
+    axis direction for s = 5.  The rest direction's extra binade (offset -1)
+    keeps it below its own limit 2^(2-s): no stored value is clamped and the
+    mass differs from FP64's by the rounding noise only.  With s = 7 (rest
+    limit 0.031) the corner values are clamped at every step and the run
+    loses mass: the same check fails there."""
+    p = Params(omega=1.008553, u_lid=0.1)
+    ref = OracleLattice(4, 64, 64, p, np.float64).init().run(400)
+    rest_dev = np.abs(ref.interior[0] - 4 / 9).max()
+    assert 0.055 < rest_dev < 0.0625, rest_dev          # the case the offset is for
+    m_ref = ref.rho.astype(np.float64).sum()
+    drift = {}
+    for s in (5, 7):
+        q = OracleLattice(4, 64, 64, p, storage="e3m12dev", scale_log2=s).init().run(400)
+        # the state holds D * 2^-120 = (f - w) 2^(s-120); the rest direction's
+        # largest magnitude there is (2 - 2^-12) 2^-119 for every s
+        top = (2.0 - 2.0 ** -12) * 2.0 ** -119
+        clamped = int((np.abs(q.d[0, 1:-1, 1:-1].astype(np.float64)) == top).sum())
+        drift[s] = (q.rho.astype(np.float64).sum() - m_ref) / m_ref
+        if s == 5:
+            assert clamped == 0
+        else:
+            assert clamped > 0
+    assert abs(drift[5]) < 2e-6, drift                   # noise: ~5e-7 expected here
+    assert drift[7] < -20 * abs(drift[5]), drift
