@@ -196,11 +196,15 @@ static unsigned int q_fmix32(unsigned int h)
     return h;
 }
 
+/* Random bits of population d (0-based): d = 0, 1, 2 from the first word,
+ * d = 5, 6 from the second, 7 bits each; the opposite directions 3, 4, 7, 8
+ * take 127 - r of their partner 1, 2, 5, 6 (antithetic pairs). */
 static unsigned int q_pop_bits(unsigned int h0, unsigned int h1, int d)
 {
+    static const int partner[9] = {0, 1, 2, 1, 2, 5, 6, 5, 6};
+    if (partner[d] != d) return 127u - q_pop_bits(h0, h1, partner[d]);
     if (d < 4) return (h0 >> (7 * d)) & 127u;
-    if (d < 8) return (h1 >> (7 * (d - 4))) & 127u;
-    return ((h0 >> 28) | ((h1 >> 28) << 4)) & 127u;
+    return (h1 >> (7 * (d - 4))) & 127u;
 }
 
 static float q_round_sr(int kind, float v, unsigned int r, int d)

@@ -156,16 +156,18 @@ def test_stochastic_rounding_is_unbiased_over_its_random_bits(kind):
 
 
 def test_stochastic_rounding_keeps_slow_decay_that_nearest_even_freezes():
-    """Shear wave 1x256, omega 1, u_max 0.05: an update of ~1e-4 of the
+    """Shear wave 32x256, omega 1, u_max 0.05: an update of ~1e-4 of the
     amplitude per step is a fifth of an FP16 step.  Nearest-even returns the
     old value every step (the wave never decays); stochastic rounding keeps
-    the decay, nu within 1e-3 of (1/omega - 1/2)/3."""
+    the decay, nu within 1e-3 of (1/omega - 1/2)/3 (2.6e-4 measured; the 32
+    columns average the rounding noise of the fit, which for one column
+    reaches 1.6e-3)."""
     from fastfortranboltzmann_amd.analysis import decay_amplitude, fit_viscosity
 
     p = Params(omega=1.0, u_max=0.05, n_sin=1.0)
     res = {}
     for rounding in ("nearest", "stochastic"):
-        o = OracleLattice(1, 1, 256, p, storage="fp16dev", scale_log2=17, rounding=rounding).init()
+        o = OracleLattice(1, 32, 256, p, storage="fp16dev", scale_log2=17, rounding=rounding).init()
         amps, times = [], []
         for t in range(200, 2001, 200):
             o.run(200)
