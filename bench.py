@@ -214,7 +214,24 @@ def cpu_baseline(precision: str) -> dict | None:
                                   "compiled by flang -O3, one thread = one coarray image"}
         except (subprocess.TimeoutExpired, OSError):
             pass
-    # the CPU oracle (the reference restated in C) on all host cores
+    return cpu_oracle_threads(precision, 10.0)
+
+
+def host_cpu() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_oracle_threads(precision: str, seconds: float = 6.0) -> dict:
+    """The CPU oracle (the reference restated in C, same operation order) on up
+    to 16 host threads (OpenMP over rows), timed for about `seconds` on the
+    same 4096^2 cavity."""
     import numpy as np
 
     from oracle.oracle import OracleLattice, Params, max_threads, set_threads
@@ -225,13 +242,13 @@ def cpu_baseline(precision: str) -> dict | None:
                       np.float64 if precision == "fp64" else np.float32).init()
     o.run(1)
     steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < 10.0:
+    while time.perf_counter() - t0 < seconds:
         o.run(1)
         steps += 1
     dt = time.perf_counter() - t0
-    return {"value": 4096 * 4096 * steps / dt / 1e6, "unit": "MLUPS", "cores": max_threads(),
-            "kind": "port", "sample": f"{steps} steps of the same 4096^2 cavity ({precision}), "
-                                      "CPU oracle, OpenMP"}
+    return {"value": round(4096 * 4096 * steps / dt / 1e6, 1), "unit": "MLUPS", "cores": max_threads(),
+            "kind": "port", "host": host_cpu(),
+            "sample": f"{steps} steps of the same 4096^2 cavity ({precision}), CPU oracle, OpenMP"}
 
 
 def main() -> int:
@@ -278,6 +295,8 @@ def main() -> int:
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.precision)
+        if cpu is not None:
+            cpu["host"] = host_cpu()
     line = {
         "metric": "MLUPS + % HBM roofline (plain FP64/FP32 vs reduced/batched/adjoint) at stated error",
         "value": round(main_run["mlups"], 1),

@@ -79,6 +79,23 @@ def build_fortran_host(verbose: bool = False, defines: list[str] | None = None) 
     return HOST_BIN
 
 
+RUN_BIN = os.path.join(PKG, "bin", "ffb_run")
+
+
+def build_cpp_driver(verbose: bool = False) -> str:
+    """Compile the C++ driver (cpp/ffb_run.cpp) against libffb_hip.so with g++:
+    host code over the C ABI only, no HIP headers."""
+    os.makedirs(os.path.dirname(RUN_BIN), exist_ok=True)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+           "-I", os.path.join(REPO, "include"), os.path.join(PKG, "cpp", "ffb_run.cpp"),
+           "-o", RUN_BIN + ".tmp", "-L", PKG, "-lffb_hip", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, capture_output=not verbose)
+    os.replace(RUN_BIN + ".tmp", RUN_BIN)
+    return RUN_BIN
+
+
 def build(verbose: bool = False) -> str:
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-I", CSRC,
            "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
@@ -90,9 +107,11 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("hipcc failed building libffb_hip.so")
     os.replace(LIB + ".tmp", LIB)
     build_fortran_host(verbose)
+    build_cpp_driver(verbose)
     return LIB
 
 
 if __name__ == "__main__":
     print(build(verbose=True))
     print(HOST_BIN if os.path.exists(HOST_BIN) else "(no flang: Fortran host not built)")
+    print(RUN_BIN)
