@@ -11,7 +11,9 @@ from oracle.oracle import OracleLattice, Params
 
 pytestmark = pytest.mark.gpu
 
-FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 12), ("bf16dev", 0)]
+# the 1-3-12 form at s = 2: the random states below reach |f - w| ~ 0.3 (rest
+# direction), inside its range 2^(2-s) there
+FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 12), ("bf16dev", 0), ("e3m12dev", 2)]
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 
 

@@ -83,7 +83,7 @@ def test_gpu_matches_oracle_small_shapes(shape, mode, precision, vec, monkeypatc
 
 
 @pytest.mark.parametrize("precision,storage_scale", [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 16),
-                                                     ("bf16dev", 0)])
+                                                     ("bf16dev", 0), ("e3m12dev", 5)])
 @pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(1, 1), (2, 7), (65, 17), (300, 7), (1030, 5)], ids=lambda s: f"{s[0]}x{s[1]}")
 @pytest.mark.parametrize("vec", [1, 4], ids=["scalar", "vec4"])
@@ -116,7 +116,7 @@ def test_plane_layout_matches_oracle(shape, mode, precision, storage_scale, vec,
         np.testing.assert_array_equal(uy, o.u_y)
 
 
-@pytest.mark.parametrize("precision,storage_scale", [("fp32", 0), ("fp64", 0), ("fp16dev", 16)])
+@pytest.mark.parametrize("precision,storage_scale", [("fp32", 0), ("fp64", 0), ("fp16dev", 16), ("e3m12dev", 5)])
 @pytest.mark.parametrize("group", [1, 3, 16])
 @pytest.mark.parametrize("shape", [(65, 17), (300, 7), (257, 130), (1030, 5)], ids=lambda s: f"{s[0]}x{s[1]}")
 def test_xcd_tile_groups_match_oracle(shape, group, precision, storage_scale, monkeypatch):
