@@ -813,4 +813,37 @@ __global__ void __launch_bounds__(256) moments_partial_kernel(StepArgs<C> a, dou
     }
 }
 
+// Largest |f - w| of the owned cells per direction class (0 rest, 1 axis,
+// 2 diagonal), in FP64: one partial per block, reduced by extent_final_kernel.
+// The deviation forms give |D| 2^-s of the stored value as the step reads it.
+__host__ __device__ constexpr int dir_class(int d) { return d == 0 ? 0 : (d <= 4 ? 1 : 2); }
+template <typename S, typename C>
+__global__ void __launch_bounds__(256) extent_partial_kernel(StepArgs<C> a, double *partial) {
+    __shared__ double sh[3][256];
+    double mx[3] = {0.0, 0.0, 0.0};
+    const long long n = (long long)a.nx * a.ny;
+    const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(a.src);
+    for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+        const int y = (int)(k / a.nx), x = (int)(k - (long long)y * a.nx);
+        const size_t c = (size_t)y * (size_t)a.pitch + (size_t)x;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const double v = (double)ld_pop<S>(f, (size_t)d * (size_t)a.plane + c, d);
+            const double dev = Store<S>::dev ? fabs(v * (double)a.inv_scale) : fabs(v - (double)a.w[d]);
+            mx[dir_class(d)] = fmax(mx[dir_class(d)], dev);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = mx[j];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = fmax(sh[j][threadIdx.x], sh[j][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 }  // namespace ffb

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -256,6 +257,32 @@ int dispatch_init(ffb_lattice *L) {
     case FFB_PREC_E3M12_DEV: return launch_init<e3m12, float>(L);
     default: return launch_init<float, float>(L);
     }
+}
+
+__global__ void __launch_bounds__(256) extent_final_kernel(const double *partial, int n, double *out) {
+    __shared__ double sh[3][256];
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int k = (int)threadIdx.x; k < n; k += 256)
+        for (int j = 0; j < 3; ++j) m[j] = fmax(m[j], partial[3 * k + j]);
+    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = m[j];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = fmax(sh[j][threadIdx.x], sh[j][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <typename S, typename C> int launch_extent(ffb_lattice *L) {
+    StepArgs<C> a = make_args<C>(L, 0);
+    hipLaunchKernelGGL((extent_partial_kernel<S, C>), dim3((unsigned)L->mom_blocks), dim3(256), 0, L->stream, a,
+                       L->mom_partial);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(extent_final_kernel, dim3(1), dim3(256), 0, L->stream, L->mom_partial, L->mom_blocks,
+                       L->mom_out);
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
 }
 
 template <typename S, typename C> int launch_moments(ffb_lattice *L) {
@@ -683,6 +710,39 @@ int ffb_reset_timer(ffb_lattice_t *L) {
 int ffb_step_count(ffb_lattice_t *L, int64_t *steps) {
     if (!L || !steps) return fail(FFB_E_INVALID, "ffb_step_count: null argument");
     *steps = L->steps;
+    return FFB_OK;
+}
+
+int ffb_deviation_extent(ffb_lattice_t *L, double max_dev[3], double limit[3]) {
+    if (!L || !max_dev || !limit) return fail(FFB_E_INVALID, "ffb_deviation_extent: null argument");
+    HIPCHK(hipSetDevice(L->cfg.device));
+    int rc;
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: rc = launch_extent<double, double>(L); break;
+    case FFB_PREC_FP16_DEV: rc = launch_extent<__half, float>(L); break;
+    case FFB_PREC_BF16_DEV: rc = launch_extent<__hip_bfloat16, float>(L); break;
+    case FFB_PREC_FP32_DEV: rc = launch_extent<f32dev, float>(L); break;
+    case FFB_PREC_E3M12_DEV: rc = launch_extent<e3m12, float>(L); break;
+    default: rc = launch_extent<float, float>(L); break;
+    }
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(max_dev, L->mom_out, 3 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    // largest |f - w| the stored form represents, per class
+    const int s = L->cfg.storage_scale_log2;
+    for (int j = 0; j < 3; ++j) {
+        const int d = j == 0 ? 0 : (j == 1 ? 1 : 5);
+        double lim;
+        switch (L->cfg.precision) {
+        case FFB_PREC_FP64: lim = DBL_MAX; break;
+        case FFB_PREC_FP32: lim = FLT_MAX; break;
+        case FFB_PREC_FP32_DEV: lim = std::ldexp((double)FLT_MAX, -s); break;
+        case FFB_PREC_FP16_DEV: lim = std::ldexp(65504.0, -s); break;
+        case FFB_PREC_BF16_DEV: lim = std::ldexp(3.3895313892515355e38, -s); break;
+        default: lim = std::ldexp(2.0 - std::ldexp(1.0, -12), -s - store_off<e3m12>(d)); break;
+        }
+        limit[j] = lim;
+    }
     return FFB_OK;
 }
 

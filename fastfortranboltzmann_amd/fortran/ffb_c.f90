@@ -11,7 +11,7 @@ module ffb_c
 
     public :: ffb_config_t, ffb_config_default, ffb_create, ffb_destroy, ffb_init, ffb_step, &
         ffb_sync, ffb_read_macros, ffb_read_populations, ffb_elapsed_ms, ffb_reset_timer, &
-        ffb_moments, ffb_check, ffb_error_message
+        ffb_moments, ffb_deviation_extent, ffb_check, ffb_error_message
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
     public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV, &
         FFB_PREC_E3M12_DEV
@@ -108,6 +108,15 @@ module ffb_c
             type(c_ptr), value :: lattice
             real(c_double), intent(out) :: mass, mom_x, mom_y
         end function ffb_moments
+
+        ! largest |f - w| per direction class (rest, axis, diagonal) and the
+        ! storage form's limit for it
+        integer(c_int) function ffb_deviation_extent(lattice, max_dev, limit) &
+                bind(c, name="ffb_deviation_extent")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lattice
+            real(c_double), intent(out) :: max_dev(3), limit(3)
+        end function ffb_deviation_extent
 
         type(c_ptr) function c_ffb_last_error() bind(c, name="ffb_last_error")
             import :: c_ptr

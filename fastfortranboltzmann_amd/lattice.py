@@ -181,6 +181,16 @@ class Lattice:
         check(lib().ffb_moments(self.handle, ctypes.byref(m), ctypes.byref(mx), ctypes.byref(my)))
         return m.value, mx.value, my.value
 
+    def deviation_extent(self):
+        """(max_dev, limit): the largest |f - w| of the current state per
+        direction class (rest, axis, diagonal) and the largest the storage form
+        holds (ffb_deviation_extent); max_dev == limit shows clamping in the
+        1-3-12 form."""
+        dev = (ctypes.c_double * 3)()
+        lim = (ctypes.c_double * 3)()
+        check(lib().ffb_deviation_extent(self.handle, dev, lim))
+        return np.array(dev[:]), np.array(lim[:])
+
     def elapsed_ms(self) -> float:
         ms = ctypes.c_double()
         check(lib().ffb_elapsed_ms(self.handle, ctypes.byref(ms)))

@@ -169,6 +169,17 @@ int ffb_step_count(ffb_lattice_t *lat, int64_t *steps);
  * mom_x = sum rho u_x, mom_y = sum rho u_y (conservation drift). */
 int ffb_moments(ffb_lattice_t *lat, double *mass, double *mom_x, double *mom_y);
 
+/* Largest |f_i - w_i| of the current state over the owned cells, per
+ * direction class (max_dev[0] rest, [1] axis, [2] diagonal), in FP64, and
+ * the largest |f - w| the storage form holds for that class (limit[]): for
+ * the 1-3-12 form (2 - 2^-12) 2^(-s - off) (off = -1, 0, +1), values beyond it
+ * are clamped, so max_dev == limit shows clamping and max_dev near limit a
+ * scale s without headroom; FP16 65504 2^-s (overflow beyond); the FP32/BF16
+ * forms their type's largest finite value times 2^-s; plain FP32/FP64 the
+ * type's largest value.  Synchronises.  No reference counterpart (a check of
+ * the storage_scale_log2 choice). */
+int ffb_deviation_extent(ffb_lattice_t *lat, double max_dev[3], double limit[3]);
+
 /* Device pointers for zero-copy interop (torch, hipGraph capture):
  * *cur = populations the next step reads, *stream = the handle's hipStream_t;
  * population d of cell (x, y) is element d * plane + y * pitch + x of *cur

@@ -113,3 +113,55 @@ def test_parameters_changed_between_steps(mode, precision, scale):
         _lib.check(_lib.lib().ffb_set_parameters(lat.handle, 0.9, o.params.force, 0.09, 0.09))
         lat.step(6, write_macro_last=True)
         np.testing.assert_array_equal(lat.interior(), o.interior)
+
+
+@pytest.mark.parametrize("precision,scale", FORMS)
+def test_deviation_extent_matches_the_state(precision, scale):
+    """ffb_deviation_extent: the largest |f - w| per direction class (rest,
+    axis, diagonal) equals the one computed from the oracle's state after the
+    same steps (bitwise the GPU's), and the limit is the storage form's."""
+    nx, ny, steps = 41, 23, 30
+    print(f"[gpu] deviation extent {precision}", flush=True)
+    p = Params(omega=1.3, u_lid=0.08)
+    o = _oracle(4, nx, ny, p, precision, scale).init()
+    dt = np.float64 if precision == "fp64" else np.float32
+    f0 = _random_state(nx, ny, 11, dt)
+    o.set_populations(f0).run(steps)
+    with _lattice(4, nx, ny, p, precision, scale) as lat:
+        lat.init()
+        lat.set_populations(f0)
+        lat.step(steps)
+        dev, lim = lat.deviation_extent()
+    if o.storage is None:
+        w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, dtype=dt).astype(np.float64)
+        a = np.abs(o.interior.astype(np.float64) - w[:, None, None])
+    else:
+        a = np.abs(o.d[:, 1:-1, 1:-1].astype(np.float64)) * 2.0 ** -o.work_log2
+    want = np.array([a[0].max(), a[1:5].max(), a[5:9].max()])
+    np.testing.assert_array_equal(dev, want)
+    top = {"fp64": np.finfo(np.float64).max, "fp32": np.finfo(np.float32).max,
+           "fp32dev": np.finfo(np.float32).max * 2.0 ** -scale, "fp16dev": 65504.0 * 2.0 ** -scale,
+           "bf16dev": 3.3895313892515355e38 * 2.0 ** -scale}
+    if precision == "e3m12dev":
+        np.testing.assert_array_equal(lim, (2 - 2.0 ** -12) * 2.0 ** (-scale - np.array([-1, 0, 1])))
+    else:
+        np.testing.assert_array_equal(lim, [top[precision]] * 3)
+    assert np.all(dev < lim)
+
+
+def test_deviation_extent_shows_clamping_of_the_1_3_12_form():
+    """Sliding lid u = 0.1 at omega 1.0085: the rest deviation at the lid's
+    corners reaches 0.062.  With s = 5 it stays under the rest direction's
+    limit 2^(2-s) (no clamping); with s = 7 (limit 2^-5) it is clamped, and
+    the extent equals the limit."""
+    res = {}
+    for s in (5, 7):
+        print(f"[gpu] deviation extent clamp s={s}", flush=True)
+        p = Params(omega=1.008553, u_lid=0.1)
+        with _lattice(4, 64, 64, p, "e3m12dev", s) as lat:
+            lat.init().step(400)
+            res[s] = lat.deviation_extent()
+    dev, lim = res[5]
+    assert 0.055 < dev[0] < lim[0] and np.all(dev < lim)
+    dev, lim = res[7]
+    assert dev[0] == lim[0]
