@@ -20,6 +20,7 @@ EXPORTS = (
     "ffb_config_default", "ffb_create", "ffb_destroy", "ffb_init", "ffb_step", "ffb_sync",
     "ffb_read_macros", "ffb_read_populations", "ffb_write_populations", "ffb_read_pf_strips",
     "ffb_elapsed_ms", "ffb_reset_timer", "ffb_step_count", "ffb_moments", "ffb_deviation_extent",
+    "ffb_default_scale_log2",
     "ffb_device_info",
     "ffb_bench_copy", "ffb_last_error", "ffb_version",
     "ffb_batch_create", "ffb_batch_destroy", "ffb_batch_init", "ffb_batch_step", "ffb_batch_sync",
@@ -101,6 +102,7 @@ def lib() -> ctypes.CDLL:
         "ffb_step_count": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "ffb_moments": ([vp, dp, dp, dp], ctypes.c_int),
         "ffb_deviation_extent": ([vp, dp, dp], ctypes.c_int),
+        "ffb_default_scale_log2": ([i32, ctypes.c_double], ctypes.c_int32),
         "ffb_device_info": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                              ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
                             ctypes.c_int),

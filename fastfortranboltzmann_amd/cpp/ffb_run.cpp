@@ -31,7 +31,8 @@ void usage(FILE *out) {
                  "               [--device D] [--macros FILE]\n"
                  "Runs one lattice through the C ABI and prints one JSON line (kernel time, MLUPS,\n"
                  "FP64 sums of mass and momentum, largest speed).  --u is the lid, wall or wave\n"
-                 "speed of the mode; --re sets omega from nu = u * n_y / re (settings.f90:170-172).\n");
+                 "speed of the mode; --re sets omega from nu = u * n_y / re (settings.f90:170-172);\n"
+                 "without --scale the deviation forms take ffb_default_scale_log2 at that speed.\n");
 }
 
 [[noreturn]] void die(const char *what) {
@@ -77,7 +78,7 @@ int main(int argc, char **argv) {
     cfg.n_y = 256;
     std::string precision = "fp32", macros_path;
     double re = 0.0, u = -1.0;
-    bool omega_given = false;
+    bool omega_given = false, scale_given = false;
     long long steps = 1000, warmup = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -103,7 +104,7 @@ int main(int argc, char **argv) {
         else if (a == "--force") cfg.body_force = num("--force");
         else if (a == "--rho-in") cfg.rho_in = num("--rho-in");
         else if (a == "--rho-out") cfg.rho_out = num("--rho-out");
-        else if (a == "--scale") cfg.storage_scale_log2 = (int32_t)num("--scale");
+        else if (a == "--scale") cfg.storage_scale_log2 = (int32_t)num("--scale"), scale_given = true;
         else if (a == "--rounding") {
             if (v == "stochastic") cfg.storage_rounding = FFB_ROUND_STOCHASTIC;
             else if (v == "nearest") cfg.storage_rounding = FFB_ROUND_NEAREST;
@@ -120,12 +121,16 @@ int main(int argc, char **argv) {
         cfg.u_wall = u;
         cfg.u_max = u;
     }
+    const double speed = cfg.sim_mode == FFB_COUETTE_FLOW ? cfg.u_wall
+                         : cfg.sim_mode == FFB_SHEAR_WAVE ? cfg.u_max
+                                                          : cfg.u_lid;
     if (re > 0.0) {
         if (omega_given) bad_arg("give --omega or --re, not both");
-        const double speed = cfg.sim_mode == FFB_COUETTE_FLOW ? cfg.u_wall
-                             : cfg.sim_mode == FFB_SHEAR_WAVE ? cfg.u_max
-                                                              : cfg.u_lid;
         cfg.omega = 1.0 / (3.0 * speed * cfg.n_y / re + 0.5);
+    }
+    if (!scale_given) {  // the library's default for the form at the mode's speed
+        cfg.storage_scale_log2 = ffb_default_scale_log2(cfg.precision, speed < 0 ? -speed : speed);
+        if (cfg.storage_scale_log2 == INT32_MIN) bad_arg("no default --scale for this precision and speed");
     }
     if (steps < 1 || warmup < 0 || steps > INT32_MAX || warmup > INT32_MAX) bad_arg("bad step counts");
 
@@ -169,9 +174,9 @@ int main(int argc, char **argv) {
 
     const double mlups = (double)cells * (double)steps / (ms * 1e-3) / 1e6;
     std::printf("{\"driver\": \"ffb_run\", \"n_x\": %d, \"n_y\": %d, \"sim_mode\": %d, \"precision\": \"%s\", "
-                "\"omega\": %.17g, \"steps\": %lld, \"warmup\": %lld, \"kernel_ms\": %.6f, \"ms_per_step\": %.6f, "
+                "\"omega\": %.17g, \"storage_scale_log2\": %d, \"steps\": %lld, \"warmup\": %lld, \"kernel_ms\": %.6f, \"ms_per_step\": %.6f, "
                 "\"mlups\": %.1f, \"mass\": %.17g, \"mom_x\": %.17g, \"mom_y\": %.17g, \"u_max\": %.17g}\n",
-                cfg.n_x, cfg.n_y, cfg.sim_mode, precision.c_str(), cfg.omega, steps, warmup, ms, ms / (double)steps,
+                cfg.n_x, cfg.n_y, cfg.sim_mode, precision.c_str(), cfg.omega, cfg.storage_scale_log2, steps, warmup, ms, ms / (double)steps,
                 mlups, mass, mom_x, mom_y, std::sqrt(u_top));
     return 0;
 }

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -711,6 +712,27 @@ int ffb_step_count(ffb_lattice_t *L, int64_t *steps) {
     if (!L || !steps) return fail(FFB_E_INVALID, "ffb_step_count: null argument");
     *steps = L->steps;
     return FFB_OK;
+}
+
+int32_t ffb_default_scale_log2(int32_t precision, double speed) {
+    if (!(speed >= 0.0 && speed <= 1.0)) return INT32_MIN;
+    const double u = speed, rho_dev = 0.05;
+    switch (precision) {
+    case FFB_PREC_FP16_DEV:
+        return (int32_t)std::floor(std::log2(16384.0 / ((4.0 / 9.0) * (rho_dev + 3.0 * u + 6.0 * u * u))));
+    case FFB_PREC_E3M12_DEV: {
+        // per class w (rho_dev + 3 |c.u| + 4.5 (c.u)^2 + 1.5 u^2) with |c.u| <= |c| u, times 2^off
+        const double rest = (4.0 / 9.0) * (rho_dev + 1.5 * u * u) / 2.0;
+        const double axis = (1.0 / 9.0) * (rho_dev + 3.0 * u + 6.0 * u * u);
+        const double diag = (1.0 / 36.0) * (rho_dev + 3.0 * std::sqrt(2.0) * u + 10.5 * u * u) * 2.0;
+        return (int32_t)std::floor(std::log2(1.5 / std::max(rest, std::max(axis, diag))));
+    }
+    case FFB_PREC_FP32:
+    case FFB_PREC_FP64:
+    case FFB_PREC_BF16_DEV:
+    case FFB_PREC_FP32_DEV: return 0;
+    default: return INT32_MIN;
+    }
 }
 
 int ffb_deviation_extent(ffb_lattice_t *L, double max_dev[3], double limit[3]) {

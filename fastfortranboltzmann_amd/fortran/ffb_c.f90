@@ -11,7 +11,7 @@ module ffb_c
 
     public :: ffb_config_t, ffb_config_default, ffb_create, ffb_destroy, ffb_init, ffb_step, &
         ffb_sync, ffb_read_macros, ffb_read_populations, ffb_elapsed_ms, ffb_reset_timer, &
-        ffb_moments, ffb_deviation_extent, ffb_check, ffb_error_message
+        ffb_moments, ffb_deviation_extent, ffb_default_scale_log2, ffb_check, ffb_error_message
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
     public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV, &
         FFB_PREC_E3M12_DEV
@@ -117,6 +117,14 @@ module ffb_c
             type(c_ptr), value :: lattice
             real(c_double), intent(out) :: max_dev(3), limit(3)
         end function ffb_deviation_extent
+
+        ! default storage_scale_log2 of a form for flows of the given largest speed
+        integer(c_int32_t) function ffb_default_scale_log2(precision, speed) &
+                bind(c, name="ffb_default_scale_log2")
+            import :: c_int32_t, c_double
+            integer(c_int32_t), value :: precision
+            real(c_double), value :: speed
+        end function ffb_default_scale_log2
 
         type(c_ptr) function c_ffb_last_error() bind(c, name="ffb_last_error")
             import :: c_ptr

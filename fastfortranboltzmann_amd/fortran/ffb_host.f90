@@ -26,7 +26,7 @@ program ffb_host
     logical :: fp64
     integer(int32) :: step, next_stop, chunk
     integer(int64) :: clock_start, clock_end, clock_rate
-    real(real64) :: kernel_seconds, elapsed_seconds, other_seconds, ms, mlups, mass, mx, my
+    real(real64) :: kernel_seconds, elapsed_seconds, other_seconds, ms, mlups, mass, mx, my, speed
     character(len=:), allocatable :: out_dir
 
     call load_settings(s)
@@ -70,6 +70,22 @@ program ffb_host
     case default
         error stop "error: FFB_PRECISION must be fp32, fp64, fp16dev, bf16dev, fp32dev or e3m12dev"
     end select
+    if (s%storage_scale_log2 == STORAGE_SCALE_DEFAULT) then
+        ! the library's default for the form at the mode's speed
+        select case (s%sim_mode)
+        case (SIM_SHEAR_WAVE)
+            speed = s%u_max
+        case (SIM_COUETTE_FLOW)
+            speed = s%u_wall
+        case (SIM_SLIDING_LID)
+            speed = s%u_lid
+        case default
+            speed = max(s%u_lid, s%u_wall, s%u_max)
+        end select
+        cfg%storage_scale_log2 = ffb_default_scale_log2(cfg%precision, real(abs(speed), c_double))
+        if (cfg%storage_scale_log2 == -huge(1_c_int32_t) - 1) &
+            error stop "error: no default storage scale for this precision and speed"
+    end if
 
     if (fp64) then
         allocate(rho64(s%n_x, s%n_y), ux64(s%n_x, s%n_y), uy64(s%n_x, s%n_y))

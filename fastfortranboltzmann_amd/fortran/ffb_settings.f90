@@ -10,13 +10,17 @@
 ! an environment variable of the same name (e.g. FFB_N_X=4096) overrides the
 ! compiled value at run time.  FFB_PRECISION=fp32|fp64|fp16dev|bf16dev|fp32dev|e3m12dev and
 ! FFB_STORAGE_SCALE_LOG2 (and FFB_STORAGE_ROUNDING=stochastic|nearest for the
-! 16-bit forms) select the storage forms the reference lacks.
+! 16-bit forms) select the storage forms the reference lacks; without
+! FFB_STORAGE_SCALE_LOG2 the scale is the library's default for the form and
+! the mode's speed (ffb_default_scale_log2).
 module ffb_settings
     use iso_fortran_env, only: int32, int64, real64
     implicit none
     private
 
     public :: run_settings_t, load_settings, mode_name, reynolds_number
+    ! storage_scale_log2 not given: the library's default for the form and speed
+    integer(int32), parameter, public :: STORAGE_SCALE_DEFAULT = -huge(1_int32)
 
     integer(int32), parameter, public :: SIM_SHEAR_WAVE = 1, SIM_COUETTE_FLOW = 2, &
         SIM_POISEUILLE_FLOW = 3, SIM_SLIDING_LID = 4
@@ -85,7 +89,7 @@ contains
 #else
         s%precision = "fp32"
 #endif
-        s%storage_scale_log2 = 0
+        s%storage_scale_log2 = STORAGE_SCALE_DEFAULT
         s%storage_rounding = "stochastic"
 
         call env_int("FFB_N_X", s%n_x)

@@ -30,31 +30,23 @@ from .lattice import Lattice, Settings, omega_for_reynolds
 FP16_MAX = 65504.0
 
 
-def default_scale_log2(precision: str, u_ref: float = 0.1, rho_dev: float = 0.05) -> int:
-    """Scale 2^s for the deviation store: FP16 puts the largest expected
-    deviation near 2^14 (a factor 4 below the FP16 maximum), so the small
-    deviations of the resting bulk stay in the normal range; BF16 has FP32's
-    exponent range and needs no scale (s = 0).
+def default_scale_log2(precision: str, u_ref: float = 0.1) -> int:
+    """Scale 2^s for the deviation store for flows with largest speed u_ref
+    (ffb_default_scale_log2 of the library, host arithmetic only): FP16 puts
+    the largest expected deviation (4/9)(0.05 + 3u + 6u^2) near 2^14, a factor
+    4 below the FP16 maximum, so the small deviations of the resting bulk stay
+    in the normal range; the 1-3-12 form puts the largest per-direction-class
+    bound at 3/4 of its largest magnitude (s = 5 for u = 0.1); BF16 and FP32
+    deviations need no scale (s = 0)."""
+    from ._lib import lib
+    from .lattice import PRECISIONS
 
-    The bound is |f_i - w_i| <= w_0 (rho_dev + 3 u + 6 u^2) with w_0 = 4/9 the
-    largest weight, u the largest expected speed, rho_dev the largest density
-    deviation."""
-    if precision in ("bf16dev", "fp32dev"):
-        return 0
-    if precision == "e3m12dev":
-        # 1-3-12 layout: |f - w| < 2^(1-s-off) for a direction with exponent
-        # offset off (rest -1, axis 0, diagonal +1).  Largest expected
-        # deviation per class, w (rho_dev + 3 |c.u| + 4.5 (c.u)^2 + 1.5 u^2)
-        # with |c.u| <= |c| u, times 2^off, put at 3/4 of the largest
-        # magnitude (values beyond it are clamped, so the headroom matters).
-        d_max = max((4.0 / 9.0) * (rho_dev + 1.5 * u_ref * u_ref) / 2.0,
-                    (1.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref),
-                    (1.0 / 36.0) * (rho_dev + 3.0 * math.sqrt(2.0) * u_ref + 10.5 * u_ref * u_ref) * 2.0)
-        return int(math.floor(math.log2(1.5 / d_max)))
-    if precision != "fp16dev":
+    if precision not in PRECISIONS or not precision.endswith("dev"):
         raise ValueError(f"not a deviation form: {precision}")
-    d_max = (4.0 / 9.0) * (rho_dev + 3.0 * u_ref + 6.0 * u_ref * u_ref)
-    return int(math.floor(math.log2(2.0 ** 14 / d_max)))
+    s = int(lib().ffb_default_scale_log2(PRECISIONS[precision], float(u_ref)))
+    if s == -2 ** 31:
+        raise ValueError(f"no default scale for {precision} at speed {u_ref}")
+    return s
 
 
 def cavity_settings(n: int, re: float, u_lid: float, precision: str, scale_log2: int = 0) -> Settings:

@@ -180,6 +180,17 @@ int ffb_moments(ffb_lattice_t *lat, double *mass, double *mom_x, double *mom_y);
  * the storage_scale_log2 choice). */
 int ffb_deviation_extent(ffb_lattice_t *lat, double max_dev[3], double limit[3]);
 
+/* Default storage_scale_log2 of a storage form for flows with largest speed
+ * `speed` (the lid, wall or wave speed) and density deviations up to 0.05:
+ * FP16 puts the largest expected |f - w|, (4/9)(0.05 + 3u + 6u^2), near 2^14
+ * (a factor 4 below its maximum) so small deviations stay normal; the 1-3-12
+ * form puts the largest per-class bound w (0.05 + 3|c|u + ...) 2^off at 3/4 of
+ * its largest magnitude (s = 5 for u = 0.1); BF16, FP32 deviations and the
+ * plain forms 0.  Host arithmetic only (no GPU).  Returns the scale, or
+ * INT32_MIN for an unknown precision or a speed outside [0, 1].  No reference
+ * counterpart. */
+int32_t ffb_default_scale_log2(int32_t precision, double speed);
+
 /* Device pointers for zero-copy interop (torch, hipGraph capture):
  * *cur = populations the next step reads, *stream = the handle's hipStream_t;
  * population d of cell (x, y) is element d * plane + y * pitch + x of *cur

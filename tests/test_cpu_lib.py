@@ -70,3 +70,32 @@ def test_python_mirror_settings():
     s = Settings(N_Y=4096, SIM_MODE=4, U_LID=0.1, OMEGA=omega_for_reynolds(1000, 0.1, 4096))
     assert abs(s.OMEGA - 0.578436) < 1e-6
     assert abs(reynolds(s) - 1000) < 1e-9
+
+
+def test_default_scale_log2_contract():
+    """ffb_default_scale_log2 (host arithmetic, no GPU): for each speed, the
+    largest s that keeps the documented bound inside the form's headroom --
+    FP16: (4/9)(0.05 + 3u + 6u^2) 2^s <= 2^14; 1-3-12: every class bound
+    w (0.05 + 3|c|u + 4.5|c|^2 u^2 + 1.5 u^2) 2^off 2^s <= 3/4 of 2^1 --
+    so s + 1 breaks it; 0 for the forms that need no scale; INT32_MIN for an
+    unknown precision or a speed outside [0, 1]."""
+    import math
+
+    from fastfortranboltzmann_amd import _lib
+
+    f = _lib.lib().ffb_default_scale_log2
+    for u in (0.0, 0.01, 0.05, 0.1, 0.137, 0.2, 0.3, 0.5, 1.0):
+        b16 = (4 / 9) * (0.05 + 3 * u + 6 * u * u)
+        s = f(_lib.PREC_FP16_DEV, u)
+        assert b16 * 2.0 ** s <= 2.0 ** 14 < b16 * 2.0 ** (s + 1), (u, s)
+        bounds = [(4 / 9) * (0.05 + 1.5 * u * u) * 2.0 ** -1,                       # rest, off -1
+                  (1 / 9) * (0.05 + 3 * u + 6 * u * u),                             # axis, off 0
+                  (1 / 36) * (0.05 + 3 * math.sqrt(2) * u + 10.5 * u * u) * 2.0]    # diagonal, off +1
+        s = f(_lib.PREC_E3M12_DEV, u)
+        assert max(bounds) * 2.0 ** s <= 1.5 < max(bounds) * 2.0 ** (s + 1), (u, s)
+        for p in (_lib.PREC_FP32, _lib.PREC_FP64, _lib.PREC_BF16_DEV, _lib.PREC_FP32_DEV):
+            assert f(p, u) == 0
+    assert f(_lib.PREC_E3M12_DEV, 0.1) == 5 and f(_lib.PREC_FP16_DEV, 0.1) == 16
+    for p, u in ((_lib.PREC_E3M12_DEV, -0.1), (_lib.PREC_E3M12_DEV, 1.5), (_lib.PREC_FP16_DEV, float("nan")),
+                 (9, 0.1), (-1, 0.1)):
+        assert f(p, u) == -2 ** 31

@@ -83,3 +83,34 @@ def test_host_run_matches_golden_and_harness_contract(tmp_path, name):
     np.testing.assert_allclose(mag, np.sqrt(g["u_x"] ** 2 + g["u_y"] ** 2), rtol=1e-6 if dt == np.float32 else 1e-14)
     # initial state exported at step 0
     assert (d / "density_000000000.bin").exists()
+
+
+@needs_host
+@pytest.mark.gpu
+def test_host_deviation_form_takes_the_library_default_scale(tmp_path):
+    """FFB_PRECISION=e3m12dev without FFB_STORAGE_SCALE_LOG2: the host asks the
+    library for the default scale at the lid speed (s = 5 for u = 0.1), and its
+    exported fields are bitwise the Python front end's at that scale."""
+    from fastfortranboltzmann_amd import Lattice, Settings
+    from fastfortranboltzmann_amd.precision import default_scale_log2
+
+    nx, ny, steps = 40, 32, 60
+    print("[gpu] fortran host e3m12dev default scale", flush=True)
+    env = dict(os.environ, FFB_N_X=str(nx), FFB_N_Y=str(ny), FFB_N_STEPS=str(steps), FFB_SIM_MODE="4",
+               FFB_OMEGA="1.3_FP", FFB_U_LID="0.1_FP", FFB_PRECISION="e3m12dev", FFB_EXPORT_MACROS="true",
+               FFB_EXPORT_ENDPOINT_STATES="true", FFB_EXPORT_INTERVAL="100000", FFB_EXPORT_NUM="t",
+               I_X="1", I_Y="1", FOR_COARRAY_NUM_IMAGES="1")
+    env.pop("FFB_STORAGE_SCALE_LOG2", None)
+    r = subprocess.run([HOST], cwd=tmp_path, capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s = default_scale_log2("e3m12dev", 0.1)
+    assert s == 5
+    with Lattice(Settings(N_X=nx, N_Y=ny, SIM_MODE=4, OMEGA=1.3, U_LID=0.1, precision="e3m12dev",
+                          storage_scale_log2=s)) as lat:
+        lat.init().step(steps, write_macro_last=True)
+        rho, ux, uy = lat.macros()
+    d = tmp_path / "output" / "t"
+    step = f"{steps:09d}"
+    for field, want in (("density", rho), ("velocity_x", ux), ("velocity_y", uy)):
+        got = np.fromfile(d / f"{field}_{step}.bin", dtype=np.float32).reshape(ny, nx)
+        np.testing.assert_array_equal(got, want)
