@@ -52,7 +52,8 @@ extern "C" {
 
 /* storage precision of the populations.  FP64 / FP32 are the reference's
  * FFB_FP64 on / off builds (source/settings.f90:7-13).  The deviation forms
- * store D_i = (f_i - w_i) * 2^storage_scale_log2 as FP16, BF16 or FP32 and run
+ * store D_i = (f_i - w_i) * 2^storage_scale_log2 as FP16, BF16, FP32 or the
+ * 1-3-12 layout below and run
  * the step on D in FP32 (boundary rules and BGK collision in deviation space,
  * FP32 arithmetic on values of the size of the deviations); they have no
  * reference counterpart.  FFB_PREC_FP32_DEV has the FP32 form's traffic and
@@ -74,11 +75,11 @@ extern "C" {
 #define FFB_PREC_FP32_DEV 4
 #define FFB_PREC_E3M12_DEV 5
 
-/* rounding of the 16-bit deviation forms' stored values.  Nearest-even loses
- * every update smaller than half an FP16/BF16 step, so slow flows stagnate
- * (a 256^2 shear wave at omega 1 does not decay at all); stochastic rounding,
- * with random bits from a hash of (cell, step), keeps those updates on
- * average and stays reproducible. */
+/* rounding of the 16-bit deviation forms' stored values (FP16, BF16, 1-3-12).
+ * Nearest-even loses every update smaller than half a storage step, so slow
+ * flows stagnate (a 256^2 shear wave at omega 1 does not decay at all in
+ * FP16); stochastic rounding, with random bits from a hash of (cell, step),
+ * keeps those updates on average and stays reproducible. */
 #define FFB_ROUND_NEAREST 0
 #define FFB_ROUND_STOCHASTIC 1
 
