@@ -135,24 +135,33 @@ def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
 
 def batched(single_fp32_mlups: float, steps: int = 200) -> dict:
     """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
-    Couette / body-force channel), FP32, autotuned persistent launch."""
+    Couette / body-force channel), FP32, autotuned persistent launch; the same
+    batch in the 1-3-12 deviation form (each lattice at its default scale,
+    stochastic rounding) beside it."""
     from fastfortranboltzmann_amd.batch import Batch, mixed_batch
 
     lats = mixed_batch(0, 512)
-    with Batch(lats, "fp32") as b:
-        tun = b.autotune(20)
-        info = b.info()
-        b.init()
-        b.step(2)
-        b.sync()
-        b.reset_timer()
-        b.step(steps)
-        b.sync()
-        ms = b.elapsed_ms()
-    mlups = info["cells"] * steps / (ms * 1e-3) / 1e6
+    out = {}
+    for form in ("fp32", "e3m12dev"):
+        with Batch(lats, form) as b:
+            tun = b.autotune(20)
+            info = b.info()
+            b.init()
+            b.step(2)
+            b.sync()
+            b.reset_timer()
+            b.step(steps)
+            b.sync()
+            ms = b.elapsed_ms()
+        out[form] = (info["cells"] * steps / (ms * 1e-3) / 1e6, tun)
+    mlups, tun = out["fp32"]
     return {"lattices": len(lats), "cells": info["cells"], "steps": steps, "mlups": round(mlups, 1),
             "vs_single_4096_fp32": round(mlups / single_fp32_mlups, 3),
-            "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]}}
+            "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]},
+            "e3m12dev": {"mlups": round(out["e3m12dev"][0], 1),
+                         "vs_batch_fp32": round(out["e3m12dev"][0] / mlups, 3),
+                         "tuning": {"tile_w": out["e3m12dev"][1]["tile_w"],
+                                    "grid_blocks": out["e3m12dev"][1]["grid_blocks"]}}}
 
 
 def adjoint_cost(n: int = 512, steps: int = 2000) -> dict:

@@ -36,7 +36,7 @@ class FfbBatchLattice(ctypes.Structure):
 
     _fields_ = [
         ("n_x", ctypes.c_int32), ("n_y", ctypes.c_int32), ("kind", ctypes.c_int32),
-        ("reserved", ctypes.c_int32), ("omega", ctypes.c_double), ("rho_0", ctypes.c_double),
+        ("storage_scale_log2", ctypes.c_int32), ("omega", ctypes.c_double), ("rho_0", ctypes.c_double),
         ("u_wall", ctypes.c_double), ("body_force", ctypes.c_double),
         ("u_max", ctypes.c_double), ("n_sin", ctypes.c_double),
     ]

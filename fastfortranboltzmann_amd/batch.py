@@ -1,7 +1,8 @@
 """Batched form: hundreds of independent lattices of uneven size, each with
 its own omega and boundary kind, advanced by one persistent launch per step
-(hipGraph-replayed step pairs) in libffb_hip.so; see ffb_batch_* in
-include/ffb.h.  No reference counterpart.
+(hipGraph-replayed step pairs) in libffb_hip.so, in any storage form (the
+deviation forms with a scale per lattice); see ffb_batch_* in include/ffb.h.
+No reference counterpart.
 
 The mixed workload (``mixed_batch``): side lengths uniform on [64, 1024]
 (x and y drawn separately, so non-square), omega uniform on [0.6, 1.9], kinds
@@ -18,8 +19,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, PREC_FP32, PREC_FP32_DEV, PREC_FP64, SHEAR_WAVE, SLIDING_LID,
-                   FfbBatchLattice, FfbBatchTuning, check, lib)
+from ._lib import (BODY_FORCE_CHANNEL, COUETTE_FLOW, SHEAR_WAVE, SLIDING_LID, FfbBatchLattice, FfbBatchTuning,
+                   check, lib)
+from .lattice import PRECISIONS
 
 
 @dataclass
@@ -33,9 +35,30 @@ class BatchLattice:
     body_force: float = 0.0
     u_max: float = 0.0
     n_sin: float = 1.0
+    storage_scale_log2: int | None = None  # deviation forms; None: the default at speed()
 
-    def c(self) -> FfbBatchLattice:
-        return FfbBatchLattice(n_x=self.n_x, n_y=self.n_y, kind=self.kind, reserved=0,
+    def speed(self) -> float:
+        """Largest expected speed: the wave amplitude, the wall speed, or for a
+        body-force channel its steady centre-line speed F n_y^2 / (8 nu),
+        capped at 0.1."""
+        if self.kind == SHEAR_WAVE:
+            return abs(self.u_max)
+        if self.kind in (COUETTE_FLOW, SLIDING_LID):
+            return abs(self.u_wall)
+        nu = (1.0 / self.omega - 0.5) / 3.0
+        return min(0.1, abs(self.body_force) * self.n_y ** 2 / (8.0 * nu))
+
+    def scale_for(self, precision: str) -> int:
+        if self.storage_scale_log2 is not None:
+            return int(self.storage_scale_log2)
+        if not precision.endswith("dev"):
+            return 0
+        from .precision import default_scale_log2
+        return default_scale_log2(precision, self.speed())
+
+    def c(self, precision: str = "fp32") -> FfbBatchLattice:
+        return FfbBatchLattice(n_x=self.n_x, n_y=self.n_y, kind=self.kind,
+                               storage_scale_log2=self.scale_for(precision),
                                omega=self.omega, rho_0=self.rho_0, u_wall=self.u_wall,
                                body_force=self.body_force, u_max=self.u_max, n_sin=self.n_sin)
 
@@ -60,14 +83,15 @@ def mixed_batch(seed: int = 0, count: int = 512, side=(64, 1024), omega=(0.6, 1.
 
 class Batch:
     def __init__(self, lattices, precision: str = "fp32", device: int = 0):
-        codes = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp32dev": PREC_FP32_DEV}
-        if precision not in codes:
-            raise ValueError("the batched form stores FP32, FP64 or FP32 deviations (fp32dev)")
+        if precision not in PRECISIONS:
+            raise ValueError(f"unknown precision {precision!r}")
         self.lattices = list(lattices)
+        self.precision = precision
+        self.scales = [l.scale_for(precision) for l in self.lattices]
         self.dtype = np.dtype(np.float64 if precision == "fp64" else np.float32)
-        arr = (FfbBatchLattice * len(self.lattices))(*[l.c() for l in self.lattices])
+        arr = (FfbBatchLattice * len(self.lattices))(*[l.c(precision) for l in self.lattices])
         h = ctypes.c_void_p()
-        check(lib().ffb_batch_create(arr, len(self.lattices), codes[precision], device, ctypes.byref(h)))
+        check(lib().ffb_batch_create(arr, len(self.lattices), PRECISIONS[precision], device, ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -138,10 +162,11 @@ class Batch:
 
 def batch_study(lattices, steps: int, autotune: bool = True, trial_steps: int = 20,
                 compare_fp64: bool = True, log=print, forms=("fp32",)) -> dict:
-    """Aggregate MLUPS of the batch in each storage form of `forms` ("fp32",
-    "fp32dev"; events around `steps` steps) and the per-instance relative L2
-    error of u against an FP64 batch of the same lattices.  The first form's
-    figures are also the top-level keys."""
+    """Aggregate MLUPS of the batch in each storage form of `forms` (any of
+    "fp32", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"; the deviation forms at
+    each lattice's default scale; events around `steps` steps) and the
+    per-instance relative L2 error of u against an FP64 batch of the same
+    lattices.  The first form's figures are also the top-level keys."""
     res = {"forms": {}}
     fields = {}
     for form in forms:
@@ -199,7 +224,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--trial-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--forms", default="fp32,fp32dev", help="storage forms, comma-separated")
+    ap.add_argument("--forms", default="fp32,fp32dev,e3m12dev,fp16dev", help="storage forms, comma-separated")
     args = ap.parse_args(argv)
     lats = mixed_batch(args.seed, args.count)
     res = batch_study(lats, args.steps, trial_steps=args.trial_steps, compare_fp64=not args.no_fp64,

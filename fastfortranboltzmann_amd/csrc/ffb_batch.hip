@@ -46,6 +46,11 @@ template <typename C> struct BatchLat {
     int tile_start;       // first row segment
     int segs_per_row;     // ceil(nx / 64)
     C omega, corr8, corr9, force, rho_0;
+    C scale, inv_scale;   // 2^s and 2^-s of the deviation forms (s - 120 for the 1-3-12 form)
+    // deviation forms: the collide_dev constants of the lattice, computed once on
+    // the host (StepArgs' kw, kw_lo, kf, om1, half_force)
+    C kw[9], kw_lo[9], kf[9];
+    C om1, half_force;
 };
 
 struct Seg {
@@ -61,6 +66,8 @@ template <typename C> struct BatchArgs {
     int write_macro;
     C *rho_o, *ux_o, *uy_o;
     const C *prof;        // shear-wave u_x(y) rows of all SW lattices
+    const unsigned *step_ctr;  // steps done before this launch (16-bit forms: stochastic rounding)
+    int stochastic;       // 16-bit forms: 1 = stochastic rounding of the stored values
     C w[9];
 };
 
@@ -80,7 +87,7 @@ template <typename T> __device__ __forceinline__ T load_uniform(const T *p) {
 template <typename S, typename C>
 __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const BatchLat<C> &L) {
     StepArgs<C> a;
-    using R = C;  // the batch stores FP32 / FP64 populations or FP32 deviations (scale 1)
+    using R = typename Store<S>::R;
     a.src = static_cast<const R *>(b.src_pool) + L.off;
     a.dst = static_cast<R *>(b.dst_pool) + L.off;
     a.nx = L.nx;
@@ -100,27 +107,34 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     a.write_macro = b.write_macro;
 #pragma unroll
     for (int d = 0; d < 9; ++d) a.w[d] = b.w[d];
-    a.scale = 1;
-    a.inv_scale = 1;
+    a.scale = L.scale;
+    a.inv_scale = L.inv_scale;
     a.n_tiles_x = 0;
     a.n_blocks = 0;
     a.xcd_group = 0;
     a.force = L.force;
-    if constexpr (Store<S>::dev) {  // deviation-space constants of collide_dev (2^s = 1)
-        a.om1 = (C)1 - a.omega;
-        const C pre = ((C)1 - (C)0.5 * a.omega) * a.force;
+    a.stochastic = 0;
+    a.step_seed = 0;
+    if constexpr (Store<S>::dev) {  // deviation-space constants of collide_dev, from the host
+        a.om1 = L.om1;
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
-            const double wd = d == 0 ? 4.0 / 9.0 : (d <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
-            const double kd = (double)a.omega * wd;
-            a.kw[d] = (C)kd;
-            a.kw_lo[d] = (C)(kd - (double)a.kw[d]);
-            a.kf[d] = a.w[d] * pre;
+            a.kw[d] = L.kw[d];
+            a.kw_lo[d] = L.kw_lo[d];
+            a.kf[d] = L.kf[d];
         }
-        a.half_force = (C)0.5 * a.force;
+        a.half_force = L.half_force;
+        if constexpr (sizeof(R) == 2) {
+            a.stochastic = b.stochastic;
+            a.step_seed = load_uniform(b.step_ctr) + 1u;  // the step this launch computes
+        }
     }
     return a;
 }
+
+// Step counter of the stochastic rounding: advanced after every step launch
+// (inside the captured step pairs too), read by the next launch.
+__global__ void batch_count_kernel(unsigned *ctr, int reset) { ctr[0] = reset ? 0u : ctr[0] + 1u; }
 
 // One step of every lattice: a persistent grid walks the row segments, one
 // wave per segment.
@@ -135,8 +149,9 @@ __global__ void __launch_bounds__(TW) batch_step_kernel(BatchArgs<C> b) {
         const int x = (t - L.tile_start - y * L.segs_per_row) * 64 + lane;
         if (x >= L.nx) continue;
         const StepArgs<C> a = lattice_args<S, C>(b, L);
-        const C *f = static_cast<const C *>(a.src);
-        C *g = static_cast<C *>(a.dst);
+        using R = typename Store<S>::R;
+        const R *f = static_cast<const R *>(a.src);
+        R *g = static_cast<R *>(a.dst);
         switch (L.kind) {
         case SW: step_cell<S, C, SW>(a, f, g, x, y); break;
         case CF: step_cell<S, C, CF>(a, f, g, x, y); break;
@@ -159,7 +174,7 @@ __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
         const int x = (t - L.tile_start - y * L.segs_per_row) * 64 + lane;
         if (x >= L.nx) continue;
         const int q = y * L.nx + x;
-        C *g = static_cast<C *>(b.dst_pool) + L.off;
+        typename Store<S>::R *g = static_cast<typename Store<S>::R *>(b.dst_pool) + L.off;
         const size_t c = (size_t)y * (size_t)L.pitch + (size_t)x;
         const bool sw = L.kind == SW;
         const C ux = sw ? b.prof[L.prof_off + y] : (C)0;
@@ -174,7 +189,7 @@ __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
             } else {
                 feq = b.w[d] * L.rho_0;
             }
-            g[(size_t)d * (size_t)L.plane + c] = Store<S>::from_f(feq, b.w[d], (C)1);
+            g[(size_t)d * (size_t)L.plane + c] = pop_out<S>(Store<S>::from_f(feq, b.w[d], L.scale), d);
         }
         const size_t m = (size_t)L.macro_off + (size_t)q;
         b.rho_o[m] = L.rho_0;
@@ -188,7 +203,10 @@ __global__ void __launch_bounds__(TW) batch_init_kernel(BatchArgs<C> b) {
 struct ffb_batch {
     int precision = FFB_PREC_FP32;
     int device = 0;
-    size_t elem = 4;
+    size_t elem = 4;          // bytes per stored population
+    size_t celem = 4;         // bytes per macro / profile value (the compute type)
+    int rounding = FFB_ROUND_STOCHASTIC;
+    void *step_ctr = nullptr; // steps done (one unsigned), for the stochastic rounding
     std::vector<ffb_batch_lattice_t> desc;
     std::vector<long long> off, plane, macro_off, prof_off;
     std::vector<int> pitch;
@@ -246,10 +264,30 @@ template <typename C> int upload_tables(ffb_batch *B) {
             for (int k = 0; k < L.segs_per_row; ++k) tiles.push_back(Seg{i, y});
         L.omega = (C)d.omega;
         const C w6 = (C)1 / (C)36;
-        L.corr8 = (C)6 * w6 * (C)d.rho_0 * (C)d.u_wall;  // as couette_flow.f90:118-119
-        L.corr9 = L.corr8;
         L.force = (C)d.body_force;
         L.rho_0 = (C)d.rho_0;
+        // deviation forms: the lattice's 2^s (the 1-3-12 form works on D * 2^-120)
+        const bool dev = B->precision != FFB_PREC_FP32 && B->precision != FFB_PREC_FP64;
+        const int sw = dev ? d.storage_scale_log2 - (B->precision == FFB_PREC_E3M12_DEV ? 120 : 0) : 0;
+        L.scale = (C)std::ldexp(1.0, sw);
+        L.inv_scale = (C)std::ldexp(1.0, -sw);
+        // as couette_flow.f90:118-119, ((6*w_6)*rho_0)*u, with u*2^s in the deviation forms
+        L.corr8 = (C)6 * w6 * (C)d.rho_0 * (dev ? (C)d.u_wall * L.scale : (C)d.u_wall);
+        L.corr9 = L.corr8;
+        // collide_dev constants, as the single-domain form computes them:
+        // omega w_d 2^s with the exact weights as a float pair, the Guo source
+        // factors (w_d ((1 - omega/2) F)) 2^s
+        L.om1 = (C)1 - L.omega;
+        const C pre = ((C)1 - (C)0.5 * L.omega) * L.force;
+        for (int dd = 0; dd < 9; ++dd) {
+            const double wd = dd == 0 ? 4.0 / 9.0 : (dd <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
+            const double kd = (double)L.omega * wd * (double)L.scale;
+            L.kw[dd] = (C)kd;
+            L.kw_lo[dd] = (C)(kd - (double)L.kw[dd]);
+            const C w = dd == 0 ? (C)4 / (C)9 : (dd <= 4 ? (C)1 / (C)9 : (C)1 / (C)36);
+            L.kf[dd] = (w * pre) * L.scale;
+        }
+        L.half_force = (C)0.5 * L.force;
     }
     if (B->lats) (void)hipFree(B->lats);
     if (B->segs) (void)hipFree(B->segs);
@@ -275,6 +313,8 @@ template <typename C> BatchArgs<C> batch_args(const ffb_batch *B, int parity, in
     b.ux_o = static_cast<C *>(B->macro[1]);
     b.uy_o = static_cast<C *>(B->macro[2]);
     b.prof = static_cast<const C *>(B->prof);
+    b.step_ctr = static_cast<const unsigned *>(B->step_ctr);
+    b.stochastic = B->rounding == FFB_ROUND_STOCHASTIC;
     b.w[0] = (C)4 / (C)9;
     for (int d = 1; d <= 4; ++d) b.w[d] = (C)1 / (C)9;
     for (int d = 5; d <= 8; ++d) b.w[d] = (C)1 / (C)36;
@@ -293,10 +333,19 @@ template <typename S, typename C> void launch_step_c(const ffb_batch *B, int par
     }
 }
 
+bool is_16bit(int p) { return p == FFB_PREC_FP16_DEV || p == FFB_PREC_BF16_DEV || p == FFB_PREC_E3M12_DEV; }
+
 void launch_step(const ffb_batch *B, int parity, int write_macro) {
-    if (B->precision == FFB_PREC_FP64) launch_step_c<double, double>(B, parity, write_macro);
-    else if (B->precision == FFB_PREC_FP32_DEV) launch_step_c<f32dev, float>(B, parity, write_macro);
-    else launch_step_c<float, float>(B, parity, write_macro);
+    switch (B->precision) {
+    case FFB_PREC_FP64: launch_step_c<double, double>(B, parity, write_macro); break;
+    case FFB_PREC_FP32_DEV: launch_step_c<f32dev, float>(B, parity, write_macro); break;
+    case FFB_PREC_FP16_DEV: launch_step_c<__half, float>(B, parity, write_macro); break;
+    case FFB_PREC_BF16_DEV: launch_step_c<__hip_bfloat16, float>(B, parity, write_macro); break;
+    case FFB_PREC_E3M12_DEV: launch_step_c<e3m12, float>(B, parity, write_macro); break;
+    default: launch_step_c<float, float>(B, parity, write_macro); break;
+    }
+    if (is_16bit(B->precision))  // the next launch computes the next step
+        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 0);
 }
 
 int drop_graph(ffb_batch *B) {
@@ -336,6 +385,14 @@ template <typename S, typename C> int init_c(ffb_batch *B) {
     return FFB_OK;
 }
 
+template <typename S, typename C> int init_any(ffb_batch *B) {
+    if (is_16bit(B->precision)) {
+        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 1);
+        BHIPCHK(hipGetLastError());
+    }
+    return init_c<S, C>(B);
+}
+
 int apply_tuning(ffb_batch *B, const ffb_batch_tuning_t &t) {
     if (t.tile_w != 64 && t.tile_w != 128 && t.tile_w != 256 && t.tile_w != 512 && t.tile_w != 1024)
         return fail(FFB_E_INVALID, "ffb_batch: tile_w must be 64, 128, 256, 512 or 1024");
@@ -366,11 +423,14 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
                      ffb_batch_t **out) {
     if (!lattices || !out || count < 1) return fail(FFB_E_INVALID, "ffb_batch_create: bad argument");
     *out = nullptr;
-    if (precision != FFB_PREC_FP32 && precision != FFB_PREC_FP64 && precision != FFB_PREC_FP32_DEV)
-        return fail(FFB_E_INVALID,
-                    "ffb_batch_create: precision must be FFB_PREC_FP32, FFB_PREC_FP64 or FFB_PREC_FP32_DEV");
+    if (precision < FFB_PREC_FP32 || precision > FFB_PREC_E3M12_DEV)
+        return fail(FFB_E_INVALID, "ffb_batch_create: unknown precision " + std::to_string(precision));
     for (int i = 0; i < count; ++i) {
         const ffb_batch_lattice_t &d = lattices[i];
+        if (precision != FFB_PREC_FP32 && precision != FFB_PREC_FP64 &&
+            (d.storage_scale_log2 < -30 || d.storage_scale_log2 > 30))
+            return fail(FFB_E_INVALID, "ffb_batch_create: lattice " + std::to_string(i) +
+                                           " storage_scale_log2 outside [-30, 30]");
         if (d.n_x < 1 || d.n_y < 1 || (long long)d.n_x * d.n_y > (1LL << 30))
             return fail(FFB_E_INVALID, "ffb_batch_create: lattice " + std::to_string(i) + " has a bad size");
         if (d.kind != FFB_SHEAR_WAVE && d.kind != FFB_COUETTE_FLOW && d.kind != FFB_SLIDING_LID &&
@@ -386,7 +446,8 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     ffb_batch *B = new ffb_batch();
     B->precision = precision;
     B->device = device;
-    B->elem = precision == FFB_PREC_FP64 ? 8 : 4;
+    B->elem = precision == FFB_PREC_FP64 ? 8 : (is_16bit(precision) ? 2 : 4);
+    B->celem = precision == FFB_PREC_FP64 ? 8 : 4;
     B->desc.assign(lattices, lattices + count);
     long long off = 0, moff = 0, poff = 0;
     // each lattice in the row-blocked layout of the single domain: per row y
@@ -417,8 +478,9 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     for (int k = 0; k < 2; ++k)
         if ((e = hipMalloc(&B->pool[k], (size_t)B->pool_elems * B->elem)) != hipSuccess) return bail(e, "pools");
     for (int k = 0; k < 3; ++k)
-        if ((e = hipMalloc(&B->macro[k], (size_t)B->macro_elems * B->elem)) != hipSuccess) return bail(e, "macros");
-    if ((e = hipMalloc(&B->prof, (size_t)B->prof_elems * B->elem)) != hipSuccess) return bail(e, "profiles");
+        if ((e = hipMalloc(&B->macro[k], (size_t)B->macro_elems * B->celem)) != hipSuccess) return bail(e, "macros");
+    if ((e = hipMalloc(&B->prof, (size_t)B->prof_elems * B->celem)) != hipSuccess) return bail(e, "profiles");
+    if ((e = hipMalloc(&B->step_ctr, 4)) != hipSuccess) return bail(e, "step counter");
     if ((e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "stream");
     if ((e = hipEventCreate(&B->ev_start)) != hipSuccess) return bail(e, "event");
     if ((e = hipEventCreate(&B->ev_stop)) != hipSuccess) return bail(e, "event");
@@ -427,7 +489,7 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     // host with the C library's sin in the storage precision (as the single-domain
     // initial condition, initialization.f90:42)
     {
-        std::vector<unsigned char> prof((size_t)B->prof_elems * B->elem, 0);
+        std::vector<unsigned char> prof((size_t)B->prof_elems * B->celem, 0);
         for (int i = 0; i < count; ++i) {
             const ffb_batch_lattice_t &d = lattices[i];
             if (d.kind != FFB_SHEAR_WAVE) continue;
@@ -464,6 +526,7 @@ int ffb_batch_destroy(ffb_batch_t *B) {
     for (void *p : B->pool) if (p) (void)hipFree(p);
     for (void *p : B->macro) if (p) (void)hipFree(p);
     if (B->prof) (void)hipFree(B->prof);
+    if (B->step_ctr) (void)hipFree(B->step_ctr);
     if (B->lats) (void)hipFree(B->lats);
     if (B->segs) (void)hipFree(B->segs);
     if (B->ev_start) (void)hipEventDestroy(B->ev_start);
@@ -479,8 +542,14 @@ int ffb_batch_init(ffb_batch_t *B) {
     B->cur = 0;
     B->steps = 0;
     B->timing = false;
-    if (B->precision == FFB_PREC_FP64) return init_c<double, double>(B);
-    return B->precision == FFB_PREC_FP32_DEV ? init_c<f32dev, float>(B) : init_c<float, float>(B);
+    switch (B->precision) {
+    case FFB_PREC_FP64: return init_any<double, double>(B);
+    case FFB_PREC_FP32_DEV: return init_any<f32dev, float>(B);
+    case FFB_PREC_FP16_DEV: return init_any<__half, float>(B);
+    case FFB_PREC_BF16_DEV: return init_any<__hip_bfloat16, float>(B);
+    case FFB_PREC_E3M12_DEV: return init_any<e3m12, float>(B);
+    default: return init_any<float, float>(B);
+    }
 }
 
 int ffb_batch_step(ffb_batch_t *B, int32_t n_steps, int32_t write_macro_last) {
@@ -611,17 +680,48 @@ int ffb_batch_read_populations(ffb_batch_t *B, int32_t index, void *f_halo) {
                            stage.size(), hipMemcpyDeviceToHost, B->stream));
     BHIPCHK(hipStreamSynchronize(B->stream));
     const size_t hp = (size_t)d.n_x + 2, hplane = hp * ((size_t)d.n_y + 2);
-    std::memset(f_halo, 0, 9 * hplane * B->elem);
-    for (int dd = 0; dd < 9; ++dd)
-        for (int y = 0; y < d.n_y; ++y)
-            std::memcpy((unsigned char *)f_halo + ((size_t)dd * hplane + (size_t)(y + 1) * hp + 1) * B->elem,
-                        stage.data() + ((size_t)dd * plane + (size_t)y * pitch) * B->elem, (size_t)d.n_x * B->elem);
-    if (B->precision == FFB_PREC_FP32_DEV) {  // stored deviations: f = w + D
-        float *f = static_cast<float *>(f_halo);
-        for (int dd = 0; dd < 9; ++dd) {
-            const float w = dd == 0 ? 4.0f / 9.0f : (dd <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f);
-            for (int y = 1; y <= d.n_y; ++y)
-                for (size_t x = 1; x <= (size_t)d.n_x; ++x) f[(size_t)dd * hplane + (size_t)y * hp + x] += w;
+    std::memset(f_halo, 0, 9 * hplane * B->celem);
+    if (B->precision == FFB_PREC_FP32 || B->precision == FFB_PREC_FP64) {
+        for (int dd = 0; dd < 9; ++dd)
+            for (int y = 0; y < d.n_y; ++y)
+                std::memcpy((unsigned char *)f_halo + ((size_t)dd * hplane + (size_t)(y + 1) * hp + 1) * B->elem,
+                            stage.data() + ((size_t)dd * plane + (size_t)y * pitch) * B->elem,
+                            (size_t)d.n_x * B->elem);
+        return FFB_OK;
+    }
+    // deviation forms: f = w + D * 2^-s, D as the step reads it (the 1-3-12
+    // form: the stored value times 2^-off of its direction, on the 2^-120 scale)
+    const int s = d.storage_scale_log2 - (B->precision == FFB_PREC_E3M12_DEV ? 120 : 0);
+    const float inv = (float)std::ldexp(1.0, -s);
+    float *f = static_cast<float *>(f_halo);
+    for (int dd = 0; dd < 9; ++dd) {
+        const float w = dd == 0 ? 4.0f / 9.0f : (dd <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f);
+        for (int y = 0; y < d.n_y; ++y) {
+            const size_t src = (size_t)dd * plane + (size_t)y * pitch;
+            float *row = f + (size_t)dd * hplane + (size_t)(y + 1) * hp + 1;
+            for (size_t x = 0; x < (size_t)d.n_x; ++x) {
+                float v;
+                if (B->precision == FFB_PREC_FP32_DEV) {
+                    v = ((const float *)stage.data())[src + x];
+                } else {
+                    const uint16_t h = ((const uint16_t *)stage.data())[src + x];
+                    uint32_t u;
+                    if (B->precision == FFB_PREC_FP16_DEV) {
+                        __half hv;
+                        std::memcpy(&hv, &h, 2);
+                        v = __half2float(hv);
+                        std::memcpy(&u, &v, 4);
+                    } else if (B->precision == FFB_PREC_BF16_DEV) {
+                        u = (uint32_t)h << 16;
+                    } else {
+                        u = ((uint32_t)(int32_t)(int16_t)h << 11) & 0x83ffffffu;
+                    }
+                    std::memcpy(&v, &u, 4);
+                    if (B->precision == FFB_PREC_E3M12_DEV && store_off<e3m12>(dd) != 0)
+                        v = v * pow2f(-store_off<e3m12>(dd));
+                }
+                row[x] = w + v * inv;
+            }
         }
     }
     return FFB_OK;

@@ -219,7 +219,7 @@ typedef struct ffb_batch_lattice {
     int32_t kind;         /* FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_SLIDING_LID or
                              FFB_BODY_FORCE_CHANNEL (Poiseuille flow needs the
                              single-domain API) */
-    int32_t reserved;
+    int32_t storage_scale_log2; /* s of the deviation forms (D = (f - w) 2^s); ignored otherwise */
     double omega;
     double rho_0;
     double u_wall;        /* Couette top wall / sliding lid speed */
@@ -234,8 +234,11 @@ typedef struct ffb_batch_tuning {
 
 typedef struct ffb_batch ffb_batch_t;
 
-/* precision: FFB_PREC_FP32, FFB_PREC_FP64 or FFB_PREC_FP32_DEV (FP32
- * deviations with scale 1; populations read back as f = w + D). */
+/* precision: any FFB_PREC_* form.  The deviation forms use each lattice's
+ * storage_scale_log2 (ffb_default_scale_log2 at the lattice's speed is a good
+ * choice; 0 for FP32 deviations) and round the 16-bit stores stochastically,
+ * with random bits from (cell of the lattice, step), as the single-domain form;
+ * populations read back as f = w + D 2^-s in FP32. */
 int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t precision, int32_t device,
                      ffb_batch_t **out);
 int ffb_batch_destroy(ffb_batch_t *batch);

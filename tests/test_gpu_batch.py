@@ -1,7 +1,9 @@
 """Batched form on the GPU: every lattice of a mixed batch, advanced by the
 persistent hipGraph-replayed launch, is bitwise equal to the oracle run of
-that lattice alone (FP32 and FP64), across graph replays, odd/even step
-counts, re-tuning and the autotuner's re-initialisation."""
+that lattice alone (FP32, FP64 and the deviation forms at each lattice's own
+scale, the 16-bit ones with stochastic rounding, whose step index the graph
+replays advance on the device), across graph replays, odd/even step counts,
+re-tuning and the autotuner's re-initialisation."""
 import numpy as np
 import pytest
 
@@ -27,22 +29,22 @@ def small_batch():
     ]
 
 
-def oracle_of(L, dtype, steps):
+def oracle_of(L, dtype, steps, scale=0):
     p = Params(omega=L.omega, rho_0=L.rho_0, u_max=L.u_max, n_sin=L.n_sin, u_wall=L.u_wall,
                u_lid=L.u_wall, force=L.body_force)
-    if dtype == "fp32dev":  # FP32 deviations, scale 1
-        return OracleLattice(L.kind, L.n_x, L.n_y, p, storage="fp32dev").init().run(steps)
+    if isinstance(dtype, str):  # a deviation form at the lattice's scale
+        return OracleLattice(L.kind, L.n_x, L.n_y, p, storage=dtype, scale_log2=scale).init().run(steps)
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"])
 @pytest.mark.parametrize("tile_w,grid", [(256, 0), (128, 8), (512, 3), (1024, 0)])
 def test_batch_matches_oracle_per_lattice(precision, tile_w, grid):
     from fastfortranboltzmann_amd.batch import Batch
 
     print(f"[gpu] batch {precision} tile_w={tile_w} grid={grid}", flush=True)
     lats = small_batch()
-    dt = {"fp64": np.float64, "fp32": np.float32, "fp32dev": "fp32dev"}[precision]
+    dt = {"fp64": np.float64, "fp32": np.float32}.get(precision, precision)
     with Batch(lats, precision) as b:
         b.set_tuning(tile_w, grid)
         b.init()
@@ -51,7 +53,7 @@ def test_batch_matches_oracle_per_lattice(precision, tile_w, grid):
             b.step(chunk, write_macro_last=True)
             done += chunk
             for i, L in enumerate(lats):
-                o = oracle_of(L, dt, done)
+                o = oracle_of(L, dt, done, b.scales[i])
                 np.testing.assert_array_equal(b.populations(i)[:, 1:-1, 1:-1], o.interior,
                                               err_msg=f"lattice {i} after {done} steps")
                 rho, ux, uy = b.macros(i)
@@ -59,18 +61,23 @@ def test_batch_matches_oracle_per_lattice(precision, tile_w, grid):
                 np.testing.assert_array_equal(rho, o.rho)
 
 
-def test_batch_autotune_reinitialises_and_keeps_results():
+@pytest.mark.parametrize("precision", ["fp32", "e3m12dev"])
+def test_batch_autotune_reinitialises_and_keeps_results(precision):
+    """The autotuner's trials advance the state (and, in the 16-bit forms, the
+    step index of the stochastic rounding); its re-initialisation restarts
+    both, so the run afterwards equals the oracle's from step 0."""
     from fastfortranboltzmann_amd.batch import Batch
 
-    print("[gpu] batch autotune", flush=True)
+    print(f"[gpu] batch autotune {precision}", flush=True)
     lats = small_batch()
-    with Batch(lats, "fp32") as b:
+    with Batch(lats, precision) as b:
         best = b.autotune(4)
         assert best["tile_w"] in (64, 128, 256, 512, 1024) and best["ms_per_step"] > 0
         b.step(9, write_macro_last=True)
+        dt = np.float32 if precision == "fp32" else precision
         for i, L in enumerate(lats):
             np.testing.assert_array_equal(b.populations(i)[:, 1:-1, 1:-1],
-                                          oracle_of(L, np.float32, 9).interior)
+                                          oracle_of(L, dt, 9, b.scales[i]).interior)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
