@@ -378,6 +378,40 @@ __device__ __forceinline__ C pull_edge(const StepArgs<C> &a, const typename Stor
     return LD<S, C>(a, f, d, xs, ys);
 }
 
+// The nine stored elements a domain-edge cell (x, y) pulls in the modes
+// without pressure increments (SW, CF, SL, BF): the halo entries of
+// pull_edge, with the source element of every direction chosen first
+// (integer selects) and the nine loads then issued unconditionally, so that
+// an edge cell waits for memory once, not once per direction.  A bounce-back
+// source is the plane of opp(d), stored with the offset of d (same direction
+// class).  The moving-wall terms of CF and SL are added after decoding
+// (pull_pops).
+template <typename S, typename C, int MODE, typename L>
+__device__ __forceinline__ void pull_edge_cell(const StepArgs<C> &a, const typename Store<S>::R *f, int x, int y,
+                                               L (&raw)[9]) {
+    static_assert(MODE != PF, "PF edge cells add the strip increments (pull_edge)");
+    static_assert(store_off<S>(opp_of(1)) == store_off<S>(1) && store_off<S>(opp_of(5)) == store_off<S>(5),
+                  "opposite directions share a storage offset");
+    const int nx = a.nx, ny = a.ny;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        int xs = x - cx_of(d), ys = y - cy_of(d), ds = d;
+        const bool xo = (xs < 0) | (xs >= nx);
+        const bool yo = (ys < 0) | (ys >= ny);
+        if (MODE == SW) {
+            xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+            ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+        } else if ((MODE == SL) ? (xo | yo) : yo) {
+            ds = opp_of(d);
+            xs = x;
+            ys = y;
+        } else if (xo) {
+            xs = xs < 0 ? xs + nx : xs - nx;
+        }
+        raw[d] = f[(size_t)ds * (size_t)a.plane + (size_t)ys * (size_t)a.pitch + (size_t)xs];
+    }
+}
+
 // BGK collision of fuzed_pull_streaming_collision_local_unrolled_universal
 // (source/layouts/xyd/simulation.f90:224-293), same operation order.  f[]
 // holds the pulled populations on entry and the post-collision ones on exit.
@@ -581,29 +615,66 @@ __device__ __forceinline__ void buf_st_nt(R v, __amdgpu_buffer_rsrc_t r, unsigne
     }
 }
 
-// One cell, any position: plain pull in the interior, boundary rule on the edge.
+// Element xb/sizeof(R) of a row: raw buffer load for 4- and 8-byte elements,
+// global load for 2-byte ones (faster there, measured).
+template <typename R>
+__device__ __forceinline__ R ld_row(const R *row, unsigned xb) {
+    if constexpr (sizeof(R) >= 4) return buf_ld<R>(row_rsrc(row), xb);
+    else return *at(row, xb);
+}
+
+// The nine populations cell (x, y) pulls, as working values: plain pull in
+// the interior, boundary rule on the edge.
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void pull_pops(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f, int x,
+                                          int y, C (&v)[9]) {
+    using R = typename Store<S>::R;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    const R *fr = f + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
+    const unsigned xb = (unsigned)x * (unsigned)sizeof(R);
+    const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
+    if constexpr (MODE == PF) {
+        if (!edge) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) v[d] = pop_in<S>(ld_row(fr + d * P + pull_dy(d) * W + pull_dx(d), xb), d);
+        } else {
+            v[0] = LD<S, C>(a, f, 0, x, y);
+#pragma unroll
+            for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
+        }
+    } else {
+        // Both branches only load (2-byte elements held zero-extended, one per
+        // register) and decoding follows the join: a wave holding interior and
+        // edge cells does not wait for memory inside either branch.
+        using L = std::conditional_t<sizeof(R) == 2, unsigned, R>;
+        L raw[9];
+        if (!edge) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) raw[d] = ld_row(fr + d * P + pull_dy(d) * W + pull_dx(d), xb);
+        } else {
+            pull_edge_cell<S, C, MODE>(a, f, x, y, raw);
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) v[d] = pop_in<S>((R)raw[d], d);
+        if (MODE == CF || MODE == SL) {  // moving top wall: bounce-back of d = 7, 8 from row ny (pull_edge)
+            if (y == a.ny - 1) {
+                v[7] = v[7] - a.corr8;
+                v[8] = v[8] + a.corr9;
+            }
+        }
+    }
+}
+
+// One cell, any position: pull, collide, store.
 template <typename S, typename C, int MODE>
 __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
                                           typename Store<S>::R *__restrict__ g, int x, int y) {
     using R = typename Store<S>::R;
     const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
-    const R *fr = f + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
-    R *gr = g + (ptrdiff_t)y * W;
+    R *gr = g + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
     const unsigned xb = (unsigned)x * (unsigned)sizeof(R);
-    const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
     C v[9];
-    if (!edge) {
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            const R *row = fr + d * P + pull_dy(d) * W + pull_dx(d);
-            if constexpr (sizeof(R) >= 4) v[d] = pop_in<S>(buf_ld<R>(row_rsrc(row), xb), d);
-            else v[d] = pop_in<S>(*at(row, xb), d);
-        }
-    } else {
-        v[0] = LD<S, C>(a, f, 0, x, y);
-#pragma unroll
-        for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
-    }
+    pull_pops<S, C, MODE>(a, f, x, y, v);
     collide_mode<S, C, MODE>(a, v, x, y);
     if constexpr (Store<S>::dev && sizeof(R) == 2) {
         if (a.stochastic) {

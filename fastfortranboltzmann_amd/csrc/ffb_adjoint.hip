@@ -78,18 +78,7 @@ __device__ __forceinline__ void reader_of(int x, int y, int j, int nx, int ny, i
 // (FP32 / FP64 storage) or scaled deviations D = (p - w) 2^s (deviation form).
 template <typename S, typename C, int MODE>
 __device__ __forceinline__ void pull_cell(const StepArgs<C> &s, int x, int y, C (&p)[9]) {
-    const typename Store<S>::R *f = static_cast<const typename Store<S>::R *>(s.src);
-    const bool edge = (x == 0) | (x == s.nx - 1) | (y == 0) | (y == s.ny - 1);
-    if (!edge) {
-        const size_t c = (size_t)y * (size_t)s.pitch + (size_t)x;
-#pragma unroll
-        for (int d = 0; d < 9; ++d)
-            p[d] = ld_pop<S>(f, (size_t)d * (size_t)s.plane + c + (ptrdiff_t)pull_dy(d) * (ptrdiff_t)s.pitch + pull_dx(d), d);
-    } else {
-        p[0] = LD<S, C>(s, f, 0, x, y);
-#pragma unroll
-        for (int d = 1; d < 9; ++d) p[d] = pull_edge<S, C, MODE, 2>(s, f, d, x, y);
-    }
+    pull_pops<S, C, MODE>(s, static_cast<const typename Store<S>::R *>(s.src), x, y, p);
 }
 
 template <typename C> struct Moments {

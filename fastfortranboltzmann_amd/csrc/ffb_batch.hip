@@ -630,7 +630,15 @@ int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *
     int cus = 0;
     BHIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device));
     const int tws[] = {64, 128, 256, 512, 1024};
-    const int per_cu[] = {0, 2, 4, 8, 16};  // 0: one tile per workgroup (no persistence)
+    // workgroups per CU of the persistent grid (0: one tile per workgroup, no
+    // persistence).  7, 14 and 28 hold 7 waves per SIMD at 256, 128 and 64
+    // threads, 6 and 12 six (a memory-bound step can run faster below full
+    // occupancy); grids of several resident rounds (up to 384 workgroups per
+    // CU) shorten each wave's persistent loop while still sparing most
+    // workgroup launches: measured on 512 lattices, the 16-bit forms ran
+    // fastest with 96 workgroups of 256 threads per CU, FP32 without
+    // persistence.
+    const int per_cu[] = {0, 2, 4, 6, 7, 8, 12, 14, 16, 28, 48, 96, 192, 384};
     ffb_batch_tuning_t best{256, 0};
     double best_t = 1e30;
     int rc = ffb_batch_init(B);
