@@ -606,12 +606,12 @@ __device__ __forceinline__ R buf_ld(__amdgpu_buffer_rsrc_t r, unsigned xb) {
 }
 // Nontemporal store (cache policy bit 1, "nt").
 template <typename R>
-__device__ __forceinline__ void buf_st_nt(R v, __amdgpu_buffer_rsrc_t r, unsigned xb) {
-    if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, xb, 0, 2);
-    else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, xb, 0, 2);
+__device__ __forceinline__ void buf_st_nt(R v, __amdgpu_buffer_rsrc_t r, unsigned xb, unsigned so = 0) {
+    if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, xb, so, 2);
+    else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, xb, so, 2);
     else {
         typedef unsigned u2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, 0, 2);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, so, 2);
     }
 }
 
@@ -674,8 +674,19 @@ __device__ __forceinline__ void pull_pops(const StepArgs<C> &a, const typename S
     }
 }
 
+// Stored value of direction d of a cell: nontemporal buffer store at byte xb
+// of the direction row d of row gr, through that row's own descriptor, or,
+// with ONE_DESC, through one descriptor at direction 0 and a scalar offset
+// (the same descriptor for the nine rows: fewer scalar registers; needs the
+// nine rows within 4 GiB, which the caller guarantees).
+template <typename R, bool ONE_DESC>
+__device__ __forceinline__ void store_pop(R v, R *gr, ptrdiff_t P, unsigned xb, int d) {
+    if constexpr (ONE_DESC) buf_st_nt<R>(v, row_rsrc(gr), xb, (unsigned)d * (unsigned)P * (unsigned)sizeof(R));
+    else buf_st_nt<R>(v, row_rsrc(gr + d * P), xb);
+}
+
 // One cell, any position: pull, collide, store.
-template <typename S, typename C, int MODE>
+template <typename S, typename C, int MODE, bool ONE_DESC = false>
 __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
                                           typename Store<S>::R *__restrict__ g, int x, int y) {
     using R = typename Store<S>::R;
@@ -689,12 +700,12 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
         if (a.stochastic) {
             const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
 #pragma unroll
-            for (int d = 0; d < 9; ++d) buf_st_nt<R>(pop_out_sr<S>(v[d], pop_bits(b, d), d), row_rsrc(gr + d * P), xb);
+            for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC>(pop_out_sr<S>(v[d], pop_bits(b, d), d), gr, P, xb, d);
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) buf_st_nt<R>(pop_out<S>(v[d], d), row_rsrc(gr + d * P), xb);
+    for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC>(pop_out<S>(v[d], d), gr, P, xb, d);
 }
 
 // One fused pull-stream-collide step over one domain.  Each thread updates V

@@ -215,7 +215,7 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
  * size and the persistent grid by timing the candidates on the batch.
  * ------------------------------------------------------------------------ */
 typedef struct ffb_batch_lattice {
-    int32_t n_x, n_y;
+    int32_t n_x, n_y;     /* n_x <= 2^24, n_x * n_y <= 2^30 */
     int32_t kind;         /* FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_SLIDING_LID or
                              FFB_BODY_FORCE_CHANNEL (Poiseuille flow needs the
                              single-domain API) */

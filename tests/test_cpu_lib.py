@@ -56,6 +56,31 @@ def test_invalid_config_is_rejected_with_message():
     assert h.value is None
 
 
+def test_invalid_batch_is_rejected_before_any_gpu_call():
+    """ffb_batch_create checks every lattice before it looks for a device:
+    sizes (n_x <= 2^24, n_x * n_y <= 2^30), kind, precision, scale."""
+    from fastfortranboltzmann_amd import _lib
+
+    L = _lib.lib()
+
+    def create(prec=_lib.PREC_FP16_DEV, **kw):
+        d = _lib.FfbBatchLattice(n_x=64, n_y=64, kind=_lib.SHEAR_WAVE, storage_scale_log2=8, omega=1.0,
+                                 rho_0=1.0, u_wall=0.0, body_force=0.0, u_max=0.01, n_sin=1.0)
+        for k, v in kw.items():
+            setattr(d, k, v)
+        h = ctypes.c_void_p()
+        rc = L.ffb_batch_create(ctypes.byref(d), 1, prec, 0, ctypes.byref(h))
+        return rc, L.ffb_last_error(), h
+
+    for kw, msg in (({"n_x": (1 << 24) + 1, "n_y": 1}, b"bad size"), ({"n_x": 0}, b"bad size"),
+                    ({"n_x": 1 << 16, "n_y": (1 << 14) + 1}, b"bad size"), ({"kind": 3}, b"kind"),
+                    ({"storage_scale_log2": 31}, b"storage_scale_log2")):
+        rc, err, h = create(**kw)
+        assert rc == 1 and msg in err and h.value is None, (kw, err)
+    rc, err, _ = create(prec=9)
+    assert rc == 1 and b"precision" in err
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU may be present")
 def test_create_without_gpu_fails_loudly():
     from fastfortranboltzmann_amd import FfbError, Lattice
