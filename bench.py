@@ -3,8 +3,11 @@
 Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
-FP32 populations.  One "step" is one fused pull-stream-collide pass over the
-whole domain (one kernel launch); inputs are resident in HBM before timing.
+FP32 populations.  One "step" is one fused pull-stream-collide update of the
+whole domain; the library runs the 4- and 8-byte forms as step pairs, two
+updates per kernel launch (temporal blocking, bitwise the same result), and
+the line also reports one launch per step ("single_pass").  Inputs are
+resident in HBM before timing.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
 
@@ -39,12 +42,47 @@ def cavity_omega(re: float = 1000.0, u: float = 0.1, n: int = 4096) -> float:
     return 1.0 / (3.0 * u * n / re + 0.5)
 
 
-def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0) -> dict:
+class pass_mode:
+    """Lattices created inside run step pairs in one launch (pairs=True), one
+    launch per step (False), or the library's default (None)."""
+
+    def __init__(self, pairs: bool | None):
+        self.pairs = pairs
+
+    def __enter__(self):
+        self.saved = os.environ.get("FFB_TWO_STEP")
+        if self.pairs is not None:
+            os.environ["FFB_TWO_STEP"] = "1" if self.pairs else "0"
+        return self
+
+    def __exit__(self, *exc):
+        if self.saved is None:
+            os.environ.pop("FFB_TWO_STEP", None)
+        else:
+            os.environ["FFB_TWO_STEP"] = self.saved
+
+
+def uses_pairs(precision: str) -> bool:
+    """Whether the library runs step pairs for this form (FFB_TWO_STEP or its default)."""
+    env = os.environ.get("FFB_TWO_STEP")
+    if env is not None:
+        try:
+            return int(env) != 0
+        except ValueError:
+            return False
+    return BYTES_PER_POP[precision] >= 4
+
+
+def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0,
+            pairs: bool | None = None) -> dict:
     from fastfortranboltzmann_amd import Lattice, Settings
 
     s = Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=cavity_omega(n=n), U_LID=0.1, RHO_0=1.0,
                  precision=precision, storage_scale_log2=scale_log2)
-    with Lattice(s) as lat:
+    with pass_mode(pairs):
+        paired = uses_pairs(precision)
+        lat = Lattice(s)
+    with lat:
         lat.init()
         lat.step(warmup)
         lat.sync()
@@ -56,14 +94,18 @@ def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0
         ev_ms = lat.elapsed_ms()
         mass = lat.moments()[0]
     cells = n * n
-    kernel_ms = ev_ms / steps  # one launch per step for this workload
-    alg_bytes = 18 * BYTES_PER_POP[precision] * cells
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    # launches: step pairs (and a single step for an odd count), or one per step
+    launches = steps // 2 + steps % 2 if paired else steps
+    per_launch = 2 if paired and steps % 2 == 0 else 1
+    kernel_ms = ev_ms / launches  # the kernel's average launch duration
+    alg_bytes = 18 * BYTES_PER_POP[precision] * cells  # per cell update (step)
+    achieved = alg_bytes * steps / (ev_ms * 1e-3) / 1e9
     return {
         "mlups": cells * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms": kernel_ms,
-        "alg_bytes_per_launch": alg_bytes,
+        "steps_per_launch": per_launch,
+        "alg_bytes_per_launch": alg_bytes * per_launch,
         "achieved_gbs": achieved,
         "mass_per_cell": mass / cells,
     }
@@ -86,6 +128,8 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
     for form in forms:
         s = default_scale_log2(form) if form.endswith("dev") else 0
         rates[form] = run_form(cavity_settings(n, 5000.0, 0.1, form, s), steps, chunk=steps)["mlups"]
+    with pass_mode(False):  # FP32 with one launch per step, as the 16-bit forms run
+        fp32_single = run_form(cavity_settings(n, 5000.0, 0.1, "fp32", 0), steps, chunk=steps)["mlups"]
     ref = run_form(cavity_settings(err_n, 1000.0, 0.1, "fp64"), err_steps, chunk=err_steps)
     for form in forms:
         s = default_scale_log2(form) if form.endswith("dev") else 0
@@ -93,6 +137,8 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
         c = compare(ref, r)
         sw = shear_wave_viscosity(form)
         out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
+                     "speedup_vs_fp32_single_pass": round(rates[form] / fp32_single, 3),
+                     "step_pairs": uses_pairs(form),
                      "achieved_gbs": round(rates[form] * 1e6 * 18 * BYTES_PER_POP[form] / 1e9, 1),
                      "scale_log2": s, "rel_l2_u_vs_fp64": c["rel_l2_u"], "max_abs_rho_vs_fp64": c["max_abs_rho"],
                      "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"], "mass_drift": r["mass_drift"],
@@ -100,6 +146,7 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
                      "rounding": "stochastic" if form in ("fp16dev", "bf16dev", "e3m12dev") else "nearest"}
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
+    out["fp32_single_pass_mlups"] = round(fp32_single, 1)
     return out
 
 
@@ -133,7 +180,7 @@ def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
             "amplitude_ratio": amps[-1] / 0.05}
 
 
-def batched(single_fp32_mlups: float, steps: int = 200) -> dict:
+def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int = 200) -> dict:
     """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
     Couette / body-force channel), FP32, autotuned persistent launch; the same
     batch in the 1-3-12 deviation form (each lattice at its default scale,
@@ -157,6 +204,7 @@ def batched(single_fp32_mlups: float, steps: int = 200) -> dict:
     mlups, tun = out["fp32"]
     return {"lattices": len(lats), "cells": info["cells"], "steps": steps, "mlups": round(mlups, 1),
             "vs_single_4096_fp32": round(mlups / single_fp32_mlups, 3),
+            "vs_single_pass_4096_fp32": round(mlups / single_pass_fp32_mlups, 3),
             "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]},
             "e3m12dev": {"mlups": round(out["e3m12dev"][0], 1),
                          "vs_batch_fp32": round(out["e3m12dev"][0] / mlups, 3),
@@ -281,18 +329,29 @@ def main() -> int:
     print(f"bench: cavity {n}^2 {args.precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     main_run = run_gpu(n, args.precision, args.steps, args.warmup)
-    extra = {}
+    print("bench: one launch per step", file=sys.stderr, flush=True)
+    single = run_gpu(n, args.precision, args.steps, args.warmup, pairs=False)
+    extra = {"steps_per_launch": main_run["steps_per_launch"],
+             "single_pass": {"mlups": round(single["mlups"], 1), "kernel_ms": round(single["kernel_ms"], 5),
+                             "achieved_gbs": round(single["achieved_gbs"], 1),
+                             "frac": round(single["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                             "traffic": traffic_from_profiles(f"sl{n}_{args.precision}_single_pass")}}
     if not args.no_fp64 and args.precision != "fp64":
         print("bench: fp64 run", file=sys.stderr, flush=True)
-        r64 = run_gpu(n, "fp64", max(1, args.steps // 2), args.warmup)
+        r64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup)
+        s64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup, pairs=False)
         extra["fp64"] = {"mlups": round(r64["mlups"], 1), "achieved_gbs": round(r64["achieved_gbs"], 1),
                          "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                         "kernel_ms": round(r64["kernel_ms"], 5)}
+                         "kernel_ms": round(r64["kernel_ms"], 5), "steps_per_launch": r64["steps_per_launch"],
+                         "single_pass": {"mlups": round(s64["mlups"], 1),
+                                         "achieved_gbs": round(s64["achieved_gbs"], 1),
+                                         "frac": round(s64["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                         "kernel_ms": round(s64["kernel_ms"], 5)}}
     if not args.no_extras and args.precision == "fp32":
         print("bench: reduced storage", file=sys.stderr, flush=True)
         extra["reduced_storage"] = reduced_storage(2 * n if n <= 4096 else n, max(50, args.steps // 10))
         print("bench: batched", file=sys.stderr, flush=True)
-        extra["batched"] = batched(main_run["mlups"])
+        extra["batched"] = batched(main_run["mlups"], single["mlups"])
         print("bench: adjoint", file=sys.stderr, flush=True)
         extra["adjoint"] = adjoint_cost()
         print("bench: shear wave 256^2", file=sys.stderr, flush=True)

@@ -685,16 +685,32 @@ __device__ __forceinline__ void store_pop(R v, R *gr, ptrdiff_t P, unsigned xb, 
     else buf_st_nt<R>(v, row_rsrc(gr + d * P), xb);
 }
 
-// One cell, any position: pull, collide, store.
+// Stored form of the post-collision values v of cell (x, y): the 16-bit
+// forms round stochastically with the bits of (cell, a.step_seed) when asked.
+template <typename S, typename C>
+__device__ __forceinline__ void encode_cell(const StepArgs<C> &a, const C (&v)[9], int x, int y,
+                                            typename Store<S>::R (&o)[9]) {
+    if constexpr (Store<S>::dev && sizeof(typename Store<S>::R) == 2) {
+        if (a.stochastic) {
+            const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
+#pragma unroll
+            for (int d = 0; d < 9; ++d) o[d] = pop_out_sr<S>(v[d], pop_bits(b, d), d);
+            return;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 9; ++d) o[d] = pop_out<S>(v[d], d);
+}
+
+// Collision of the pulled values v of cell (x, y) and the stores of its new
+// populations into g.
 template <typename S, typename C, int MODE, bool ONE_DESC = false>
-__device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
-                                          typename Store<S>::R *__restrict__ g, int x, int y) {
+__device__ __forceinline__ void collide_store(const StepArgs<C> &a, C (&v)[9], typename Store<S>::R *__restrict__ g,
+                                              int x, int y) {
     using R = typename Store<S>::R;
     const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
     R *gr = g + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
     const unsigned xb = (unsigned)x * (unsigned)sizeof(R);
-    C v[9];
-    pull_pops<S, C, MODE>(a, f, x, y, v);
     collide_mode<S, C, MODE>(a, v, x, y);
     if constexpr (Store<S>::dev && sizeof(R) == 2) {
         if (a.stochastic) {
@@ -706,6 +722,15 @@ __device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename S
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC>(pop_out<S>(v[d], d), gr, P, xb, d);
+}
+
+// One cell, any position: pull, collide, store.
+template <typename S, typename C, int MODE, bool ONE_DESC = false>
+__device__ __forceinline__ void step_cell(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
+                                          typename Store<S>::R *__restrict__ g, int x, int y) {
+    C v[9];
+    pull_pops<S, C, MODE>(a, f, x, y, v);
+    collide_store<S, C, MODE, ONE_DESC>(a, v, g, x, y);
 }
 
 // One fused pull-stream-collide step over one domain.  Each thread updates V
@@ -791,6 +816,76 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) __builtin_nontemporal_store(out[d], reinterpret_cast<RV *>(at(gr + d * P, xb)));
+}
+
+// Two steps in one pass (temporal blocking), the modes without strip
+// passes (SW, CF, SL, BF).  A workgroup owns an output tile of (64 - 2) x
+// (ROWS - 2) cells and computes step 1 on the tile plus a one-cell ring
+// (64 x ROWS cells, one per thread; on a periodic axis the ring cell is the
+// wrapped cell, whose step-1 values the reference's halo copy would hold),
+// keeps the ring's results in LDS in the stored representation (so the 16-bit
+// forms round exactly as two single steps do), and computes step 2 of the
+// tile from LDS: pulled values from the neighbouring LDS cells, and, where
+// the source lies beyond a wall, the cell's own opposite population plus the
+// moving-wall term, as pull_edge.  Every cell update is the single step's
+// arithmetic, so the result is bitwise that of two step_kernel launches.
+// Step 1 runs with a.step_seed and no macro output, step 2 with
+// a.step_seed + 1 and a.write_macro.  HBM traffic per two steps: ~(1.2 + 1)
+// x 9 x s bytes per cell against 36 x s.
+template <typename R> __host__ __device__ constexpr int two_step_rows() { return sizeof(R) == 8 ? 8 : 16; }
+
+template <typename S, typename C, int MODE>
+__global__ void __launch_bounds__(64 * two_step_rows<typename Store<S>::R>()) step2_kernel(StepArgs<C> a) {
+    static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
+    using R = typename Store<S>::R;
+    constexpr int ROWS = two_step_rows<R>();
+    __shared__ R lds[9][ROWS][64];
+    constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
+    const int lane = (int)threadIdx.x & 63, r = (int)threadIdx.x >> 6;
+    const int nx = a.nx, ny = a.ny;
+    const int ux = (int)blockIdx.x * 62 - 1 + lane, uy = (int)blockIdx.y * (ROWS - 2) - 1 + r;  // unwrapped
+    const R *__restrict__ f = static_cast<const R *>(a.src);
+    R *__restrict__ g = static_cast<R *>(a.dst);
+    // step 1 on the region cell (needed where -1 <= ux <= nx, -1 <= uy <= ny)
+    int gx = ux, gy = uy;
+    bool in1 = (ux >= -1) & (ux <= nx) & (uy >= -1) & (uy <= ny);
+    if (PX) gx = ux < 0 ? ux + nx : (ux >= nx ? ux - nx : ux);
+    else in1 &= (ux >= 0) & (ux < nx);
+    if (PY) gy = uy < 0 ? uy + ny : (uy >= ny ? uy - ny : uy);
+    else in1 &= (uy >= 0) & (uy < ny);
+    if (in1) {
+        StepArgs<C> a1 = a;
+        a1.write_macro = 0;
+        C v[9];
+        pull_pops<S, C, MODE>(a1, f, gx, gy, v);
+        collide_mode<S, C, MODE>(a1, v, gx, gy);
+        R o[9];
+        encode_cell<S, C>(a1, v, gx, gy, o);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[d];
+    }
+    __syncthreads();
+    // step 2 on the tile
+    if ((lane >= 1) & (lane <= 62) & (r >= 1) & (r <= ROWS - 2) & (ux < nx) & (uy < ny)) {
+        C v[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const int sx = ux - cx_of(d), sy = uy - cy_of(d);
+            const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
+            const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
+            const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
+            v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
+        }
+        if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
+            if (uy == ny - 1) {
+                v[7] = v[7] - a.corr8;
+                v[8] = v[8] + a.corr9;
+            }
+        }
+        StepArgs<C> a2 = a;
+        a2.step_seed = a.step_seed + 1u;
+        collide_store<S, C, MODE>(a2, v, g, ux, uy);
+    }
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,

@@ -203,6 +203,34 @@ int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) 
     }
 }
 
+// Two steps in one launch (step2_kernel): tiles of 62 x (rows - 2) cells.
+template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_macro) {
+    const StepArgs<C> a = make_args<C>(L, write_macro);
+    constexpr int ROWS = two_step_rows<typename Store<S>::R>();
+    const dim3 grid((unsigned)((a.nx + 61) / 62), (unsigned)((a.ny + ROWS - 3) / (ROWS - 2)));
+    const dim3 block(64 * ROWS);
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW>), grid, block, 0, L->stream, a); break;
+    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF>), grid, block, 0, L->stream, a); break;
+    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((step2_kernel<S, C, BF>), grid, block, 0, L->stream, a); break;
+    case FFB_SLIDING_LID: hipLaunchKernelGGL((step2_kernel<S, C, SL>), grid, block, 0, L->stream, a); break;
+    default: return fail(FFB_E_INVALID, "two-step pass: Poiseuille flow is not supported");
+    }
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+int dispatch_step2(ffb_lattice *L, int write_macro) {
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: return launch_step2<double, double>(L, write_macro);
+    case FFB_PREC_FP16_DEV: return launch_step2<__half, float>(L, write_macro);
+    case FFB_PREC_BF16_DEV: return launch_step2<__hip_bfloat16, float>(L, write_macro);
+    case FFB_PREC_FP32_DEV: return launch_step2<f32dev, float>(L, write_macro);
+    case FFB_PREC_E3M12_DEV: return launch_step2<e3m12, float>(L, write_macro);
+    default: return launch_step2<float, float>(L, write_macro);
+    }
+}
+
 int dispatch_step(ffb_lattice *L, int write_macro) {
     switch (L->cfg.precision) {
     case FFB_PREC_FP64: return launch_step<double, double>(L, write_macro);
@@ -479,6 +507,11 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         const int g = rows ? (half ? 2 * tiles_x : (cfg->precision == FFB_PREC_FP64 ? std::max(1, tiles_x / 2) : 0)) : 0;
         L->xcd_group = (int)std::max(0LL, env_ll("FFB_XCD_GROUP", g));
     }
+    // step pairs in one pass (step2_kernel) for the 4- and 8-byte forms, where
+    // they were measured faster (FP32 4096^2 cavity 192.5 -> 146.9 us per step,
+    // FP64 387.5 -> 239.2); the 16-bit forms, bound by their arithmetic, ran
+    // 14 % slower that way and keep one pass per step.  FFB_TWO_STEP=0/1 overrides.
+    L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);
@@ -551,13 +584,24 @@ int ffb_step(ffb_lattice_t *L, int32_t n_steps, int32_t write_macro_last) {
         HIPCHK(hipEventRecord(L->ev_start, L->stream));
         L->timing = true;
     }
-    for (int32_t k = 0; k < n_steps; ++k) {
+    const bool pairs = L->two_step && L->cfg.sim_mode != FFB_POISEUILLE_FLOW;
+    for (int32_t k = 0; k < n_steps;) {
+        if (pairs && n_steps - k >= 2) {  // steps k+1, k+2 in one pass; the result lands in the other set
+            const int wm = (write_macro_last && k + 2 == n_steps) ? 1 : 0;
+            int rc = dispatch_step2(L, wm);
+            if (rc) return rc;
+            L->cur = 1 - L->cur;
+            L->steps += 2;
+            k += 2;
+            continue;
+        }
         const int wm = (write_macro_last && k == n_steps - 1) ? 1 : 0;
         int rc = dispatch_step(L, wm);
         if (rc) return rc;
         L->cur = 1 - L->cur;
         if (L->cfg.sim_mode == FFB_POISEUILLE_FLOW) L->strip_old = 1 - L->strip_old;
         ++L->steps;
+        ++k;
     }
     HIPCHK(hipEventRecord(L->ev_stop, L->stream));
     return FFB_OK;

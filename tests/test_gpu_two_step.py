@@ -1,0 +1,69 @@
+"""Step pairs in one pass (temporal blocking, step2_kernel) against one pass
+per step (step_kernel): bitwise equal populations and macro fields in every
+mode without a strip pass and every storage form, from random states that
+make every boundary rule carry a distinct value, on shapes smaller and larger
+than one 62 x 14 (FP64: 62 x 6) tile, with odd step counts (the last step of
+an odd count runs alone) and macro output on the last step.  The pair path
+is the default for the 4- and 8-byte forms, so the oracle parity tests cover
+it too; FFB_TWO_STEP selects the path when a lattice is created."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = {1: "shear wave", 2: "Couette", 4: "sliding lid", 5: "body-force channel"}
+FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 10), ("bf16dev", 2), ("e3m12dev", 3)]
+SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15)]
+
+
+def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps):
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    monkeypatch.setenv("FFB_TWO_STEP", "1" if pairs else "0")
+    s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=0.83, U_MAX=0.02, N_SIN=1.0, U_WALL=0.05, U_LID=0.07,
+                 BODY_FORCE=1e-5, precision=precision, storage_scale_log2=scale,
+                 storage_rounding="stochastic")
+    with Lattice(s) as lat:
+        lat.init()
+        lat.set_populations(f0)
+        lat.step(steps, write_macro_last=True)
+        return lat.populations(), lat.macros(), lat.steps
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
+@pytest.mark.parametrize("form", FORMS, ids=[f[0] for f in FORMS])
+@pytest.mark.parametrize("mode", list(MODES), ids=list(MODES.values()))
+def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape):
+    precision, scale = form
+    nx, ny = shape
+    print(f"[gpu] two-step pairs mode {mode} {precision} {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(100 * mode + nx + 7 * ny)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    dt = np.float64 if precision == "fp64" else np.float32
+    f0 = (w[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dt)
+    for steps in (2, 7):
+        fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps)
+        fb, mb, nb = _run(monkeypatch, False, mode, nx, ny, precision, scale, f0, steps)
+        assert na == nb == steps
+        np.testing.assert_array_equal(fa, fb)
+        for a, b in zip(ma, mb):
+            np.testing.assert_array_equal(a, b)
+        assert not np.array_equal(fa, f0.astype(fa.dtype))  # the state moved
+
+
+def test_pairs_in_chunks_equal_one_call(monkeypatch):
+    """A run split into calls of odd length (pairs, then a single step per
+    call) lands on the same state as one call."""
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    print("[gpu] two-step pairs chunked 200x90 fp32 cavity", flush=True)
+    monkeypatch.setenv("FFB_TWO_STEP", "1")
+    s = Settings(N_X=200, N_Y=90, SIM_MODE=4, OMEGA=1.2, U_LID=0.1, precision="fp32")
+    with Lattice(s) as a, Lattice(s) as b:
+        a.init().step(30, write_macro_last=True)
+        b.init()
+        for n in (3, 5, 1, 7, 14):
+            b.step(n, write_macro_last=True)
+        np.testing.assert_array_equal(a.populations(), b.populations())
+        for x, y in zip(a.macros(), b.macros()):
+            np.testing.assert_array_equal(x, y)
