@@ -819,72 +819,86 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
 }
 
 // Two steps in one pass (temporal blocking), the modes without strip
-// passes (SW, CF, SL, BF).  A workgroup owns an output tile of (64 - 2) x
-// (ROWS - 2) cells and computes step 1 on the tile plus a one-cell ring
-// (64 x ROWS cells, one per thread; on a periodic axis the ring cell is the
-// wrapped cell, whose step-1 values the reference's halo copy would hold),
-// keeps the ring's results in LDS in the stored representation (so the 16-bit
-// forms round exactly as two single steps do), and computes step 2 of the
-// tile from LDS: pulled values from the neighbouring LDS cells, and, where
+// passes (SW, CF, SL, BF).  A workgroup of WAVES waves owns an output tile of
+// 62 x (WAVES RPT - 2) cells and computes step 1 on the tile plus a one-cell
+// ring: a region of 64 x WAVES RPT cells, RPT rows per thread (wave w takes
+// rows w, w + WAVES, ...).  On a periodic axis the ring cell is the wrapped
+// cell, whose step-1 values the reference's halo copy would hold.  The
+// region's step-1 results stay in LDS in the stored representation (so the
+// 16-bit forms round exactly as two single steps do), and step 2 of the tile
+// reads them there: pulled values from the neighbouring LDS cells, and, where
 // the source lies beyond a wall, the cell's own opposite population plus the
 // moving-wall term, as pull_edge.  Every cell update is the single step's
 // arithmetic, so the result is bitwise that of two step_kernel launches.
 // Step 1 runs with a.step_seed and no macro output, step 2 with
-// a.step_seed + 1 and a.write_macro.  HBM traffic per two steps: ~(1.2 + 1)
-// x 9 x s bytes per cell against 36 x s.
-template <typename R> __host__ __device__ constexpr int two_step_rows() { return sizeof(R) == 8 ? 8 : 16; }
-
-template <typename S, typename C, int MODE>
-__global__ void __launch_bounds__(64 * two_step_rows<typename Store<S>::R>()) step2_kernel(StepArgs<C> a) {
+// a.step_seed + 1 and a.write_macro.  Tiles are numbered row by row
+// (a.n_tiles_x per tile row) and mapped to workgroups XCD-grouped
+// (tile_of_block).  HBM traffic per two steps: ~(1.2 + 1) x 9 x s bytes per
+// cell against 36 x s.
+template <typename S, typename C, int MODE, int WAVES, int RPT>
+__global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
     static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
     using R = typename Store<S>::R;
-    constexpr int ROWS = two_step_rows<R>();
+    constexpr int ROWS = WAVES * RPT;
     __shared__ R lds[9][ROWS][64];
     constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
-    const int lane = (int)threadIdx.x & 63, r = (int)threadIdx.x >> 6;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nx = a.nx, ny = a.ny;
-    const int ux = (int)blockIdx.x * 62 - 1 + lane, uy = (int)blockIdx.y * (ROWS - 2) - 1 + r;  // unwrapped
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
+    const int ty = t / a.n_tiles_x, tx = t - ty * a.n_tiles_x;
+    const int ux = tx * 62 - 1 + lane;  // unwrapped region column
+    const int uy0 = ty * (ROWS - 2) - 1;  // unwrapped first region row
     const R *__restrict__ f = static_cast<const R *>(a.src);
     R *__restrict__ g = static_cast<R *>(a.dst);
-    // step 1 on the region cell (needed where -1 <= ux <= nx, -1 <= uy <= ny)
-    int gx = ux, gy = uy;
-    bool in1 = (ux >= -1) & (ux <= nx) & (uy >= -1) & (uy <= ny);
+    // step 1 on the region cells (needed where -1 <= ux <= nx, -1 <= uy <= ny)
+    int gx = ux;
+    bool xin = (ux >= -1) & (ux <= nx);
     if (PX) gx = ux < 0 ? ux + nx : (ux >= nx ? ux - nx : ux);
-    else in1 &= (ux >= 0) & (ux < nx);
-    if (PY) gy = uy < 0 ? uy + ny : (uy >= ny ? uy - ny : uy);
-    else in1 &= (uy >= 0) & (uy < ny);
-    if (in1) {
-        StepArgs<C> a1 = a;
-        a1.write_macro = 0;
-        C v[9];
-        pull_pops<S, C, MODE>(a1, f, gx, gy, v);
-        collide_mode<S, C, MODE>(a1, v, gx, gy);
-        R o[9];
-        encode_cell<S, C>(a1, v, gx, gy, o);
+    else xin &= (ux >= 0) & (ux < nx);
+    StepArgs<C> a1 = a;
+    a1.write_macro = 0;
 #pragma unroll
-        for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[d];
+    for (int k = 0; k < RPT; ++k) {
+        const int r = w + WAVES * k, uy = uy0 + r;
+        int gy = uy;
+        bool in1 = xin & (uy >= -1) & (uy <= ny);
+        if (PY) gy = uy < 0 ? uy + ny : (uy >= ny ? uy - ny : uy);
+        else in1 &= (uy >= 0) & (uy < ny);
+        if (in1) {
+            C v[9];
+            pull_pops<S, C, MODE>(a1, f, gx, gy, v);
+            collide_mode<S, C, MODE>(a1, v, gx, gy);
+            R o[9];
+            encode_cell<S, C>(a1, v, gx, gy, o);
+#pragma unroll
+            for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[d];
+        }
     }
     __syncthreads();
     // step 2 on the tile
-    if ((lane >= 1) & (lane <= 62) & (r >= 1) & (r <= ROWS - 2) & (ux < nx) & (uy < ny)) {
-        C v[9];
+    StepArgs<C> a2 = a;
+    a2.step_seed = a.step_seed + 1u;
 #pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            const int sx = ux - cx_of(d), sy = uy - cy_of(d);
-            const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
-            const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
-            const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
-            v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
-        }
-        if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
-            if (uy == ny - 1) {
-                v[7] = v[7] - a.corr8;
-                v[8] = v[8] + a.corr9;
+    for (int k = 0; k < RPT; ++k) {
+        const int r = w + WAVES * k, uy = uy0 + r;
+        if ((lane >= 1) & (lane <= 62) & (r >= 1) & (r <= ROWS - 2) & (ux < nx) & (uy < ny)) {
+            C v[9];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const int sx = ux - cx_of(d), sy = uy - cy_of(d);
+                const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
+                const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
+                const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
+                v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
             }
+            if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
+                if (uy == ny - 1) {
+                    v[7] = v[7] - a.corr8;
+                    v[8] = v[8] + a.corr9;
+                }
+            }
+            collide_store<S, C, MODE>(a2, v, g, ux, uy);
         }
-        StepArgs<C> a2 = a;
-        a2.step_seed = a.step_seed + 1u;
-        collide_store<S, C, MODE>(a2, v, g, ux, uy);
     }
 }
 

@@ -203,21 +203,33 @@ int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) 
     }
 }
 
-// Two steps in one launch (step2_kernel): tiles of 62 x (rows - 2) cells.
-template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_macro) {
-    const StepArgs<C> a = make_args<C>(L, write_macro);
-    constexpr int ROWS = two_step_rows<typename Store<S>::R>();
-    const dim3 grid((unsigned)((a.nx + 61) / 62), (unsigned)((a.ny + ROWS - 3) / (ROWS - 2)));
-    const dim3 block(64 * ROWS);
+// Two steps in one launch (step2_kernel): tiles of 62 x (WAVES RPT - 2) cells.
+template <typename S, typename C, int WAVES, int RPT> int launch_step2_shape(ffb_lattice *L, StepArgs<C> a) {
+    constexpr int ROWS = WAVES * RPT;
+    a.n_tiles_x = (a.nx + 61) / 62;
+    a.n_blocks = a.n_tiles_x * ((a.ny + ROWS - 3) / (ROWS - 2));
+    a.xcd_group = L->pair_xcd_group;
+    const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
     switch (L->cfg.sim_mode) {
-    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW>), grid, block, 0, L->stream, a); break;
-    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF>), grid, block, 0, L->stream, a); break;
-    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((step2_kernel<S, C, BF>), grid, block, 0, L->stream, a); break;
-    case FFB_SLIDING_LID: hipLaunchKernelGGL((step2_kernel<S, C, SL>), grid, block, 0, L->stream, a); break;
+    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((step2_kernel<S, C, BF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_SLIDING_LID: hipLaunchKernelGGL((step2_kernel<S, C, SL, WAVES, RPT>), grid, block, 0, L->stream, a); break;
     default: return fail(FFB_E_INVALID, "two-step pass: Poiseuille flow is not supported");
     }
     HIPCHK(hipGetLastError());
     return FFB_OK;
+}
+
+// Region shapes, measured on the 4096^2 and 8192^2 cavities with 4 tiles per
+// XCD group: 4-byte and smaller elements 16 waves x 2 rows (64 x 32 cells, LDS
+// 73.7 KB for FP32; FP32 4096^2 150.5-152.1 us per step against 153.4-154.2
+// with one row per thread, same box), FP64 8 waves x 1 row (64 x 8, 36.9 KB;
+// 219-220 us per step, against 263 with 16 waves x 1 row on another box).
+template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_macro) {
+    const StepArgs<C> a = make_args<C>(L, write_macro);
+    if constexpr (sizeof(typename Store<S>::R) <= 4) return launch_step2_shape<S, C, 16, 2>(L, a);
+    else return launch_step2_shape<S, C, 8, 1>(L, a);
 }
 
 int dispatch_step2(ffb_lattice *L, int write_macro) {
@@ -512,6 +524,7 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // FP64 387.5 -> 239.2); the 16-bit forms, bound by their arithmetic, ran
     // 14 % slower that way and keep one pass per step.  FFB_TWO_STEP=0/1 overrides.
     L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
+    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 4));
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);

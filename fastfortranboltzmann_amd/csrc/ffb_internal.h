@@ -46,6 +46,7 @@ struct ffb_lattice {
     int vec = 1;             // cells per thread of the step kernel (1, 2, 4, 8)
     int xcd_group = 0;       // G consecutive tiles per XCD, 0: plain order (tile_of_block)
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
+    int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
     long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
     long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };

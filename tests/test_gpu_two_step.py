@@ -2,7 +2,7 @@
 per step (step_kernel): bitwise equal populations and macro fields in every
 mode without a strip pass and every storage form, from random states that
 make every boundary rule carry a distinct value, on shapes smaller and larger
-than one 62 x 14 (FP64: 62 x 6) tile, with odd step counts (the last step of
+than one 62 x 30 (FP64: 62 x 6) tile, with odd step counts (the last step of
 an odd count runs alone) and macro output on the last step.  The pair path
 is the default for the 4- and 8-byte forms, so the oracle parity tests cover
 it too; FFB_TWO_STEP selects the path when a lattice is created."""
@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 MODES = {1: "shear wave", 2: "Couette", 4: "sliding lid", 5: "body-force channel"}
 FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 10), ("bf16dev", 2), ("e3m12dev", 3)]
-SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15)]
+SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15), (70, 67)]
 
 
 def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps):
