@@ -614,6 +614,17 @@ __device__ __forceinline__ void buf_st_nt(R v, __amdgpu_buffer_rsrc_t r, unsigne
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, so, 2);
     }
 }
+// Store with the default cache policy (partial cache lines that neighbouring
+// workgroups on one XCD write merge in its L2).
+template <typename R>
+__device__ __forceinline__ void buf_st(R v, __amdgpu_buffer_rsrc_t r, unsigned xb) {
+    if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, xb, 0, 0);
+    else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, xb, 0, 0);
+    else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, 0, 0);
+    }
+}
 
 // Element xb/sizeof(R) of a row: raw buffer load for 4- and 8-byte elements,
 // global load for 2-byte ones (faster there, measured).
@@ -679,9 +690,10 @@ __device__ __forceinline__ void pull_pops(const StepArgs<C> &a, const typename S
 // with ONE_DESC, through one descriptor at direction 0 and a scalar offset
 // (the same descriptor for the nine rows: fewer scalar registers; needs the
 // nine rows within 4 GiB, which the caller guarantees).
-template <typename R, bool ONE_DESC>
+template <typename R, bool ONE_DESC, bool NT = true>
 __device__ __forceinline__ void store_pop(R v, R *gr, ptrdiff_t P, unsigned xb, int d) {
-    if constexpr (ONE_DESC) buf_st_nt<R>(v, row_rsrc(gr), xb, (unsigned)d * (unsigned)P * (unsigned)sizeof(R));
+    if constexpr (!NT) buf_st<R>(v, row_rsrc(gr + d * P), xb);
+    else if constexpr (ONE_DESC) buf_st_nt<R>(v, row_rsrc(gr), xb, (unsigned)d * (unsigned)P * (unsigned)sizeof(R));
     else buf_st_nt<R>(v, row_rsrc(gr + d * P), xb);
 }
 
@@ -704,7 +716,7 @@ __device__ __forceinline__ void encode_cell(const StepArgs<C> &a, const C (&v)[9
 
 // Collision of the pulled values v of cell (x, y) and the stores of its new
 // populations into g.
-template <typename S, typename C, int MODE, bool ONE_DESC = false>
+template <typename S, typename C, int MODE, bool ONE_DESC = false, bool NT = true>
 __device__ __forceinline__ void collide_store(const StepArgs<C> &a, C (&v)[9], typename Store<S>::R *__restrict__ g,
                                               int x, int y) {
     using R = typename Store<S>::R;
@@ -716,12 +728,12 @@ __device__ __forceinline__ void collide_store(const StepArgs<C> &a, C (&v)[9], t
         if (a.stochastic) {
             const CellBits b = cell_bits((unsigned long long)y * (unsigned long long)a.nx + (unsigned)x, a.step_seed);
 #pragma unroll
-            for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC>(pop_out_sr<S>(v[d], pop_bits(b, d), d), gr, P, xb, d);
+            for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC, NT>(pop_out_sr<S>(v[d], pop_bits(b, d), d), gr, P, xb, d);
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC>(pop_out<S>(v[d], d), gr, P, xb, d);
+    for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC, NT>(pop_out<S>(v[d], d), gr, P, xb, d);
 }
 
 // One cell, any position: pull, collide, store.
@@ -835,7 +847,7 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
 // (a.n_tiles_x per tile row) and mapped to workgroups XCD-grouped
 // (tile_of_block).  HBM traffic per two steps: ~(1.2 + 1) x 9 x s bytes per
 // cell against 36 x s.
-template <typename S, typename C, int MODE, int WAVES, int RPT>
+template <typename S, typename C, int MODE, int WAVES, int RPT, bool NT>
 __global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
     static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
     using R = typename Store<S>::R;
@@ -897,7 +909,7 @@ __global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
                     v[8] = v[8] + a.corr9;
                 }
             }
-            collide_store<S, C, MODE>(a2, v, g, ux, uy);
+            collide_store<S, C, MODE, false, NT>(a2, v, g, ux, uy);
         }
     }
 }

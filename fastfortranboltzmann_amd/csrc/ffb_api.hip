@@ -204,17 +204,17 @@ int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) 
 }
 
 // Two steps in one launch (step2_kernel): tiles of 62 x (WAVES RPT - 2) cells.
-template <typename S, typename C, int WAVES, int RPT> int launch_step2_shape(ffb_lattice *L, StepArgs<C> a) {
+template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_shape(ffb_lattice *L, StepArgs<C> a) {
     constexpr int ROWS = WAVES * RPT;
     a.n_tiles_x = (a.nx + 61) / 62;
     a.n_blocks = a.n_tiles_x * ((a.ny + ROWS - 3) / (ROWS - 2));
     a.xcd_group = L->pair_xcd_group;
     const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
     switch (L->cfg.sim_mode) {
-    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW, WAVES, RPT>), grid, block, 0, L->stream, a); break;
-    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
-    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((step2_kernel<S, C, BF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
-    case FFB_SLIDING_LID: hipLaunchKernelGGL((step2_kernel<S, C, SL, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
+    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
+    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((step2_kernel<S, C, BF, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
+    case FFB_SLIDING_LID: hipLaunchKernelGGL((step2_kernel<S, C, SL, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
     default: return fail(FFB_E_INVALID, "two-step pass: Poiseuille flow is not supported");
     }
     HIPCHK(hipGetLastError());
@@ -228,8 +228,14 @@ template <typename S, typename C, int WAVES, int RPT> int launch_step2_shape(ffb
 // 219-220 us per step, against 263 with 16 waves x 1 row on another box).
 template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_macro) {
     const StepArgs<C> a = make_args<C>(L, write_macro);
-    if constexpr (sizeof(typename Store<S>::R) <= 4) return launch_step2_shape<S, C, 16, 2>(L, a);
-    else return launch_step2_shape<S, C, 8, 1>(L, a);
+    const bool nt = L->pair_nt != 0;
+    if constexpr (sizeof(typename Store<S>::R) <= 4) {
+        if (L->pair_shape == 2)
+            return nt ? launch_step2_shape<S, C, 8, 2, true>(L, a) : launch_step2_shape<S, C, 8, 2, false>(L, a);
+        return nt ? launch_step2_shape<S, C, 16, 2, true>(L, a) : launch_step2_shape<S, C, 16, 2, false>(L, a);
+    } else {
+        return nt ? launch_step2_shape<S, C, 8, 1, true>(L, a) : launch_step2_shape<S, C, 8, 1, false>(L, a);
+    }
 }
 
 int dispatch_step2(ffb_lattice *L, int write_macro) {
@@ -525,6 +531,12 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // 14 % slower that way and keep one pass per step.  FFB_TWO_STEP=0/1 overrides.
     L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
     L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 4));
+    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", 1);
+    // pair stores: default cache policy for 4-byte and narrower elements (the
+    // partial lines at the ends of a 62-cell row, written by two x-neighbouring
+    // tiles on one XCD, merge in its L2; FP32 4096^2 152 -> 133 us per step),
+    // nontemporal for FP64 (220.8 against 223.1 us)
+    L->pair_nt = (int)env_ll("FFB_PAIR_NT", L->elem >= 8 ? 1 : 0);
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);

@@ -47,6 +47,8 @@ struct ffb_lattice {
     int xcd_group = 0;       // G consecutive tiles per XCD, 0: plain order (tile_of_block)
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
     int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
+    int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows
+    int pair_nt = 1;         // pair stores: 1 nontemporal, 0 default cache policy
     long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
     long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };
