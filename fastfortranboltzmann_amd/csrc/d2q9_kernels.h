@@ -847,17 +847,17 @@ __global__ void __launch_bounds__(TW) step_kernel(StepArgs<C> a) {
 // (a.n_tiles_x per tile row) and mapped to workgroups XCD-grouped
 // (tile_of_block).  HBM traffic per two steps: ~(1.2 + 1) x 9 x s bytes per
 // cell against 36 x s.
+// The pair of one tile (tx, ty) of domain a, LDS of the workgroup; every
+// thread of the workgroup calls it with the same tile.
 template <typename S, typename C, int MODE, int WAVES, int RPT, bool NT>
-__global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
+__device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
+                                          typename Store<S>::R (&lds)[9][WAVES * RPT][64]) {
     static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
     using R = typename Store<S>::R;
     constexpr int ROWS = WAVES * RPT;
-    __shared__ R lds[9][ROWS][64];
     constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int nx = a.nx, ny = a.ny;
-    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
-    const int ty = t / a.n_tiles_x, tx = t - ty * a.n_tiles_x;
     const int ux = tx * 62 - 1 + lane;  // unwrapped region column
     const int uy0 = ty * (ROWS - 2) - 1;  // unwrapped first region row
     const R *__restrict__ f = static_cast<const R *>(a.src);
@@ -912,6 +912,14 @@ __global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
             collide_store<S, C, MODE, false, NT>(a2, v, g, ux, uy);
         }
     }
+}
+
+template <typename S, typename C, int MODE, int WAVES, int RPT, bool NT>
+__global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
+    __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
+    const int ty = t / a.n_tiles_x;
+    pair_tile<S, C, MODE, WAVES, RPT, NT>(a, t - ty * a.n_tiles_x, ty, lds);
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,

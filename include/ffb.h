@@ -237,7 +237,10 @@ typedef struct ffb_batch_tuning {
 
 typedef struct ffb_batch ffb_batch_t;
 
-/* precision: any FFB_PREC_* form.  The deviation forms use each lattice's
+/* FP32, FP64 and FP32-deviation batches advance in step pairs, two steps per
+ * launch (bitwise the same; FFB_BATCH_PAIRS=0 at creation selects one launch
+ * per step); the tuning below then applies to odd remainders.
+ * precision: any FFB_PREC_* form.  The deviation forms use each lattice's
  * storage_scale_log2 (ffb_default_scale_log2 at the lattice's speed is a good
  * choice; 0 for FP32 deviations) and round the 16-bit stores stochastically,
  * with random bits from (cell of the lattice, step), as the single-domain form;

@@ -3,7 +3,9 @@ persistent hipGraph-replayed launch, is bitwise equal to the oracle run of
 that lattice alone (FP32, FP64 and the deviation forms at each lattice's own
 scale, the 16-bit ones with stochastic rounding, whose step index the graph
 replays advance on the device), across graph replays, odd/even step counts,
-re-tuning and the autotuner's re-initialisation."""
+re-tuning and the autotuner's re-initialisation; with step pairs in one
+launch (the default for FP32, FP64 and FP32 deviations) and with one launch
+per step."""
 import numpy as np
 import pytest
 
@@ -37,12 +39,20 @@ def oracle_of(L, dtype, steps, scale=0):
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
+@pytest.mark.parametrize("pairs", ["default", "0"])
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"])
 @pytest.mark.parametrize("tile_w,grid", [(256, 0), (128, 8), (512, 3), (1024, 0)])
-def test_batch_matches_oracle_per_lattice(precision, tile_w, grid):
+def test_batch_matches_oracle_per_lattice(monkeypatch, precision, tile_w, grid, pairs):
+    """pairs "default": step pairs in one launch for FP32, FP64 and FP32
+    deviations (the tuning then applies to the odd remainders), one launch per
+    step for the 16-bit forms; "0": one launch per step everywhere."""
     from fastfortranboltzmann_amd.batch import Batch
 
-    print(f"[gpu] batch {precision} tile_w={tile_w} grid={grid}", flush=True)
+    print(f"[gpu] batch {precision} tile_w={tile_w} grid={grid} pairs={pairs}", flush=True)
+    if pairs == "0":
+        monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
+    else:
+        monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
     lats = small_batch()
     dt = {"fp64": np.float64, "fp32": np.float32}.get(precision, precision)
     with Batch(lats, precision) as b:
