@@ -887,26 +887,35 @@ __device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
         }
     }
     __syncthreads();
-    // step 2 on the tile
+    // step 2 on the tile.  A tile whose output cells all have their eight
+    // neighbours inside the domain (most tiles) pulls without the boundary
+    // rules: a branch uniform over the workgroup.
     StepArgs<C> a2 = a;
     a2.step_seed = a.step_seed + 1u;
+    const int x0 = tx * 62;
+    const bool inner = (x0 >= 1) & (x0 + 62 <= nx - 1) & (uy0 >= 0) & (uy0 + ROWS <= ny);
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int r = w + WAVES * k, uy = uy0 + r;
         if ((lane >= 1) & (lane <= 62) & (r >= 1) & (r <= ROWS - 2) & (ux < nx) & (uy < ny)) {
             C v[9];
+            if (inner) {
 #pragma unroll
-            for (int d = 0; d < 9; ++d) {
-                const int sx = ux - cx_of(d), sy = uy - cy_of(d);
-                const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
-                const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
-                const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
-                v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
-            }
-            if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
-                if (uy == ny - 1) {
-                    v[7] = v[7] - a.corr8;
-                    v[8] = v[8] + a.corr9;
+                for (int d = 0; d < 9; ++d) v[d] = pop_in<S>(lds[d][r - cy_of(d)][lane - cx_of(d)], d);
+            } else {
+#pragma unroll
+                for (int d = 0; d < 9; ++d) {
+                    const int sx = ux - cx_of(d), sy = uy - cy_of(d);
+                    const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
+                    const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
+                    const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
+                    v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
+                }
+                if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
+                    if (uy == ny - 1) {
+                        v[7] = v[7] - a.corr8;
+                        v[8] = v[8] + a.corr9;
+                    }
                 }
             }
             collide_store<S, C, MODE, false, NT>(a2, v, g, ux, uy);
