@@ -530,7 +530,10 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // FP64 387.5 -> 239.2); the 16-bit forms, bound by their arithmetic, ran
     // 14 % slower that way and keep one pass per step.  FFB_TWO_STEP=0/1 overrides.
     L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
-    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 4));
+    // pair tiles per XCD group: FP32 4096^2 at groups of 2/4/8/16/32/67 tiles
+    // 147.2/132.5/125.3/123.6/125.3/129.5 us per step, 8192^2 groups of 16/32
+    // 493/488; FP64 best at 4 (221.3 against 222.4 at 8, 226.5 at 16)
+    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", L->elem >= 8 ? 4 : 32));
     L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", 1);
     // pair stores: default cache policy for 4-byte and narrower elements (the
     // partial lines at the ends of a 62-cell row, written by two x-neighbouring

@@ -178,9 +178,9 @@ __global__ void __launch_bounds__(TW)
 // into LDS, step 2 of the tile from LDS), the tiles numbered lattice by
 // lattice (grouped by kind) and mapped to workgroups in XCD groups of 4.
 template <typename S, typename C, int WAVES, int RPT, bool NT>
-__global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, const PTile *tiles, int n) {
+__global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, const PTile *tiles, int n, int G) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
-    const int t = tile_of_block((int)blockIdx.x, n, 4);
+    const int t = tile_of_block((int)blockIdx.x, n, G);
     const PTile pt = load_uniform(tiles + t);
     const BatchLat<C> L = load_uniform(b.lats + pt.lat);
     const StepArgs<C> a = lattice_args<S, C>(b, L);
@@ -250,6 +250,8 @@ struct ffb_batch {
     void *segs = nullptr;     // Seg per row segment: (lattice, row)
     int n_tiles = 0;
     bool pairs = false;       // step pairs in one launch (batch_pair_kernel; 4- and 8-byte forms)
+    int pair_group = 4;       // consecutive pair tiles per XCD (tile_of_block)
+    int pair_shape = 2;       // 4-byte forms: 1 = 16 waves x 2 rows (62 x 30 tiles), 2 = 8 waves x 2 rows (62 x 14)
     void *ptiles = nullptr;   // PTile per step-pair tile
     int n_ptiles = 0;
     ffb_batch_tuning_t tuning{256, 0};
@@ -323,9 +325,9 @@ template <typename C> int upload_tables(ffb_batch *B) {
         }
         L.half_force = (C)0.5 * L.force;
     }
-    // step-pair tiles: 62 x 30 cells (FP64: 62 x 6), lattice by lattice
+    // step-pair tiles: 62 x 14 cells (62 x 30 with the 16-wave shape; FP64 62 x 6), lattice by lattice
     std::vector<PTile> ptiles;
-    const int out_rows = B->elem == 8 ? 6 : 30;
+    const int out_rows = B->elem == 8 ? 6 : (B->pair_shape == 2 ? 14 : 30);
     for (int i : order) {
         const ffb_batch_lattice_t &d = B->desc[(size_t)i];
         for (int ty = 0; ty < (d.n_y + out_rows - 1) / out_rows; ++ty)
@@ -400,17 +402,23 @@ template <typename S, typename C, int WAVES, int RPT, bool NT>
 void launch_pair_c(const ffb_batch *B, int parity, int write_macro) {
     const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
     hipLaunchKernelGGL((batch_pair_kernel<S, C, WAVES, RPT, NT>), dim3((unsigned)B->n_ptiles), dim3(64 * WAVES), 0,
-                       B->stream, b, static_cast<const PTile *>(B->ptiles), B->n_ptiles);
+                       B->stream, b, static_cast<const PTile *>(B->ptiles), B->n_ptiles, B->pair_group);
 }
 
-// Shapes and store policy as the single domain's pairs: 16 waves x 2 rows with
-// default-policy stores for 4-byte elements, 8 waves x 1 row with nontemporal
-// stores for FP64.
+// Store policy as the single domain's pairs (default policy for 4-byte
+// elements, nontemporal for FP64); shapes 8 waves x 2 rows (or 16 x 2) for
+// 4-byte elements, 8 waves x 1 row for FP64.
 void launch_pair(const ffb_batch *B, int parity, int write_macro) {
     switch (B->precision) {
     case FFB_PREC_FP64: launch_pair_c<double, double, 8, 1, true>(B, parity, write_macro); break;
-    case FFB_PREC_FP32_DEV: launch_pair_c<f32dev, float, 16, 2, false>(B, parity, write_macro); break;
-    default: launch_pair_c<float, float, 16, 2, false>(B, parity, write_macro); break;
+    case FFB_PREC_FP32_DEV:
+        if (B->pair_shape == 2) launch_pair_c<f32dev, float, 8, 2, false>(B, parity, write_macro);
+        else launch_pair_c<f32dev, float, 16, 2, false>(B, parity, write_macro);
+        break;
+    default:
+        if (B->pair_shape == 2) launch_pair_c<float, float, 8, 2, false>(B, parity, write_macro);
+        else launch_pair_c<float, float, 16, 2, false>(B, parity, write_macro);
+        break;
     }
 }
 
@@ -527,6 +535,14 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
         // per step; the 16-bit forms always take one launch per step
         const char *e = std::getenv("FFB_BATCH_PAIRS");
         B->pairs = B->elem >= 4 && (e && *e ? std::atoi(e) != 0 : true);
+        const char *gs = std::getenv("FFB_BATCH_PAIR_GROUP");
+        // tiles per XCD group and tile shape, measured on 512 mixed lattices
+        // (FP32, MLUPS): 62 x 30 tiles 106,400 / 106,700 / 106,600 at groups of
+        // 4 / 16 / 32; 62 x 14 tiles (8 waves) 110,400 / 111,400 / 112,100 --
+        // the lower tiles waste less at the lattices' top rows; FP64 best at 4
+        B->pair_group = gs && *gs ? std::max(0, std::atoi(gs)) : (B->elem >= 8 ? 4 : 32);
+        const char *ps = std::getenv("FFB_BATCH_PAIR_SHAPE");
+        B->pair_shape = ps && *ps ? (std::atoi(ps) == 1 ? 1 : 2) : 2;
     }
     B->desc.assign(lattices, lattices + count);
     long long off = 0, moff = 0, poff = 0;
