@@ -532,14 +532,14 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
     // pair tiles per XCD group: FP32 4096^2 at groups of 2/4/8/16/32/67 tiles
     // 147.2/132.5/125.3/123.6/125.3/129.5 us per step, 8192^2 groups of 16/32
-    // 493/488; FP64 best at 4 (221.3 against 222.4 at 8, 226.5 at 16)
-    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", L->elem >= 8 ? 4 : 32));
+    // 493/488; FP64 (default-policy stores) 223.2/220.8/219.6 at 4/16/32
+    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 32));
     L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", 1);
-    // pair stores: default cache policy for 4-byte and narrower elements (the
-    // partial lines at the ends of a 62-cell row, written by two x-neighbouring
-    // tiles on one XCD, merge in its L2; FP32 4096^2 152 -> 133 us per step),
-    // nontemporal for FP64 (220.8 against 223.1 us)
-    L->pair_nt = (int)env_ll("FFB_PAIR_NT", L->elem >= 8 ? 1 : 0);
+    // pair stores: default cache policy (the partial lines at the ends of a
+    // 62-cell row, written by two x-neighbouring tiles on one XCD, merge in its
+    // L2; FP32 4096^2 152 -> 133 us per step; FP64 with groups of 32 tiles
+    // 219.6 against 225.2 with nontemporal stores)
+    L->pair_nt = (int)env_ll("FFB_PAIR_NT", 0);
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);
