@@ -70,7 +70,7 @@ def uses_pairs(precision: str) -> bool:
             return int(env) != 0
         except ValueError:
             return False
-    return BYTES_PER_POP[precision] >= 4
+    return precision != "e3m12dev"
 
 
 def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0,

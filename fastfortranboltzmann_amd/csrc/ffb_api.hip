@@ -525,16 +525,18 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         const int g = rows ? (half ? 2 * tiles_x : (cfg->precision == FFB_PREC_FP64 ? std::max(1, tiles_x / 2) : 0)) : 0;
         L->xcd_group = (int)std::max(0LL, env_ll("FFB_XCD_GROUP", g));
     }
-    // step pairs in one pass (step2_kernel) for the 4- and 8-byte forms, where
-    // they were measured faster (FP32 4096^2 cavity 192.5 -> 146.9 us per step,
-    // FP64 387.5 -> 239.2); the 16-bit forms, bound by their arithmetic, ran
-    // 14 % slower that way and keep one pass per step.  FFB_TWO_STEP=0/1 overrides.
-    L->two_step = env_ll("FFB_TWO_STEP", L->elem >= 4 ? 1 : 0) ? 1 : 0;
+    // step pairs in one pass (step2_kernel) where they were measured faster:
+    // FP32 4096^2 cavity 192.5 -> 124 us per step, FP64 387.5 -> 218; FP16 and
+    // BF16 deviations 8192^2 382 -> 372 (8-wave regions); the 1-3-12 form, whose
+    // encoding costs more arithmetic, ran slower (394 -> 439) and keeps one pass
+    // per step.  FFB_TWO_STEP=0/1 overrides.
+    const bool pairs_default = cfg->precision != FFB_PREC_E3M12_DEV;
+    L->two_step = env_ll("FFB_TWO_STEP", pairs_default ? 1 : 0) ? 1 : 0;
     // pair tiles per XCD group: FP32 4096^2 at groups of 2/4/8/16/32/67 tiles
     // 147.2/132.5/125.3/123.6/125.3/129.5 us per step, 8192^2 groups of 16/32
     // 493/488; FP64 (default-policy stores) 223.2/220.8/219.6 at 4/16/32
     L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 32));
-    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", 1);
+    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", L->elem == 2 ? 2 : 1);  // 2: 8 waves x 2 rows
     // pair stores: default cache policy (the partial lines at the ends of a
     // 62-cell row, written by two x-neighbouring tiles on one XCD, merge in its
     // L2; FP32 4096^2 152 -> 133 us per step; FP64 with groups of 32 tiles

@@ -130,10 +130,11 @@ int ffb_init(ffb_lattice_t *lat);
 /* Enqueues n_steps time steps (main.f90:194-262 loop body, incl.
  * exchange_halos and the Poiseuille pre-pass, then execute_local_sim_step,
  * simulation.f90:18-73).  When write_macro_last != 0 the last step also
- * writes rho, u_x, u_y (the reference's write_macro_fields).  For FP32, FP64
- * and FP32 deviations (not Poiseuille flow) consecutive steps run in pairs,
- * two updates per kernel launch; the results are bitwise those of one launch
- * per step (FFB_TWO_STEP=0 at ffb_create selects that, =1 pairs for any form). */
+ * writes rho, u_x, u_y (the reference's write_macro_fields).  In every form
+ * but the 1-3-12 layout (and not in Poiseuille flow) consecutive steps run in
+ * pairs, two updates per kernel launch; the results are bitwise those of one
+ * launch per step (FFB_TWO_STEP=0 at ffb_create selects that, =1 pairs for any
+ * form). */
 int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
 
 /* Changes the physical parameters of an existing handle (the next steps use
