@@ -4,9 +4,9 @@ Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
-whole domain; the library runs the 4- and 8-byte forms as step pairs, two
-updates per kernel launch (temporal blocking, bitwise the same result), and
-the line also reports one launch per step ("single_pass").  Inputs are
+whole domain; the library runs every form but the 1-3-12 layout as step
+pairs, two updates per kernel launch (temporal blocking, bitwise the same
+result), and the line also reports one launch per step ("single_pass").  Inputs are
 resident in HBM before timing.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
