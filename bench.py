@@ -73,6 +73,19 @@ def uses_pairs(precision: str) -> bool:
     return precision != "e3m12dev"
 
 
+def batch_uses_pairs(precision: str) -> bool:
+    """Whether a batch runs step pairs (FFB_BATCH_PAIRS or its default: FP32,
+    FP64 and FP32 deviations); the autotuner then tunes the single-step kernel
+    that runs odd remainders."""
+    env = os.environ.get("FFB_BATCH_PAIRS")
+    if env:
+        try:
+            return int(env) != 0 and precision in ("fp32", "fp64", "fp32dev")
+        except ValueError:
+            pass
+    return precision in ("fp32", "fp64", "fp32dev")
+
+
 def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0,
             pairs: bool | None = None) -> dict:
     from fastfortranboltzmann_amd import Lattice, Settings
@@ -205,6 +218,7 @@ def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int 
     return {"lattices": len(lats), "cells": info["cells"], "steps": steps, "mlups": round(mlups, 1),
             "vs_single_4096_fp32": round(mlups / single_fp32_mlups, 3),
             "vs_single_pass_4096_fp32": round(mlups / single_pass_fp32_mlups, 3),
+            "step_pairs": batch_uses_pairs("fp32"),
             "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]},
             "e3m12dev": {"mlups": round(out["e3m12dev"][0], 1),
                          "vs_batch_fp32": round(out["e3m12dev"][0] / mlups, 3),

@@ -34,6 +34,14 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     assert os.environ["FFB_TWO_STEP"] == "1"
 
 
+def test_batch_pair_selection(monkeypatch):
+    monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
+    assert bench.batch_uses_pairs("fp32") and bench.batch_uses_pairs("fp64")
+    assert not bench.batch_uses_pairs("e3m12dev") and not bench.batch_uses_pairs("fp16dev")
+    monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
+    assert not bench.batch_uses_pairs("fp32")
+
+
 def test_traffic_lines_for_both_paths():
     for workload in ("sl4096_fp32", "sl4096_fp64", "sl4096_fp32_single_pass", "sl4096_fp64_single_pass"):
         t = bench.traffic_from_profiles(workload)
