@@ -4,26 +4,41 @@ Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
-whole domain; the library runs every form but the 1-3-12 layout as step
-pairs, two updates per kernel launch (temporal blocking, bitwise the same
-result), and the line also reports one launch per step ("single_pass").  Inputs are
-resident in HBM before timing.
+whole domain; the library runs consecutive steps as step pairs, two updates
+per kernel launch (temporal blocking, bitwise the same result), and the line
+also reports one launch per step ("single_pass").  Inputs are resident in
+HBM before timing.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64|...]
 
-Prints ONE JSON line: MLUPS (= N_X*N_Y*steps/s/1e6, source/layouts/xyd/main.f90:326),
-the HBM roofline of the step kernel (algorithmic 18*s bytes per cell update,
-HIP events on the library's stream), an FP64 run of the same workload, a
-copy-kernel ceiling, and the CPU baseline: the reference's own kernel
-modules compiled by flang -O3 (oracle/_ref) on one host core, or, where that
-binary is absent, the CPU oracle on all host cores.  Unless --no-extras, the
-line also carries the deviation storage forms (8192^2 speed, 1024^2 error
-against FP64), the 512-lattice batched form, the adjoint's backward/forward
-cost and the 256^2 shear-wave case with its fitted viscosity.
+Prints ONE JSON line: MLUPS (= N_X*N_Y*steps/s/1e6, source/layouts/xyd/main.f90:326)
+and the HBM roofline of the kernel that runs the workload.  `roofline.achieved`
+is the bytes one launch must move -- one read and one write of the stored
+state, 18*s bytes per cell, whatever the number of updates the launch
+performs -- over the launch's average duration (HIP events on the library's
+stream); `effective_gbs` is the per-update rate 18*s*cells*steps/time, above
+the peak for step pairs, and not a roofline figure.  `roofline.traffic` is the
+HBM traffic per launch measured by rocprofv3 (profiles/traffic.txt), reported
+only when that file was written from the kernel sources being benched.  The
+line also carries an FP64 run of the workload, a copy-kernel ceiling, and the
+CPU baseline: the reference's own kernel modules compiled by flang
+(oracle/_ref) on the host's cores.  Unless --no-extras, it also carries the
+deviation storage forms (8192^2 speed, error against FP64), the 512-lattice
+batched form, the adjoint's backward/forward cost and the 256^2 shear-wave
+case with its fitted viscosity.
+
+Profiling (repo root, GPU host):
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir_f> -o run -- python3 bench.py --profile-run
+    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir_w> -o run -- python3 bench.py --profile-run
+    python3 bench.py --record-traffic <dir_f>/run_counter_collection.csv <dir_w>/run_counter_collection.csv
+writes profiles/traffic.txt for the current kernel sources.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
+import csv
+import hashlib
 import json
 import os
 import re
@@ -36,10 +51,23 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 BYTES_PER_POP = {"fp64": 8, "fp32": 4, "fp16dev": 2, "bf16dev": 2, "fp32dev": 4, "e3m12dev": 2}
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic.txt")
+MARK_START, MARK_END = 1 << 20, 2 << 20  # bytes of the marker copies of --profile-run
+KERNEL_SOURCES = ("d2q9_kernels.h", "ffb_internal.h", "ffb_api.hip", "ffb_batch.hip", "ffb_adjoint.hip")
+CPU_THREADS = 16  # the host share of one GPU on the MI355X boxes
 
 
 def cavity_omega(re: float = 1000.0, u: float = 0.1, n: int = 4096) -> float:
     return 1.0 / (3.0 * u * n / re + 0.5)
+
+
+def kernel_source_hash() -> str:
+    """Hash of the HIP sources libffb_hip.so is built from (stamps profiles/traffic.txt)."""
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(REPO, "fastfortranboltzmann_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
 
 
 class pass_mode:
@@ -63,27 +91,36 @@ class pass_mode:
 
 
 def uses_pairs(precision: str) -> bool:
-    """Whether the library runs step pairs for this form (FFB_TWO_STEP or its default)."""
+    """Whether the library runs step pairs for this form (FFB_TWO_STEP, default on)."""
     env = os.environ.get("FFB_TWO_STEP")
     if env is not None:
         try:
             return int(env) != 0
         except ValueError:
             return False
-    return precision != "e3m12dev"
+    return precision in BYTES_PER_POP
 
 
 def batch_uses_pairs(precision: str) -> bool:
-    """Whether a batch runs step pairs (FFB_BATCH_PAIRS or its default: FP32,
-    FP64 and FP32 deviations); the autotuner then tunes the single-step kernel
-    that runs odd remainders."""
+    """Whether a batch runs step pairs (FFB_BATCH_PAIRS, default: the 4- and
+    8-byte forms; the 16-bit forms take one launch per step)."""
     env = os.environ.get("FFB_BATCH_PAIRS")
     if env:
         try:
-            return int(env) != 0 and precision in ("fp32", "fp64", "fp32dev")
+            return int(env) != 0
         except ValueError:
             pass
-    return precision in ("fp32", "fp64", "fp32dev")
+    return BYTES_PER_POP.get(precision, 0) >= 4
+
+
+def hbm_rates(mlups: float, precision: str, per_launch: int) -> dict:
+    """For a run at `mlups` with `per_launch` updates per launch: the rate at
+    which its launches move the bytes they must move (one read and one write
+    of the state per launch), that rate over the 8 TB/s peak, and the
+    per-update 'effective' rate (18*s bytes per cell update)."""
+    eff = mlups * 1e6 * 18 * BYTES_PER_POP[precision] / 1e9
+    return {"min_bytes_gbs": round(eff / per_launch, 1), "frac": round(eff / per_launch / HBM_PEAK_GBS, 4),
+            "effective_gbs": round(eff, 1)}
 
 
 def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0,
@@ -105,56 +142,77 @@ def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0
         lat.sync()
         wall = time.perf_counter() - t0
         ev_ms = lat.elapsed_ms()
-        mass = lat.moments()[0]
     cells = n * n
     # launches: step pairs (and a single step for an odd count), or one per step
     launches = steps // 2 + steps % 2 if paired else steps
     per_launch = 2 if paired and steps % 2 == 0 else 1
     kernel_ms = ev_ms / launches  # the kernel's average launch duration
-    alg_bytes = 18 * BYTES_PER_POP[precision] * cells  # per cell update (step)
-    achieved = alg_bytes * steps / (ev_ms * 1e-3) / 1e9
+    must_move = 18 * BYTES_PER_POP[precision] * cells  # one read + one write of the state
     return {
         "mlups": cells * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms": kernel_ms,
         "steps_per_launch": per_launch,
-        "alg_bytes_per_launch": alg_bytes * per_launch,
-        "achieved_gbs": achieved,
-        "mass_per_cell": mass / cells,
+        "bytes_per_launch": must_move,
+        "achieved_gbs": must_move / (kernel_ms * 1e-3) / 1e9,
+        "effective_gbs": must_move * steps / (ev_ms * 1e-3) / 1e9,
     }
 
 
-def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000) -> dict:
+def roofline(run: dict, workload: str) -> dict:
+    """Roofline block of a run_gpu result: achieved = bytes the launch must
+    move / launch duration; traffic = measured HBM bytes per launch of the
+    same kernel (null unless profiles/traffic.txt matches these sources)."""
+    tr = traffic_from_profiles(workload)
+    out = {"bound": "hbm", "achieved": round(run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(run["achieved_gbs"] / HBM_PEAK_GBS, 4),
+           "traffic": tr["bytes"] if tr else None,
+           "bytes_per_launch": run["bytes_per_launch"], "steps_per_launch": run["steps_per_launch"]}
+    if tr:
+        out["traffic_over_min"] = round(tr["bytes"] / run["bytes_per_launch"], 4)
+        out["traffic_gbs"] = round(tr["bytes"] / (run["kernel_ms"] * 1e-3) / 1e9, 1)
+        out["kernel"] = tr["kernel"]
+    else:
+        out["traffic_note"] = traffic_status()
+    return out
+
+
+def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000, big_err_steps: int = 5000) -> dict:
     """Deviation storage forms: MLUPS at n^2 (cavity, Re 5000) against FP32 at
-    the same size, and the relative L2 velocity error and mass drift against
-    FP64 on a smaller cavity run (err_n^2, Re 1000, err_steps steps), for the
-    16-bit forms (FP16, BF16, the 1-3-12 layout), FP32 deviations and plain
-    FP32; for each form also the density
-    error and the relative error of the shear-wave viscosity fitted in that form
-    (256^2, omega 1, 2000 steps) against nu = (1/omega - 1/2)/3."""
+    the same size (both in step pairs, the default; and against FP32 with one
+    launch per step), with the bytes their launches must move; the relative
+    L2 velocity error, density error and mass drift against FP64 on a 1024^2
+    cavity (Re 1000, err_steps steps) and on the n^2 Re 5000 cavity itself
+    after big_err_steps steps; and the relative error of the shear-wave
+    viscosity fitted in each form (256^2, omega 1, 2000 steps) against
+    nu = (1/omega - 1/2)/3."""
     from fastfortranboltzmann_amd.precision import (cavity_settings, compare, default_scale_log2,
                                                     run_form, shear_wave_viscosity)
 
     forms = ("fp32", "fp32dev", "fp16dev", "bf16dev", "e3m12dev")
-    out = {"size": n, "steps": steps, "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps"}
-    rates = {}
-    for form in forms:
-        s = default_scale_log2(form) if form.endswith("dev") else 0
-        rates[form] = run_form(cavity_settings(n, 5000.0, 0.1, form, s), steps, chunk=steps)["mlups"]
-    with pass_mode(False):  # FP32 with one launch per step, as the 16-bit forms run
+    out = {"size": n, "steps": steps, "error_case": f"cavity {err_n}^2 Re 1000, {err_steps} steps",
+           "error_case_big": f"cavity {n}^2 Re 5000, {big_err_steps} steps"}
+    scale = {f: (default_scale_log2(f) if f.endswith("dev") else 0) for f in forms}
+    rates = {f: run_form(cavity_settings(n, 5000.0, 0.1, f, scale[f]), steps, chunk=steps)["mlups"] for f in forms}
+    with pass_mode(False):  # FP32 with one launch per step
         fp32_single = run_form(cavity_settings(n, 5000.0, 0.1, "fp32", 0), steps, chunk=steps)["mlups"]
     ref = run_form(cavity_settings(err_n, 1000.0, 0.1, "fp64"), err_steps, chunk=err_steps)
+    ref_big = run_form(cavity_settings(n, 5000.0, 0.1, "fp64"), big_err_steps, chunk=big_err_steps)
     for form in forms:
-        s = default_scale_log2(form) if form.endswith("dev") else 0
-        r = run_form(cavity_settings(err_n, 1000.0, 0.1, form, s), err_steps, chunk=err_steps)
+        r = run_form(cavity_settings(err_n, 1000.0, 0.1, form, scale[form]), err_steps, chunk=err_steps)
         c = compare(ref, r)
+        rb = run_form(cavity_settings(n, 5000.0, 0.1, form, scale[form]), big_err_steps, chunk=big_err_steps)
+        cb = compare(ref_big, rb)
         sw = shear_wave_viscosity(form)
+        per_launch = 2 if uses_pairs(form) else 1
         out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
                      "speedup_vs_fp32_single_pass": round(rates[form] / fp32_single, 3),
-                     "step_pairs": uses_pairs(form),
-                     "achieved_gbs": round(rates[form] * 1e6 * 18 * BYTES_PER_POP[form] / 1e9, 1),
-                     "scale_log2": s, "rel_l2_u_vs_fp64": c["rel_l2_u"], "max_abs_rho_vs_fp64": c["max_abs_rho"],
-                     "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"], "mass_drift": r["mass_drift"],
+                     "steps_per_launch": per_launch, **hbm_rates(rates[form], form, per_launch),
+                     "scale_log2": scale[form], "rel_l2_u_vs_fp64": c["rel_l2_u"],
+                     "max_abs_rho_vs_fp64": c["max_abs_rho"], "rel_l2_rho_dev_vs_fp64": c["rel_l2_rho_dev"],
+                     "mass_drift": r["mass_drift"],
+                     "big": {"rel_l2_u_vs_fp64": cb["rel_l2_u"], "max_abs_u_vs_fp64": cb["max_abs_u"],
+                             "max_abs_rho_vs_fp64": cb["max_abs_rho"], "mass_drift": rb["mass_drift"]},
                      "shear_wave_nu_rel_err": sw["nu_rel_err"],
                      "rounding": "stochastic" if form in ("fp16dev", "bf16dev", "e3m12dev") else "nearest"}
     out["fp64_mass_drift"] = ref["mass_drift"]
@@ -195,35 +253,34 @@ def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
 
 def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int = 200) -> dict:
     """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
-    Couette / body-force channel), FP32, autotuned persistent launch; the same
-    batch in the 1-3-12 deviation form (each lattice at its default scale,
-    stochastic rounding) beside it."""
-    from fastfortranboltzmann_amd.batch import Batch, mixed_batch
+    Couette / body-force channel), autotuned: FP32 (step pairs), and the
+    1-3-12 and FP16 deviation forms (one launch per step), each with the
+    kernel and launch shape that ran, and every lattice's relative L2
+    velocity error against an FP64 batch of the same lattices after the same
+    steps (median / p90 / max)."""
+    from fastfortranboltzmann_amd.batch import batch_study, mixed_batch
 
     lats = mixed_batch(0, 512)
-    out = {}
-    for form in ("fp32", "e3m12dev"):
-        with Batch(lats, form) as b:
-            tun = b.autotune(20)
-            info = b.info()
-            b.init()
-            b.step(2)
-            b.sync()
-            b.reset_timer()
-            b.step(steps)
-            b.sync()
-            ms = b.elapsed_ms()
-        out[form] = (info["cells"] * steps / (ms * 1e-3) / 1e6, tun)
-    mlups, tun = out["fp32"]
-    return {"lattices": len(lats), "cells": info["cells"], "steps": steps, "mlups": round(mlups, 1),
-            "vs_single_4096_fp32": round(mlups / single_fp32_mlups, 3),
-            "vs_single_pass_4096_fp32": round(mlups / single_pass_fp32_mlups, 3),
-            "step_pairs": batch_uses_pairs("fp32"),
-            "tuning": {"tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]},
-            "e3m12dev": {"mlups": round(out["e3m12dev"][0], 1),
-                         "vs_batch_fp32": round(out["e3m12dev"][0] / mlups, 3),
-                         "tuning": {"tile_w": out["e3m12dev"][1]["tile_w"],
-                                    "grid_blocks": out["e3m12dev"][1]["grid_blocks"]}}}
+    forms = ("fp32", "e3m12dev", "fp16dev")
+    res = batch_study(lats, steps, autotune=True, trial_steps=10, forms=forms, log=lambda m: None)
+    out = {"lattices": len(lats), "cells": res["cells"], "steps": res["steps"]}
+    for form in forms:
+        rec = res["forms"][form]
+        pairs = batch_uses_pairs(form)
+        tun = rec["tuning"]
+        if pairs:
+            launch = {"kernel": "batch_pair_kernel", "pair_shape": tun["pair_shape"],
+                      "pair_group": tun["pair_group"], "pair_grid": tun["pair_grid"]}
+        else:
+            launch = {"kernel": "batch_step_kernel", "tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]}
+        err = rec["rel_l2_u_vs_fp64"]
+        out[form] = {"mlups": round(rec["mlups"], 1), "steps_per_launch": 2 if pairs else 1,
+                     **hbm_rates(rec["mlups"], form, 2 if pairs else 1), "launch": launch,
+                     "rel_l2_u_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()}}
+    out["mlups"] = out["fp32"]["mlups"]
+    out["vs_single_4096_fp32"] = round(out["fp32"]["mlups"] / single_fp32_mlups, 3)
+    out["vs_single_pass_4096_fp32"] = round(out["fp32"]["mlups"] / single_pass_fp32_mlups, 3)
+    return out
 
 
 def adjoint_cost(n: int = 512, steps: int = 2000) -> dict:
@@ -243,49 +300,232 @@ def adjoint_cost(n: int = 512, steps: int = 2000) -> dict:
             "forward_mlups": round(n * n * steps / (f_ms * 1e-3) / 1e6, 1)}
 
 
-def copy_ceiling() -> float | None:
+def copy_ceiling(nbytes: int = 1 << 30, iters: int = 20) -> float | None:
     from fastfortranboltzmann_amd import _lib
-    import ctypes
 
     L = _lib.lib()
-    if not hasattr(L, "ffb_bench_copy"):
-        return None
     gbs = ctypes.c_double()
-    if L.ffb_bench_copy(0, 1 << 30, 20, ctypes.byref(gbs)) != 0:
+    if L.ffb_bench_copy(0, nbytes, iters, ctypes.byref(gbs)) != 0:
         return None
     return gbs.value
 
 
-def traffic_from_profiles(workload: str):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    as recorded in profiles/traffic.txt ("<workload> <fetch_bytes> <write_bytes>")."""
-    path = os.path.join(REPO, "profiles", "traffic.txt")
-    if not os.path.exists(path):
+# ---------------------------------------------------------------------------
+# measured HBM traffic (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+# ---------------------------------------------------------------------------
+
+def _profile_lattice(n: int, precision: str, pairs: bool, launches: int, scale: int = 0):
+    from fastfortranboltzmann_amd import Lattice, Settings
+
+    s = Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=cavity_omega(re=1000.0 if n == 4096 else 5000.0, n=n),
+                 U_LID=0.1, RHO_0=1.0, precision=precision, storage_scale_log2=scale)
+    with pass_mode(pairs):
+        lat = Lattice(s)
+    with lat:
+        lat.init()
+        lat.sync()
+        copy_ceiling(MARK_START, 1)  # the dispatches up to the end marker are this workload's
+        lat.step(launches * (2 if pairs else 1))
+        lat.sync()
+        copy_ceiling(MARK_END, 1)
+
+
+def _profile_batch(precision: str, launches: int):
+    from fastfortranboltzmann_amd.batch import Batch, mixed_batch
+
+    with Batch(mixed_batch(0, 512), precision) as b:
+        b.autotune(10)
+        b.sync()
+        copy_ceiling(MARK_START, 1)
+        b.step(launches * (2 if batch_uses_pairs(precision) else 1))
+        b.sync()
+        copy_ceiling(MARK_END, 1)
+
+
+def _profile_workloads():
+    """(workload, run) in the order --profile-run executes them."""
+    from fastfortranboltzmann_amd.precision import default_scale_log2
+
+    k = 6
+    out = [("sl4096_fp32", lambda: _profile_lattice(4096, "fp32", True, k)),
+           ("sl4096_fp32_single_pass", lambda: _profile_lattice(4096, "fp32", False, k)),
+           ("sl4096_fp64", lambda: _profile_lattice(4096, "fp64", True, k)),
+           ("sl4096_fp64_single_pass", lambda: _profile_lattice(4096, "fp64", False, k))]
+    for form in ("fp32dev", "fp16dev", "bf16dev", "e3m12dev"):
+        s = default_scale_log2(form)
+        out.append((f"sl8192_{form}", lambda form=form, s=s: _profile_lattice(8192, form, True, k, s)))
+    out.append(("sl8192_fp32", lambda: _profile_lattice(8192, "fp32", True, k)))
+    for form in ("fp32", "e3m12dev", "fp16dev"):
+        out.append((f"batch512_{form}", lambda form=form: _profile_batch(form, k)))
+    return out
+
+
+def profile_run() -> int:
+    """The workloads of profiles/traffic.txt, each between two marker copies,
+    then a 1 GiB copy that calibrates the counters."""
+    for name, fn in _profile_workloads():
+        print(f"profile-run: {name}", file=sys.stderr, flush=True)
+        fn()
+    copy_ceiling(1 << 30, 1)
+    print("profile-run: done", file=sys.stderr, flush=True)
+    return 0
+
+
+def _counter_dispatches(path: str):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return [(r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])) for r in rows]
+
+
+def _segments(disp):
+    """The dispatches between each start marker (copy kernels over MARK_START
+    bytes: one thread per 16 bytes) and the following end marker (MARK_END
+    bytes); per segment the kernel with the most dispatches (fills, step
+    counters and initial states left out): (kernel, mean counter value, count)."""
+    start, end = MARK_START // 16, MARK_END // 16
+    segs, cur = [], None
+    for name, grid, val in disp:
+        if "copy_kernel" in name:
+            if grid == start:
+                cur = []
+            elif grid == end and cur is not None:
+                segs.append(cur)
+                cur = None
+            continue
+        if cur is None or "fill" in name.lower() or "count" in name or "init" in name:
+            continue
+        cur.append((name, val))
+    out = []
+    for seg in segs:
+        by = {}
+        for name, val in seg:
+            by.setdefault(name, []).append(val)
+        name = max(by, key=lambda k: len(by[k]))
+        out.append((name, sum(by[name]) / len(by[name]), len(by[name])))
+    return out
+
+
+def record_traffic(fetch_csv: str, write_csv: str) -> int:
+    fetch, write = _counter_dispatches(fetch_csv), _counter_dispatches(write_csv)
+    # calibration: the 1 GiB nontemporal copy (1048576 KiB read, 1048576 KiB written)
+    def calib(disp):
+        vals = [v for n, g, v in disp if "copy_kernel<3>" in n and g > (1 << 20)]
+        if not vals:
+            raise SystemExit("record-traffic: no 1 GiB copy in the counter file")
+        return 1048576.0 / (sum(vals) / len(vals))
+    fk, wk = calib(fetch), calib(write)
+    fs, ws = _segments(fetch), _segments(write)
+    names = [n for n, _ in _profile_workloads()]
+    if not (len(fs) == len(ws) == len(names)):
+        raise SystemExit(f"record-traffic: {len(fs)}/{len(ws)} segments for {len(names)} workloads")
+    lines = [f"# kernel sources {kernel_source_hash()}",
+             "# HBM bytes per launch of the kernel that runs each workload, from two rocprofv3 --pmc",
+             "# passes over `python3 bench.py --profile-run` (FETCH_SIZE, then WRITE_SIZE; KiB per",
+             "# dispatch, averaged over the workload's dispatches), scaled by what the same counters",
+             f"# read for a 1 GiB copy in the same runs: fetch x{fk:.5f}, write x{wk:.5f}.",
+             "# Commands (repo root, GPU host):",
+             "#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d <f> -o run -- python3 bench.py --profile-run",
+             "#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d <w> -o run -- python3 bench.py --profile-run",
+             "#   python3 bench.py --record-traffic <f>/run_counter_collection.csv <w>/run_counter_collection.csv",
+             "# workload fetch_bytes write_bytes dispatches kernel"]
+    for name, (kf, vf, cf), (kw, vw, cw) in zip(names, fs, ws):
+        if kf != kw:
+            raise SystemExit(f"record-traffic: {name}: kernels differ ({kf} / {kw})")
+        lines.append(f"{name} {vf * 1024 * fk:.0f} {vw * 1024 * wk:.0f} {min(cf, cw)} {kf}")
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
+def _traffic_table():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, {}
+    stamp, table = None, {}
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("# kernel sources "):
+            stamp = line.split()[-1]
+        if line.startswith("#") or not line.strip():
+            continue
+        parts = line.split(None, 4)
+        if len(parts) == 5:
+            table[parts[0]] = {"bytes": float(parts[1]) + float(parts[2]), "fetch": float(parts[1]),
+                               "write": float(parts[2]), "kernel": parts[4].strip()}
+    return stamp, table
+
+
+def traffic_status() -> str:
+    stamp, _ = _traffic_table()
+    if stamp is None:
+        return "profiles/traffic.txt missing or unstamped"
+    if stamp != kernel_source_hash():
+        return f"profiles/traffic.txt is from kernel sources {stamp}, these are {kernel_source_hash()}: not used"
+    return "workload not in profiles/traffic.txt"
+
+
+def traffic_from_profiles(workload: str) -> dict | None:
+    """Measured HBM bytes per launch of `workload` from profiles/traffic.txt,
+    or None when the file is stamped with other kernel sources (stale)."""
+    stamp, table = _traffic_table()
+    if stamp != kernel_source_hash():
         return None
-    for line in open(path):
-        parts = line.split()
-        if len(parts) >= 3 and parts[0] == workload:
-            try:
-                return float(parts[1]) + float(parts[2])
-            except ValueError:
-                return None
-    return None
+    return table.get(workload)
 
 
-def cpu_baseline(precision: str) -> dict | None:
-    exe = os.path.join(REPO, "oracle", "_ref", f"bench_sl4096_{precision}", "ref_driver")
-    if os.path.exists(exe):
-        try:
-            out = subprocess.run([exe, "-"], capture_output=True, text=True, timeout=300).stdout
-            m = re.search(r"steps=(\d+).*MLUPS=\s*([0-9.eE+-]+)", out)
-            if m:
-                return {"value": float(m.group(2)), "unit": "MLUPS", "cores": 1, "kind": "reference",
-                        "sample": f"{m.group(1)} steps of the same 4096^2 cavity ({precision}); the "
-                                  "reference's kernel modules (simulation.f90, modes/*.f90) "
-                                  "compiled by flang -O3, one thread = one coarray image"}
-        except (subprocess.TimeoutExpired, OSError):
-            pass
-    return cpu_oracle_threads(precision, 10.0)
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference's own kernels (oracle/_ref, flang)
+# ---------------------------------------------------------------------------
+
+REF_FLAGS = "flang -O3 -march=x86-64-v3 -ffp-contract=off"
+
+
+def _ref_exe(name: str) -> str:
+    return os.path.join(REPO, "oracle", "_ref", name, "ref_driver")
+
+
+def _parse_ref(out: str):
+    m = re.search(r"steps=(\d+)\s+cells=(\d+)\s+seconds=\s*([0-9.eE+-]+)", out)
+    return (int(m.group(1)), int(m.group(2)), float(m.group(3))) if m else None
+
+
+def cpu_reference(precision: str) -> dict | None:
+    """The reference's kernel modules (simulation.f90, modes/*.f90) compiled
+    by flang, timed on the host: one process on the full 4096^2 cavity (one
+    core, one coarray image), and CPU_THREADS concurrent processes each on a
+    4096 x (4096 / CPU_THREADS) strip -- the reference's 1 x I_Y decomposition
+    on all cores without its halo exchange, MLUPS = all cells x steps / the
+    slowest process's loop time (main.f90:287-326)."""
+    one = _ref_exe(f"bench_sl4096_{precision}")
+    strip = _ref_exe(f"bench_sl4096x256_{precision}")
+    if not os.path.exists(one):
+        return None
+    try:
+        r1 = _parse_ref(subprocess.run([one, "-"], capture_output=True, text=True, timeout=300).stdout)
+        res = None
+        if os.path.exists(strip):
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            procs = [subprocess.Popen([strip, "-"], stdout=subprocess.PIPE, text=True, env=env)
+                     for _ in range(CPU_THREADS)]
+            outs = [p.communicate(timeout=300)[0] for p in procs]
+            parsed = [_parse_ref(o) for o in outs]
+            if all(parsed):
+                steps, cells = parsed[0][0], sum(p[1] for p in parsed)
+                slowest = max(p[2] for p in parsed)
+                res = {"value": round(cells * steps / slowest / 1e6, 1), "cores": CPU_THREADS,
+                       "sample": f"{CPU_THREADS} concurrent processes x {steps} steps, each on a "
+                                 f"4096x{4096 // CPU_THREADS} strip of the cavity ({precision}); "
+                                 f"reference kernels ({REF_FLAGS}), one process = one coarray image, "
+                                 "no halo exchange"}
+    except (subprocess.TimeoutExpired, OSError):
+        return None
+    if r1 is None:
+        return None
+    single = {"value": round(r1[1] * r1[0] / r1[2] / 1e6, 1), "cores": 1,
+              "sample": f"{r1[0]} steps of the same 4096^2 cavity ({precision}); reference kernels "
+                        f"({REF_FLAGS}), one process = one coarray image"}
+    if res is None:
+        return {**single, "unit": "MLUPS", "kind": "reference"}
+    return {**res, "unit": "MLUPS", "kind": "reference", "single_core": single}
 
 
 def host_cpu() -> str:
@@ -301,14 +541,13 @@ def host_cpu() -> str:
 
 def cpu_oracle_threads(precision: str, seconds: float = 6.0) -> dict:
     """The CPU oracle (the reference restated in C, same operation order) on up
-    to 16 host threads (OpenMP over rows), timed for about `seconds` on the
-    same 4096^2 cavity."""
+    to CPU_THREADS host threads (OpenMP over rows), timed for about `seconds`
+    on the same 4096^2 cavity; the baseline where the reference build is absent."""
     import numpy as np
 
     from oracle.oracle import OracleLattice, Params, max_threads, set_threads
 
-    threads = min(16, os.cpu_count() or 1)
-    set_threads(threads)
+    set_threads(min(CPU_THREADS, os.cpu_count() or 1))
     o = OracleLattice(4, 4096, 4096, Params(omega=cavity_omega(), u_lid=0.1),
                       np.float64 if precision == "fp64" else np.float32).init()
     o.run(1)
@@ -318,8 +557,17 @@ def cpu_oracle_threads(precision: str, seconds: float = 6.0) -> dict:
         steps += 1
     dt = time.perf_counter() - t0
     return {"value": round(4096 * 4096 * steps / dt / 1e6, 1), "unit": "MLUPS", "cores": max_threads(),
-            "kind": "port", "host": host_cpu(),
-            "sample": f"{steps} steps of the same 4096^2 cavity ({precision}), CPU oracle, OpenMP"}
+            "kind": "port", "sample": f"{steps} steps of the same 4096^2 cavity ({precision}), CPU oracle, OpenMP"}
+
+
+def cpu_baseline(precision: str) -> dict:
+    base = cpu_reference(precision) or cpu_oracle_threads(precision)
+    base["host"] = host_cpu()
+    if precision != "fp64":
+        b64 = cpu_reference("fp64")
+        if b64:
+            base["fp64"] = {k: v for k, v in b64.items() if k not in ("unit", "kind")}
+    return base
 
 
 def main() -> int:
@@ -333,11 +581,18 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the reduced-storage, batched and adjoint measurements")
+    ap.add_argument("--profile-run", action="store_true", help="run the profiled workloads (rocprofv3 --pmc)")
+    ap.add_argument("--record-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.txt from the two counter files of --profile-run")
     args = ap.parse_args()
+    if args.record_traffic:
+        return record_traffic(*args.record_traffic)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the domain is one device-resident "
               "lattice and no multi-GPU path is built (run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.profile_run:
+        return profile_run()
 
     n = args.size
     print(f"bench: cavity {n}^2 {args.precision}, warmup {args.warmup}, steps {args.steps}",
@@ -345,22 +600,18 @@ def main() -> int:
     main_run = run_gpu(n, args.precision, args.steps, args.warmup)
     print("bench: one launch per step", file=sys.stderr, flush=True)
     single = run_gpu(n, args.precision, args.steps, args.warmup, pairs=False)
-    extra = {"steps_per_launch": main_run["steps_per_launch"],
-             "single_pass": {"mlups": round(single["mlups"], 1), "kernel_ms": round(single["kernel_ms"], 5),
-                             "achieved_gbs": round(single["achieved_gbs"], 1),
-                             "frac": round(single["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                             "traffic": traffic_from_profiles(f"sl{n}_{args.precision}_single_pass")}}
+    workload = f"sl{n}_{args.precision}"
+    extra = {"single_pass": {"mlups": round(single["mlups"], 1), "kernel_ms": round(single["kernel_ms"], 5),
+                             "roofline": roofline(single, workload + "_single_pass")}}
     if not args.no_fp64 and args.precision != "fp64":
         print("bench: fp64 run", file=sys.stderr, flush=True)
         r64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup)
         s64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup, pairs=False)
-        extra["fp64"] = {"mlups": round(r64["mlups"], 1), "achieved_gbs": round(r64["achieved_gbs"], 1),
-                         "frac": round(r64["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                         "kernel_ms": round(r64["kernel_ms"], 5), "steps_per_launch": r64["steps_per_launch"],
-                         "single_pass": {"mlups": round(s64["mlups"], 1),
-                                         "achieved_gbs": round(s64["achieved_gbs"], 1),
-                                         "frac": round(s64["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                         "kernel_ms": round(s64["kernel_ms"], 5)}}
+        extra["fp64"] = {"mlups": round(r64["mlups"], 1), "kernel_ms": round(r64["kernel_ms"], 5),
+                         "effective_gbs": round(r64["effective_gbs"], 1),
+                         "roofline": roofline(r64, f"sl{n}_fp64"),
+                         "single_pass": {"mlups": round(s64["mlups"], 1), "kernel_ms": round(s64["kernel_ms"], 5),
+                                         "roofline": roofline(s64, f"sl{n}_fp64_single_pass")}}
     if not args.no_extras and args.precision == "fp32":
         print("bench: reduced storage", file=sys.stderr, flush=True)
         extra["reduced_storage"] = reduced_storage(2 * n if n <= 4096 else n, max(50, args.steps // 10))
@@ -371,14 +622,10 @@ def main() -> int:
         print("bench: shear wave 256^2", file=sys.stderr, flush=True)
         extra["shear_wave_256_fp64"] = shear_wave_256()
     cc = copy_ceiling()
-    workload = f"sl{n}_{args.precision}"
-    traffic = traffic_from_profiles(workload)
     cpu = None
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.precision)
-        if cpu is not None:
-            cpu["host"] = host_cpu()
     line = {
         "metric": "MLUPS + % HBM roofline (plain FP64/FP32 vs reduced/batched/adjoint) at stated error",
         "value": round(main_run["mlups"], 1),
@@ -396,13 +643,12 @@ def main() -> int:
         "config": {"workload": f"lid-driven cavity {n}x{n}, bounce-back walls + moving lid u=0.1, "
                                f"Re=1000 (omega={cavity_omega(n=n):.6f}), {args.precision} populations",
                    "n_x": n, "n_y": n, "sim_mode": "sliding_lid", "precision": args.precision},
-        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+        "roofline": roofline(main_run, workload),
         "cpu_baseline": cpu,
         "kernel_ms": round(main_run["kernel_ms"], 5),
+        "effective_gbs": round(main_run["effective_gbs"], 1),
         "copy_ceiling_gbs": round(cc, 1) if cc else None,
+        "kernel_sources": kernel_source_hash(),
         **extra,
     }
     print(json.dumps(line), flush=True)

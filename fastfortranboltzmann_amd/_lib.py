@@ -14,12 +14,14 @@ LIB_PATH = os.path.join(PKG, "libffb_hip.so")
 SHEAR_WAVE, COUETTE_FLOW, POISEUILLE_FLOW, SLIDING_LID, BODY_FORCE_CHANNEL = 1, 2, 3, 4, 5
 PREC_FP32, PREC_FP64, PREC_FP16_DEV, PREC_BF16_DEV, PREC_FP32_DEV, PREC_E3M12_DEV = 0, 1, 2, 3, 4, 5
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+E_INVALID, E_HIP, E_STATE, E_DENSITY = 1, 2, 3, 4
 
 # every entry point include/ffb.h declares
 EXPORTS = (
     "ffb_config_default", "ffb_create", "ffb_destroy", "ffb_init", "ffb_step", "ffb_sync",
     "ffb_read_macros", "ffb_read_populations", "ffb_write_populations", "ffb_read_pf_strips",
-    "ffb_elapsed_ms", "ffb_reset_timer", "ffb_step_count", "ffb_moments", "ffb_deviation_extent",
+    "ffb_elapsed_ms", "ffb_reset_timer", "ffb_step_count", "ffb_moments",
+    "ffb_deviation_extent",
     "ffb_default_scale_log2",
     "ffb_device_info",
     "ffb_bench_copy", "ffb_last_error", "ffb_version",
@@ -45,7 +47,8 @@ class FfbBatchLattice(ctypes.Structure):
 class FfbBatchTuning(ctypes.Structure):
     """Mirror of ffb_batch_tuning_t (include/ffb.h)."""
 
-    _fields_ = [("tile_w", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
+    _fields_ = [("tile_w", ctypes.c_int32), ("grid_blocks", ctypes.c_int32), ("pair_shape", ctypes.c_int32),
+                ("pair_group", ctypes.c_int32), ("pair_grid", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class FfbConfig(ctypes.Structure):
@@ -60,7 +63,7 @@ class FfbConfig(ctypes.Structure):
         ("rho_in", ctypes.c_double), ("rho_out", ctypes.c_double),
         ("storage_scale_log2", ctypes.c_int32), ("device", ctypes.c_int32),
         ("body_force", ctypes.c_double),
-        ("storage_rounding", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("storage_rounding", ctypes.c_int32), ("density_checks", ctypes.c_int32),
     ]
 
 

@@ -1,8 +1,8 @@
 """Batched form: hundreds of independent lattices of uneven size, each with
-its own omega and boundary kind, advanced by one persistent launch per step
-(hipGraph-replayed step pairs) in libffb_hip.so, in any storage form (the
-deviation forms with a scale per lattice); see ffb_batch_* in include/ffb.h.
-No reference counterpart.
+its own omega and boundary kind, advanced two steps per launch of one
+persistent kernel (hipGraph-replayed step pairs) in libffb_hip.so, in any
+storage form (the deviation forms with a scale per lattice); see ffb_batch_*
+in include/ffb.h.  No reference counterpart.
 
 The mixed workload (``mixed_batch``): side lengths uniform on [64, 1024]
 (x and y drawn separately, so non-square), omega uniform on [0.6, 1.9], kinds
@@ -130,20 +130,34 @@ class Batch:
     def reset_timer(self):
         check(lib().ffb_batch_reset_timer(self._h))
 
+    @staticmethod
+    def _tuning_dict(t) -> dict:
+        return {k: getattr(t, k) for k in ("tile_w", "grid_blocks", "pair_shape", "pair_group", "pair_grid")}
+
     def info(self):
         cells, tiles, t = ctypes.c_int64(), ctypes.c_int32(), FfbBatchTuning()
         check(lib().ffb_batch_info(self._h, ctypes.byref(cells), ctypes.byref(tiles), ctypes.byref(t)))
-        return {"cells": cells.value, "tiles": tiles.value,
-                "tuning": {"tile_w": t.tile_w, "grid_blocks": t.grid_blocks}}
+        return {"cells": cells.value, "tiles": tiles.value, "tuning": self._tuning_dict(t)}
 
-    def set_tuning(self, tile_w: int, grid_blocks: int = 0):
-        t = FfbBatchTuning(tile_w=tile_w, grid_blocks=grid_blocks)
+    def set_tuning(self, tile_w: int | None = None, grid_blocks: int | None = None, **pair):
+        """Changes the launch shape; arguments left out keep their value.
+        pair: pair_shape, pair_group, pair_grid (step pairs)."""
+        cur = self.info()["tuning"]
+        if tile_w is not None:
+            cur["tile_w"] = tile_w
+        if grid_blocks is not None:
+            cur["grid_blocks"] = grid_blocks
+        for k, v in pair.items():
+            if k not in ("pair_shape", "pair_group", "pair_grid"):
+                raise TypeError(f"unknown tuning field {k!r}")
+            cur[k] = v
+        t = FfbBatchTuning(**cur)
         check(lib().ffb_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, trial_steps: int = 20):
         t, ms = FfbBatchTuning(), ctypes.c_double()
         check(lib().ffb_batch_autotune(self._h, int(trial_steps), ctypes.byref(t), ctypes.byref(ms)))
-        return {"tile_w": t.tile_w, "grid_blocks": t.grid_blocks, "ms_per_step": ms.value}
+        return {**self._tuning_dict(t), "ms_per_step": ms.value}
 
     def macros(self, i: int):
         L = self.lattices[i]

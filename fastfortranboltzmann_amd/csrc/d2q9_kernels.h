@@ -280,7 +280,22 @@ template <typename C> struct StepArgs {
     unsigned step_seed;    // step index for the stochastic-rounding hash
     int n_blocks;          // grid size (for the XCD-aware tile order)
     int xcd_group;         // G > 0: G consecutive tiles per XCD (tile_of_block)
+    int check_rho;         // FFB_DENSITY_CHECKS: flag densities rho <= 0 (or NaN) in *status
+    unsigned *status;      // bit 0: rho <= 0 met, bit 1: rho NaN met
 };
+
+// FFB_DENSITY_CHECKS (simulation.f90:229-233, :133-137,
+// modes/poiseuille_flow.f90:156-160): the reference stops at a density
+// rho <= 0 before dividing by it; here the cell sets bit 0 of the status word
+// (bit 1 for a NaN density, which the reference's test lets pass) and the
+// host reports it (ffb_sync).  A branch on a kernel argument when off.
+template <typename C>
+__device__ __forceinline__ void density_check(const StepArgs<C> &a, C rho) {
+    if (a.check_rho) {
+        if (rho <= (C)0) atomicOr(a.status, 1u);
+        else if (rho != rho) atomicOr(a.status, 2u);
+    }
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
 // (workgroup p to XCD p mod 8), each XCD with its own L2.  With G > 0 the
@@ -421,6 +436,7 @@ __device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, 
     C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
+    density_check(a, rho);
     ux = ux / rho;
     uy = uy / rho;
     const C usq = ux * ux + uy * uy;
@@ -462,6 +478,7 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
     C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
+    density_check(a, rho);
     ux = (ux + (C)0.5 * F) / rho;
     uy = uy / rho;
     const C usq = ux * ux + uy * uy;
@@ -510,6 +527,7 @@ __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int
     const C jy = (b24 + b57) + b68;
     const C drho = m0 * inv;
     const C rho = (C)1 + drho;
+    density_check(a, rho);
     const C r = (C)1 / rho;
     const C ux = (MODE == BF ? __builtin_fmaf(jx, inv, a.half_force) : jx * inv) * r;
     const C uy = (jy * inv) * r;
@@ -634,6 +652,57 @@ __device__ __forceinline__ R ld_row(const R *row, unsigned xb) {
     else return *at(row, xb);
 }
 
+// A stored population as loaded, before decoding (2-byte elements held
+// zero-extended, one per register).
+template <typename S>
+using RawPop = std::conditional_t<sizeof(typename Store<S>::R) == 2, unsigned, typename Store<S>::R>;
+
+// The loads of the nine stored elements cell (x, y) pulls, in the modes
+// without pressure increments (SW, CF, SL, BF).  Interior cells pull through
+// their row addresses, edge cells by the edge rule (which gives the plain
+// pull for an interior cell too).  Both sides only load and decoding follows
+// (raw_pops), so a caller can issue the loads early.  For 2-byte elements a
+// wave holding an edge cell (a row's first or last segment, the first or last
+// row) takes the edge rule in every lane: a uniform branch, one batch of loads
+// and one wait per wave.  For 4- and 8-byte elements only the edge lanes do,
+// and the interior lanes keep their buffer loads; measured faster there (FP32
+// batch of 512 lattices 80,300 against 78,700 MLUPS), slower for the 16-bit
+// forms (FP16 129,600 against 137,500).
+// UNIFORM_EDGE: every lane of a wave holding an edge cell takes the edge rule
+// (the default for 2-byte elements).
+template <typename S, typename C, int MODE, bool UNIFORM_EDGE = sizeof(typename Store<S>::R) == 2>
+__device__ __forceinline__ void pull_raw(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f, int x,
+                                         int y, RawPop<S> (&raw)[9]) {
+    using R = typename Store<S>::R;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    const R *fr = f + (ptrdiff_t)y * W;  // row y, direction 0 (wave-uniform)
+    const unsigned xb = (unsigned)x * (unsigned)sizeof(R);
+    const bool edge = (x == 0) | (x == a.nx - 1) | (y == 0) | (y == a.ny - 1);
+    const bool edge_rule = UNIFORM_EDGE ? __builtin_amdgcn_ballot_w64(edge) != 0 : edge;
+    if (!edge_rule) {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) raw[d] = ld_row(fr + d * P + pull_dy(d) * W + pull_dx(d), xb);
+    } else {
+        pull_edge_cell<S, C, MODE>(a, f, x, y, raw);
+    }
+}
+
+// Working values of the loaded elements of a cell of row y: decoded, plus
+// the moving-wall terms of CF and SL on the top row (bounce-back of d = 7, 8
+// from row ny, pull_edge).
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void raw_pops(const StepArgs<C> &a, const RawPop<S> (&raw)[9], int y, C (&v)[9]) {
+    using R = typename Store<S>::R;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) v[d] = pop_in<S>((R)raw[d], d);
+    if (MODE == CF || MODE == SL) {
+        if (y == a.ny - 1) {
+            v[7] = v[7] - a.corr8;
+            v[8] = v[8] + a.corr9;
+        }
+    }
+}
+
 // The nine populations cell (x, y) pulls, as working values: plain pull in
 // the interior, boundary rule on the edge.
 template <typename S, typename C, int MODE>
@@ -654,34 +723,9 @@ __device__ __forceinline__ void pull_pops(const StepArgs<C> &a, const typename S
             for (int d = 1; d < 9; ++d) v[d] = pull_edge<S, C, MODE, 2>(a, f, d, x, y);
         }
     } else {
-        // Interior cells pull through their row addresses, edge cells by the
-        // edge rule (which gives the plain pull for an interior cell too).
-        // Both sides only load (2-byte elements held zero-extended, one per
-        // register) and decoding follows the join.  For 2-byte elements a
-        // wave holding an edge cell (a row's first or last segment, the first
-        // or last row) takes the edge rule in every lane: a uniform branch,
-        // one batch of loads and one wait per wave.  For 4- and 8-byte
-        // elements only the edge lanes do, and the interior lanes keep their
-        // buffer loads; measured faster there (FP32 batch of 512 lattices
-        // 80,300 against 78,700 MLUPS), slower for the 16-bit forms (FP16
-        // 129,600 against 137,500).
-        using L = std::conditional_t<sizeof(R) == 2, unsigned, R>;
-        L raw[9];
-        const bool edge_rule = sizeof(R) == 2 ? __builtin_amdgcn_ballot_w64(edge) != 0 : edge;
-        if (!edge_rule) {
-#pragma unroll
-            for (int d = 0; d < 9; ++d) raw[d] = ld_row(fr + d * P + pull_dy(d) * W + pull_dx(d), xb);
-        } else {
-            pull_edge_cell<S, C, MODE>(a, f, x, y, raw);
-        }
-#pragma unroll
-        for (int d = 0; d < 9; ++d) v[d] = pop_in<S>((R)raw[d], d);
-        if (MODE == CF || MODE == SL) {  // moving top wall: bounce-back of d = 7, 8 from row ny (pull_edge)
-            if (y == a.ny - 1) {
-                v[7] = v[7] - a.corr8;
-                v[8] = v[8] + a.corr9;
-            }
-        }
+        RawPop<S> raw[9];
+        pull_raw<S, C, MODE>(a, f, x, y, raw);
+        raw_pops<S, C, MODE>(a, raw, y, v);
     }
 }
 
@@ -856,7 +900,10 @@ __device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
     using R = typename Store<S>::R;
     constexpr int ROWS = WAVES * RPT;
     constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
-    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    // wave index made wave-uniform: the region row of a wave then lies in
+    // scalar registers, and so do the buffer descriptors built from it (a
+    // per-lane row would wrap every buffer access in a readfirstlane loop)
+    const int lane = (int)threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int nx = a.nx, ny = a.ny;
     const int ux = tx * 62 - 1 + lane;  // unwrapped region column
     const int uy0 = ty * (ROWS - 2) - 1;  // unwrapped first region row
@@ -923,12 +970,18 @@ __device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
     }
 }
 
+// a.n_blocks tiles; a grid smaller than that walks them (persistent
+// workgroups, grid a multiple of 8 so that every tile of a block lies on the
+// block's XCD in the grouped order).
 template <typename S, typename C, int MODE, int WAVES, int RPT, bool NT>
 __global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
-    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
-    const int ty = t / a.n_tiles_x;
-    pair_tile<S, C, MODE, WAVES, RPT, NT>(a, t - ty * a.n_tiles_x, ty, lds);
+    for (int p = (int)blockIdx.x; p < a.n_blocks; p += (int)gridDim.x) {
+        const int t = tile_of_block(p, a.n_blocks, a.xcd_group);
+        const int ty = t / a.n_tiles_x;
+        pair_tile<S, C, MODE, WAVES, RPT, NT>(a, t - ty * a.n_tiles_x, ty, lds);
+        if (p + (int)gridDim.x < a.n_blocks) __syncthreads();  // the next tile rewrites the LDS
+    }
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,
@@ -953,6 +1006,7 @@ __global__ void __launch_bounds__(256) pf_strip_kernel(StepArgs<C> a) {
         const C a13 = v[1] + v[3], a24 = v[2] + v[4], a57 = v[5] + v[7], a68 = v[6] + v[8];
         const C b13 = v[1] - v[3], b24 = v[2] - v[4], b57 = v[5] - v[7], b68 = v[6] - v[8];
         const C rho = (C)1 + (((v[0] + a13) + a24) + (a57 + a68)) * inv;
+        density_check(a, rho);
         m[(side * 3 + 0) * a.ny + y] = rho;
         m[(side * 3 + 1) * a.ny + y] = (((b13 + b57) - b68) * inv) / rho;
         m[(side * 3 + 2) * a.ny + y] = (((b24 + b57) + b68) * inv) / rho;
@@ -961,6 +1015,7 @@ __global__ void __launch_bounds__(256) pf_strip_kernel(StepArgs<C> a) {
         const C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
         const C ux = f2 - f4 + f6 - f7 - f8 + f9;
         const C uy = f3 - f5 + f6 + f7 - f8 - f9;
+        density_check(a, rho);
         m[(side * 3 + 0) * a.ny + y] = rho;
         m[(side * 3 + 1) * a.ny + y] = ux / rho;
         m[(side * 3 + 2) * a.ny + y] = uy / rho;

@@ -135,6 +135,8 @@ template <typename C> StepArgs<C> make_args(const ffb_lattice *L, int write_macr
     a.step_seed = (unsigned)(L->steps + 1);  // the step this launch computes
     a.n_blocks = a.n_tiles_x * c.n_y;
     a.xcd_group = L->xcd_group;
+    a.check_rho = c.density_checks != 0;
+    a.status = L->status;
     return a;
 }
 
@@ -209,7 +211,9 @@ template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_
     a.n_tiles_x = (a.nx + 61) / 62;
     a.n_blocks = a.n_tiles_x * ((a.ny + ROWS - 3) / (ROWS - 2));
     a.xcd_group = L->pair_xcd_group;
-    const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
+    // persistent grid of L->pair_grid workgroups (rounded up to a multiple of 8), or one per tile
+    const int g = L->pair_grid > 0 ? std::min(a.n_blocks, ((L->pair_grid + 7) / 8) * 8) : a.n_blocks;
+    const dim3 grid((unsigned)g), block(64 * WAVES);
     switch (L->cfg.sim_mode) {
     case FFB_SHEAR_WAVE: hipLaunchKernelGGL((step2_kernel<S, C, SW, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
     case FFB_COUETTE_FLOW: hipLaunchKernelGGL((step2_kernel<S, C, CF, WAVES, RPT, NT>), grid, block, 0, L->stream, a); break;
@@ -447,7 +451,7 @@ void ffb_config_default(ffb_config_t *cfg) {
     cfg->device = 0;
     cfg->body_force = 0.0;
     cfg->storage_rounding = FFB_ROUND_STOCHASTIC;
-    cfg->reserved = 0;
+    cfg->density_checks = 0;
 }
 
 int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
@@ -456,6 +460,8 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     if (cfg->n_x < 1 || cfg->n_y < 1) return fail(FFB_E_INVALID, "ffb_create: n_x and n_y must be >= 1");
     if ((long long)cfg->n_x * cfg->n_y > (1LL << 34))
         return fail(FFB_E_INVALID, "ffb_create: domain larger than 2^34 cells");
+    // row stride 9 * n_x (rounded up) and byte offsets x * 8 are 32-bit values
+    if (cfg->n_x > (1 << 24)) return fail(FFB_E_INVALID, "ffb_create: n_x larger than 2^24");
     if (cfg->sim_mode < FFB_SHEAR_WAVE || cfg->sim_mode > FFB_BODY_FORCE_CHANNEL)
         return fail(FFB_E_INVALID, "ffb_create: unknown sim mode " + std::to_string(cfg->sim_mode));
     if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_E3M12_DEV)
@@ -525,23 +531,24 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
         const int g = rows ? (half ? 2 * tiles_x : (cfg->precision == FFB_PREC_FP64 ? std::max(1, tiles_x / 2) : 0)) : 0;
         L->xcd_group = (int)std::max(0LL, env_ll("FFB_XCD_GROUP", g));
     }
-    // step pairs in one pass (step2_kernel) where they were measured faster:
+    // step pairs in one pass (step2_kernel), measured faster for every form:
     // FP32 4096^2 cavity 192.5 -> 124 us per step, FP64 387.5 -> 218; FP16 and
-    // BF16 deviations 8192^2 382 -> 372 (8-wave regions); the 1-3-12 form, whose
-    // encoding costs more arithmetic, ran slower (394 -> 439) and keeps one pass
-    // per step.  FFB_TWO_STEP=0/1 overrides.
-    const bool pairs_default = cfg->precision != FFB_PREC_E3M12_DEV;
-    L->two_step = env_ll("FFB_TWO_STEP", pairs_default ? 1 : 0) ? 1 : 0;
+    // BF16 deviations 8192^2 380 -> 341-342 (8-wave regions), the 1-3-12 form
+    // 390 -> 371.  FFB_TWO_STEP=0/1 overrides.
+    L->two_step = env_ll("FFB_TWO_STEP", 1) ? 1 : 0;
     // pair tiles per XCD group: FP32 4096^2 at groups of 2/4/8/16/32/67 tiles
     // 147.2/132.5/125.3/123.6/125.3/129.5 us per step, 8192^2 groups of 16/32
     // 493/488; FP64 (default-policy stores) 223.2/220.8/219.6 at 4/16/32
     L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 32));
-    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", L->elem == 2 ? 2 : 1);  // 2: 8 waves x 2 rows
-    // pair stores: default cache policy (the partial lines at the ends of a
-    // 62-cell row, written by two x-neighbouring tiles on one XCD, merge in its
-    // L2; FP32 4096^2 152 -> 133 us per step; FP64 with groups of 32 tiles
-    // 219.6 against 225.2 with nontemporal stores)
+    // pair region of 4-byte and narrower elements: 2 = 8 waves x 2 rows (64 x 16
+    // cells, 62 x 14 outputs), 1 = 16 waves x 2 rows (64 x 32, 62 x 30).  Shape 2
+    // for FP32 (4096^2: 248-254 against 259-268 us per pair; 8192^2 995 against
+    // 1033) and the 16-bit forms (FP16 8192^2 683 against 771), shape 1 for FP32
+    // deviations (8192^2 866 against 943).
+    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", cfg->precision == FFB_PREC_FP32_DEV ? 1 : 2);
     L->pair_nt = (int)env_ll("FFB_PAIR_NT", 0);
+    // pair workgroups: 0 = one per tile, else a persistent grid of this size
+    L->pair_grid = (int)std::max(0LL, env_ll("FFB_PAIR_GRID", 0));
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);
@@ -567,6 +574,8 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     if ((e = hipMalloc(&L->mom_partial, (size_t)L->mom_blocks * 3 * sizeof(double))) != hipSuccess)
         return cleanup_fail(e, "moment partials");
     if ((e = hipMalloc(&L->mom_out, 3 * sizeof(double))) != hipSuccess) return cleanup_fail(e, "moments");
+    if ((e = hipMalloc(&L->status, sizeof(unsigned))) != hipSuccess) return cleanup_fail(e, "status word");
+    if ((e = hipMemset(L->status, 0, sizeof(unsigned))) != hipSuccess) return cleanup_fail(e, "status word");
     if ((e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking)) != hipSuccess) return cleanup_fail(e, "stream");
     if ((e = hipEventCreate(&L->ev_start)) != hipSuccess) return cleanup_fail(e, "event");
     if ((e = hipEventCreate(&L->ev_stop)) != hipSuccess) return cleanup_fail(e, "event");
@@ -585,6 +594,7 @@ int ffb_destroy(ffb_lattice_t *L) {
     if (L->ux_row) (void)hipFree(L->ux_row);
     if (L->mom_partial) (void)hipFree(L->mom_partial);
     if (L->mom_out) (void)hipFree(L->mom_out);
+    if (L->status) (void)hipFree(L->status);
     if (L->ev_start) (void)hipEventDestroy(L->ev_start);
     if (L->ev_stop) (void)hipEventDestroy(L->ev_stop);
     if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -599,6 +609,7 @@ int ffb_init(ffb_lattice_t *L) {
     L->strip_old = 0;
     L->steps = 0;
     L->timing = false;
+    HIPCHK(hipMemsetAsync(L->status, 0, sizeof(unsigned), L->stream));
     int rc = dispatch_init(L);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(L->stream));
@@ -649,6 +660,13 @@ int ffb_set_parameters(ffb_lattice_t *L, double omega, double body_force, double
 int ffb_sync(ffb_lattice_t *L) {
     if (!L) return fail(FFB_E_INVALID, "ffb_sync: null handle");
     HIPCHK(hipStreamSynchronize(L->stream));
+    if (L->cfg.density_checks) {
+        unsigned st = 0;
+        HIPCHK(hipMemcpy(&st, L->status, sizeof(st), hipMemcpyDeviceToHost));
+        // the reference's message (simulation.f90:231)
+        if (st & 1u) return fail(FFB_E_DENSITY, "error: density is zero in collision/streaming step (rho_val <= 0)");
+        if (st & 2u) return fail(FFB_E_DENSITY, "error: density is NaN in collision/streaming step");
+    }
     return FFB_OK;
 }
 

@@ -1,19 +1,24 @@
 // ffb_batch.hip -- batched form: many independent lattices of uneven size,
-// each with its own omega and boundary kind, advanced together by ONE
-// persistent kernel launch per step; the two launches of a ping-pong pair
-// are captured once in a hipGraph and replayed.  A project-written
-// autotuner picks the workgroup size and the persistent grid.  No reference
-// counterpart (the reference runs one domain per process).
+// each with its own omega and boundary kind, advanced together.  Consecutive
+// steps run as step pairs, two steps per launch of one persistent kernel over
+// the tiles of every lattice (one launch per step is the option, and runs an
+// odd remainder); the launches of a ping-pong cycle are captured once in a
+// hipGraph and replayed.  A project-written autotuner picks the launch shape
+// of the kernel the batch runs by timing the candidates on the batch.  No
+// reference counterpart (the reference runs one domain per process).
 //
 // Layout: every lattice keeps the single-domain row-blocked layout (per row
 // the nine direction rows, each n_x rounded up to 64 elements; no halo) inside
-// two pools (ping-pong); a
-// descriptor table holds each lattice's offsets and parameters.  Work is cut
-// into row segments of 64 cells (one wave each, so every load of a wave is
-// one aligned stretch of one row); segs[t] names the lattice and row of segment t,
-// and the segments are ordered by boundary kind so that the waves resident
-// at one time run the same code.  A workgroup of TW threads takes TW/64
-// consecutive segments per iteration of its persistent loop.
+// two pools (ping-pong); a descriptor table holds each lattice's offsets and
+// parameters.  Step pairs: a tile table names (lattice, tile) for every
+// 62 x 14 (or 62 x 30; FP64 62 x 6) output tile, lattice by lattice, grouped by
+// kind.  One launch per step: work is cut into row segments of 64 cells (one
+// wave each, so every load of a wave is one aligned stretch of one row);
+// segs[t] names the lattice and row of segment t, and a workgroup of TW
+// threads takes TW/64 consecutive segments per iteration of its persistent
+// loop.  The tables are built once, at creation; a change of tuning rewrites
+// the tile table in place (after dropping the graph that refers to it) and the
+// graph is captured again before the next step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -117,6 +122,8 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     a.n_tiles_x = 0;
     a.n_blocks = 0;
     a.xcd_group = 0;
+    a.check_rho = 0;
+    a.status = nullptr;
     a.force = L.force;
     a.stochastic = 0;
     a.step_seed = 0;
@@ -137,9 +144,9 @@ __device__ __forceinline__ StepArgs<C> lattice_args(const BatchArgs<C> &b, const
     return a;
 }
 
-// Step counter of the stochastic rounding: advanced after every step launch
-// (inside the captured step pairs too), read by the next launch.
-__global__ void batch_count_kernel(unsigned *ctr, int reset) { ctr[0] = reset ? 0u : ctr[0] + 1u; }
+// Step counter of the stochastic rounding: advanced by the steps of every
+// step launch (inside the captured graph too), read by the next launch.
+__global__ void batch_count_kernel(unsigned *ctr, int reset, unsigned inc) { ctr[0] = reset ? 0u : ctr[0] + inc; }
 
 // One step of every lattice: a persistent grid walks the row segments, one
 // wave per segment.  The 16-bit forms are compiled for 8 waves per SIMD (the
@@ -173,22 +180,25 @@ __global__ void __launch_bounds__(TW)
     }
 }
 
-// Step pairs of every lattice (FP32, FP32 deviations, FP64): one workgroup
-// per tile of one lattice (pair_tile: step 1 on the tile and its one-cell ring
-// into LDS, step 2 of the tile from LDS), the tiles numbered lattice by
-// lattice (grouped by kind) and mapped to workgroups in XCD groups of 4.
+// Step pairs of every lattice: workgroups walk the tile table (pair_tile:
+// step 1 on the tile and its one-cell ring into LDS, step 2 of the tile from
+// LDS), a persistent grid (a multiple of 8 workgroups) or one workgroup per
+// tile, the tiles mapped to workgroups in XCD groups of G (tile_of_block).
 template <typename S, typename C, int WAVES, int RPT, bool NT>
 __global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, const PTile *tiles, int n, int G) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
-    const int t = tile_of_block((int)blockIdx.x, n, G);
-    const PTile pt = load_uniform(tiles + t);
-    const BatchLat<C> L = load_uniform(b.lats + pt.lat);
-    const StepArgs<C> a = lattice_args<S, C>(b, L);
-    switch (L.kind) {  // uniform per workgroup
-    case SW: pair_tile<S, C, SW, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-    case CF: pair_tile<S, C, CF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-    case SL: pair_tile<S, C, SL, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-    default: pair_tile<S, C, BF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+    for (int p = (int)blockIdx.x; p < n; p += (int)gridDim.x) {
+        const int t = tile_of_block(p, n, G);
+        const PTile pt = load_uniform(tiles + t);
+        const BatchLat<C> L = load_uniform(b.lats + pt.lat);
+        StepArgs<C> a = lattice_args<S, C>(b, L);
+        switch (L.kind) {  // uniform per workgroup
+        case SW: pair_tile<S, C, SW, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+        case CF: pair_tile<S, C, CF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+        case SL: pair_tile<S, C, SL, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+        default: pair_tile<S, C, BF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+        }
+        if (p + (int)gridDim.x < n) __syncthreads();  // the next tile rewrites the LDS
     }
 }
 
@@ -249,23 +259,24 @@ struct ffb_batch {
     void *lats = nullptr;     // BatchLat<C>[count]
     void *segs = nullptr;     // Seg per row segment: (lattice, row)
     int n_tiles = 0;
-    bool pairs = false;       // step pairs in one launch (batch_pair_kernel; 4- and 8-byte forms)
-    int pair_group = 4;       // consecutive pair tiles per XCD (tile_of_block)
-    int pair_shape = 2;       // 4-byte forms: 1 = 16 waves x 2 rows (62 x 30 tiles), 2 = 8 waves x 2 rows (62 x 14)
-    void *ptiles = nullptr;   // PTile per step-pair tile
-    int n_ptiles = 0;
-    ffb_batch_tuning_t tuning{256, 0};
+    bool pairs = false;       // step pairs in one launch (batch_pair_kernel)
+    void *ptiles = nullptr;   // PTile per step-pair tile (capacity ptile_cap)
+    int n_ptiles = 0, ptile_cap = 0;
+    int ptile_shape = 0;      // pair_shape the tile table holds
+    int cus = 0;              // compute units of the device
+    ffb_batch_tuning_t tuning{};
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timing = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool graph_pairs = false; // B->pairs when the graph was captured
     long long steps = 0;
 };
 
 namespace {
 
-// workgroups of the launch: one per TW/64 segments, or the persistent grid
+// workgroups of a single-step launch: one per TW/64 segments, or the persistent grid
 int grid_for(const ffb_batch *B) {
     const int waves = B->tuning.tile_w / 64;
     const int all = (B->n_tiles + waves - 1) / waves;
@@ -273,17 +284,38 @@ int grid_for(const ffb_batch *B) {
     return (g <= 0 || g > all) ? all : g;
 }
 
-// (Re)builds the descriptor and tile tables for the current tile width.
+// output rows of a step-pair tile for a region shape: 62 x 14 (8 waves x 2
+// rows), 62 x 30 (16 waves x 2 rows); FP64 62 x 6 (8 waves x 1 row)
+int pair_out_rows(const ffb_batch *B, int shape) { return B->elem == 8 ? 6 : (shape == 1 ? 30 : 14); }
+
+// lattices in table order: grouped by kind (stable), so resident waves share a code path
+std::vector<int> lattice_order(const ffb_batch *B) {
+    std::vector<int> order(B->desc.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return B->desc[(size_t)a].kind < B->desc[(size_t)b].kind; });
+    return order;
+}
+
+// Step-pair tiles of a region shape, lattice by lattice in table order.
+std::vector<PTile> pair_tiles(const ffb_batch *B, int shape) {
+    std::vector<PTile> ptiles;
+    const int out_rows = pair_out_rows(B, shape);
+    for (int i : lattice_order(B)) {
+        const ffb_batch_lattice_t &d = B->desc[(size_t)i];
+        for (int ty = 0; ty < (d.n_y + out_rows - 1) / out_rows; ++ty)
+            for (int tx = 0; tx < (d.n_x + 61) / 62; ++tx) ptiles.push_back(PTile{i, tx, ty, 0});
+    }
+    return ptiles;
+}
+
+// Descriptor, segment and tile tables, built once at creation: the tile
+// table's allocation holds the larger of the two shapes' tile lists.
 template <typename C> int upload_tables(ffb_batch *B) {
     const int n = (int)B->desc.size();
     std::vector<BatchLat<C>> lats((size_t)n);
     std::vector<Seg> tiles;
-    // lattices grouped by kind (stable), so resident waves share their code path
-    std::vector<int> order((size_t)n);
-    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return B->desc[(size_t)a].kind < B->desc[(size_t)b].kind; });
-    for (int i : order) {
+    for (int i : lattice_order(B)) {
         const ffb_batch_lattice_t &d = B->desc[(size_t)i];
         BatchLat<C> &L = lats[(size_t)i];
         L.off = B->off[(size_t)i];
@@ -325,28 +357,37 @@ template <typename C> int upload_tables(ffb_batch *B) {
         }
         L.half_force = (C)0.5 * L.force;
     }
-    // step-pair tiles: 62 x 14 cells (62 x 30 with the 16-wave shape; FP64 62 x 6), lattice by lattice
-    std::vector<PTile> ptiles;
-    const int out_rows = B->elem == 8 ? 6 : (B->pair_shape == 2 ? 14 : 30);
-    for (int i : order) {
-        const ffb_batch_lattice_t &d = B->desc[(size_t)i];
-        for (int ty = 0; ty < (d.n_y + out_rows - 1) / out_rows; ++ty)
-            for (int tx = 0; tx < (d.n_x + 61) / 62; ++tx) ptiles.push_back(PTile{i, tx, ty, 0});
-    }
-    if (B->lats) (void)hipFree(B->lats);
-    if (B->segs) (void)hipFree(B->segs);
-    if (B->ptiles) (void)hipFree(B->ptiles);
-    B->lats = nullptr;
-    B->segs = nullptr;
-    B->ptiles = nullptr;
-    BHIPCHK(hipMalloc(&B->ptiles, ptiles.size() * sizeof(PTile)));
-    BHIPCHK(hipMemcpy(B->ptiles, ptiles.data(), ptiles.size() * sizeof(PTile), hipMemcpyHostToDevice));
-    B->n_ptiles = (int)ptiles.size();
+    B->ptile_cap = (int)std::max(pair_tiles(B, 1).size(), pair_tiles(B, 2).size());
+    BHIPCHK(hipMalloc(&B->ptiles, (size_t)B->ptile_cap * sizeof(PTile)));
     BHIPCHK(hipMalloc(&B->lats, lats.size() * sizeof(BatchLat<C>)));
     BHIPCHK(hipMalloc(&B->segs, tiles.size() * sizeof(Seg)));
     BHIPCHK(hipMemcpy(B->lats, lats.data(), lats.size() * sizeof(BatchLat<C>), hipMemcpyHostToDevice));
     BHIPCHK(hipMemcpy(B->segs, tiles.data(), tiles.size() * sizeof(Seg), hipMemcpyHostToDevice));
     B->n_tiles = (int)tiles.size();
+    B->ptile_shape = 0;
+    return FFB_OK;
+}
+
+int drop_graph(ffb_batch *B) {
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->graph_exec = nullptr;
+    B->graph = nullptr;
+    return FFB_OK;
+}
+
+// Writes the tile list of B->tuning.pair_shape into the tile table (in place,
+// synchronously; the graph, whose launches read the table, is dropped first).
+int write_pair_tiles(ffb_batch *B) {
+    const int shape = B->tuning.pair_shape;
+    if (B->ptile_shape == shape) return FFB_OK;
+    drop_graph(B);
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    const std::vector<PTile> pt = pair_tiles(B, shape);
+    if ((int)pt.size() > B->ptile_cap) return fail(FFB_E_STATE, "ffb_batch: tile table too small");
+    BHIPCHK(hipMemcpy(B->ptiles, pt.data(), pt.size() * sizeof(PTile), hipMemcpyHostToDevice));
+    B->n_ptiles = (int)pt.size();
+    B->ptile_shape = shape;
     return FFB_OK;
 }
 
@@ -384,6 +425,12 @@ template <typename S, typename C> void launch_step_c(const ffb_batch *B, int par
 
 bool is_16bit(int p) { return p == FFB_PREC_FP16_DEV || p == FFB_PREC_BF16_DEV || p == FFB_PREC_E3M12_DEV; }
 
+void count_steps(const ffb_batch *B, unsigned n) {
+    if (is_16bit(B->precision))  // the next launch computes the next step
+        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 0,
+                           n);
+}
+
 void launch_step(const ffb_batch *B, int parity, int write_macro) {
     switch (B->precision) {
     case FFB_PREC_FP64: launch_step_c<double, double>(B, parity, write_macro); break;
@@ -393,48 +440,44 @@ void launch_step(const ffb_batch *B, int parity, int write_macro) {
     case FFB_PREC_E3M12_DEV: launch_step_c<e3m12, float>(B, parity, write_macro); break;
     default: launch_step_c<float, float>(B, parity, write_macro); break;
     }
-    if (is_16bit(B->precision))  // the next launch computes the next step
-        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 0);
+    count_steps(B, 1u);
 }
 
 // One launch of step pairs (two steps) of every lattice.
 template <typename S, typename C, int WAVES, int RPT, bool NT>
 void launch_pair_c(const ffb_batch *B, int parity, int write_macro) {
     const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
-    hipLaunchKernelGGL((batch_pair_kernel<S, C, WAVES, RPT, NT>), dim3((unsigned)B->n_ptiles), dim3(64 * WAVES), 0,
-                       B->stream, b, static_cast<const PTile *>(B->ptiles), B->n_ptiles, B->pair_group);
+    const int n = B->n_ptiles, pg = B->tuning.pair_grid;
+    const int grid = pg > 0 ? std::min(n, ((pg + 7) / 8) * 8) : n;
+    hipLaunchKernelGGL((batch_pair_kernel<S, C, WAVES, RPT, NT>), dim3((unsigned)grid), dim3(64 * WAVES), 0, B->stream,
+                       b, static_cast<const PTile *>(B->ptiles), n, B->tuning.pair_group);
 }
 
-// Store policy as the single domain's pairs (default policy for 4-byte
-// elements, nontemporal for FP64); shapes 8 waves x 2 rows (or 16 x 2) for
-// 4-byte elements, 8 waves x 1 row for FP64.
+template <typename S, typename C> void launch_pair_shape(const ffb_batch *B, int parity, int write_macro) {
+    if (B->tuning.pair_shape == 1) launch_pair_c<S, C, 16, 2, false>(B, parity, write_macro);
+    else launch_pair_c<S, C, 8, 2, false>(B, parity, write_macro);
+}
+
+// Default-policy stores as the single domain's pairs (FP64: nontemporal);
+// regions 8 waves x 2 rows or 16 waves x 2 rows, FP64 8 waves x 1 row.
 void launch_pair(const ffb_batch *B, int parity, int write_macro) {
     switch (B->precision) {
     case FFB_PREC_FP64: launch_pair_c<double, double, 8, 1, true>(B, parity, write_macro); break;
-    case FFB_PREC_FP32_DEV:
-        if (B->pair_shape == 2) launch_pair_c<f32dev, float, 8, 2, false>(B, parity, write_macro);
-        else launch_pair_c<f32dev, float, 16, 2, false>(B, parity, write_macro);
-        break;
-    default:
-        if (B->pair_shape == 2) launch_pair_c<float, float, 8, 2, false>(B, parity, write_macro);
-        else launch_pair_c<float, float, 16, 2, false>(B, parity, write_macro);
-        break;
+    case FFB_PREC_FP32_DEV: launch_pair_shape<f32dev, float>(B, parity, write_macro); break;
+    case FFB_PREC_FP16_DEV: launch_pair_shape<__half, float>(B, parity, write_macro); break;
+    case FFB_PREC_BF16_DEV: launch_pair_shape<__hip_bfloat16, float>(B, parity, write_macro); break;
+    case FFB_PREC_E3M12_DEV: launch_pair_shape<e3m12, float>(B, parity, write_macro); break;
+    default: launch_pair_shape<float, float>(B, parity, write_macro); break;
     }
+    count_steps(B, 2u);
 }
 
-int drop_graph(ffb_batch *B) {
-    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->graph_exec = nullptr;
-    B->graph = nullptr;
-    return FFB_OK;
-}
-
-// Captures the two launches of one ping-pong pair (parity 0 then 1).
+// Captures the launches of one ping-pong cycle (parity 0 then 1): two pair
+// launches (four steps) or two single steps.
 int build_graph(ffb_batch *B) {
     drop_graph(B);
     BHIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    if (B->pairs) {  // two pair launches: four steps
+    if (B->pairs) {
         launch_pair(B, 0, 0);
         launch_pair(B, 1, 0);
     } else {
@@ -446,19 +489,20 @@ int build_graph(ffb_batch *B) {
     if (e != hipSuccess) return fail(FFB_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     B->graph = g;
     BHIPCHK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+    B->graph_pairs = B->pairs;
     return FFB_OK;
+}
+
+// The graph for the current tuning and pass mode, captured if missing or stale.
+int ensure_graph(ffb_batch *B) {
+    if (B->graph_exec && B->graph_pairs == B->pairs) return FFB_OK;
+    return build_graph(B);
 }
 
 template <typename S, typename C> int init_c(ffb_batch *B) {
     BatchArgs<C> b = batch_args<C>(B, 1, 0);  // dst = pool[0]
-    const dim3 grid((unsigned)((B->n_tiles + B->tuning.tile_w / 64 - 1) / (B->tuning.tile_w / 64)));
-    switch (B->tuning.tile_w) {
-    case 64: hipLaunchKernelGGL((batch_init_kernel<S, C, 64>), grid, dim3(64), 0, B->stream, b); break;
-    case 128: hipLaunchKernelGGL((batch_init_kernel<S, C, 128>), grid, dim3(128), 0, B->stream, b); break;
-    case 512: hipLaunchKernelGGL((batch_init_kernel<S, C, 512>), grid, dim3(512), 0, B->stream, b); break;
-    case 1024: hipLaunchKernelGGL((batch_init_kernel<S, C, 1024>), grid, dim3(1024), 0, B->stream, b); break;
-    default: hipLaunchKernelGGL((batch_init_kernel<S, C, 256>), grid, dim3(256), 0, B->stream, b); break;
-    }
+    const dim3 grid((unsigned)((B->n_tiles + 3) / 4));
+    hipLaunchKernelGGL((batch_init_kernel<S, C, 256>), grid, dim3(256), 0, B->stream, b);
     BHIPCHK(hipGetLastError());
     BHIPCHK(hipStreamSynchronize(B->stream));
     return FFB_OK;
@@ -466,7 +510,8 @@ template <typename S, typename C> int init_c(ffb_batch *B) {
 
 template <typename S, typename C> int init_any(ffb_batch *B) {
     if (is_16bit(B->precision)) {
-        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 1);
+        hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(1), 0, B->stream, static_cast<unsigned *>(B->step_ctr), 1,
+                           0u);
         BHIPCHK(hipGetLastError());
     }
     return init_c<S, C>(B);
@@ -475,12 +520,34 @@ template <typename S, typename C> int init_any(ffb_batch *B) {
 int apply_tuning(ffb_batch *B, const ffb_batch_tuning_t &t) {
     if (t.tile_w != 64 && t.tile_w != 128 && t.tile_w != 256 && t.tile_w != 512 && t.tile_w != 1024)
         return fail(FFB_E_INVALID, "ffb_batch: tile_w must be 64, 128, 256, 512 or 1024");
-    if (t.grid_blocks < 0) return fail(FFB_E_INVALID, "ffb_batch: grid_blocks must be >= 0");
+    if (t.grid_blocks < 0 || t.pair_grid < 0 || t.pair_group < 0)
+        return fail(FFB_E_INVALID, "ffb_batch: grid_blocks, pair_grid and pair_group must be >= 0");
+    if (t.pair_shape != 1 && t.pair_shape != 2) return fail(FFB_E_INVALID, "ffb_batch: pair_shape must be 1 or 2");
+    // the graph refers to the tuning (and to the tile table): dropped first,
+    // captured again by the next ffb_batch_step
+    drop_graph(B);
     BHIPCHK(hipStreamSynchronize(B->stream));
     B->tuning = t;
-    int rc = B->precision == FFB_PREC_FP64 ? upload_tables<double>(B) : upload_tables<float>(B);
-    if (rc) return rc;
-    return build_graph(B);
+    if (B->elem == 8) B->tuning.pair_shape = 2;  // FP64 has one region shape (its tiles do not depend on it)
+    return write_pair_tiles(B);
+}
+
+// Tuning before the autotuner runs: 256-thread workgroups, one per four row
+// segments, for single steps; step pairs in 62 x 14 tiles (FP64 62 x 6), XCD
+// groups of 32 tiles (FP64 4), one workgroup per tile.  FFB_BATCH_PAIR_GROUP,
+// FFB_BATCH_PAIR_SHAPE and FFB_BATCH_PAIR_GRID override the pair settings.
+ffb_batch_tuning_t ffb_batch_default_tuning(const ffb_batch *B) {
+    auto env = [](const char *name, int dflt) {
+        const char *v = std::getenv(name);
+        return v && *v ? std::atoi(v) : dflt;
+    };
+    ffb_batch_tuning_t t{};
+    t.tile_w = 256;
+    t.grid_blocks = 0;
+    t.pair_shape = env("FFB_BATCH_PAIR_SHAPE", 2) == 1 ? 1 : 2;
+    t.pair_group = std::max(0, env("FFB_BATCH_PAIR_GROUP", B->elem >= 8 ? 4 : 32));
+    t.pair_grid = std::max(0, env("FFB_BATCH_PAIR_GRID", 0));
+    return t;
 }
 
 template <typename C> int read_macros_c(ffb_batch *B, int i, void *rho, void *ux, void *uy) {
@@ -529,20 +596,14 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     B->device = device;
     B->elem = precision == FFB_PREC_FP64 ? 8 : (is_16bit(precision) ? 2 : 4);
     B->celem = precision == FFB_PREC_FP64 ? 8 : 4;
-    {   // step pairs for the 4- and 8-byte forms, measured faster on 512 mixed
-        // lattices (FP32 82,000 -> 107,100 MLUPS, FP64 39,100 -> 64,500, FP32
-        // deviations 83,400 -> 113,900); FFB_BATCH_PAIRS=0 selects one launch
-        // per step; the 16-bit forms always take one launch per step
+    // step pairs in one launch for the 4- and 8-byte forms, one launch per step
+    // for the 16-bit forms (512 mixed lattices, autotuned, MLUPS: FP32 121,000
+    // in pairs against 83,200; FP32 deviations 127,000 / 83,700; FP64 63,000 /
+    // 39,100; FP16 127,000 in pairs against 144,100, BF16 129,000 / 141,900,
+    // 1-3-12 120,500 / 138,200); FFB_BATCH_PAIRS=0/1 at creation overrides
+    {
         const char *e = std::getenv("FFB_BATCH_PAIRS");
-        B->pairs = B->elem >= 4 && (e && *e ? std::atoi(e) != 0 : true);
-        const char *gs = std::getenv("FFB_BATCH_PAIR_GROUP");
-        // tiles per XCD group and tile shape, measured on 512 mixed lattices
-        // (FP32, MLUPS): 62 x 30 tiles 106,400 / 106,700 / 106,600 at groups of
-        // 4 / 16 / 32; 62 x 14 tiles (8 waves) 110,400 / 111,400 / 112,100 --
-        // the lower tiles waste less at the lattices' top rows; FP64 best at 4
-        B->pair_group = gs && *gs ? std::max(0, std::atoi(gs)) : (B->elem >= 8 ? 4 : 32);
-        const char *ps = std::getenv("FFB_BATCH_PAIR_SHAPE");
-        B->pair_shape = ps && *ps ? (std::atoi(ps) == 1 ? 1 : 2) : 2;
+        B->pairs = e && *e ? std::atoi(e) != 0 : B->elem >= 4;
     }
     B->desc.assign(lattices, lattices + count);
     long long off = 0, moff = 0, poff = 0;
@@ -604,8 +665,10 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
         if ((e = hipMemcpy(B->prof, prof.data(), prof.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "profiles");
     }
-    ffb_batch_tuning_t t{256, 0};
-    int rc = apply_tuning(B, t);
+    if ((e = hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return bail(e, "device attributes");
+    int rc = B->precision == FFB_PREC_FP64 ? upload_tables<double>(B) : upload_tables<float>(B);
+    if (!rc) rc = apply_tuning(B, ffb_batch_default_tuning(B));
     if (rc) {
         ffb_batch_destroy(B);
         return rc;
@@ -661,6 +724,10 @@ int ffb_batch_step(ffb_batch_t *B, int32_t n_steps, int32_t write_macro_last) {
     int left = n_steps;
     // the last step runs outside the graph when it writes the macro fields
     const int tail = write_macro_last ? 1 : 0;
+    if (left - tail >= (B->pairs ? 4 : 2)) {
+        int rc = ensure_graph(B);
+        if (rc) return rc;
+    }
     if (B->pairs) {
         // pair launches (two steps each); the captured graph holds two of them
         // (parity 0 then 1, four steps)
@@ -741,54 +808,75 @@ int ffb_batch_set_tuning(ffb_batch_t *B, const ffb_batch_tuning_t *t) {
 int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *best_out, double *best_ms) {
     if (!B) return fail(FFB_E_INVALID, "ffb_batch_autotune: null handle");
     if (trial_steps < 2) trial_steps = 2;
-    trial_steps += trial_steps & 1;  // whole ping-pong pairs
+    trial_steps += trial_steps & 1;  // whole ping-pong cycles of single steps; pairs: trial_steps / 2 launches
     BHIPCHK(hipSetDevice(B->device));
-    int cus = 0;
-    BHIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device));
-    const int tws[] = {64, 128, 256, 512, 1024};
-    // workgroups per CU of the persistent grid (0: one tile per workgroup, no
-    // persistence).  7, 14 and 28 hold 7 waves per SIMD at 256, 128 and 64
-    // threads, 6 and 12 six (a memory-bound step can run faster below full
-    // occupancy); grids of several resident rounds (up to 384 workgroups per
-    // CU) shorten each wave's persistent loop while still sparing most
-    // workgroup launches: measured on 512 lattices, the 16-bit forms ran
-    // fastest with 96 workgroups of 256 threads per CU, FP32 without
-    // persistence.
-    const int per_cu[] = {0, 2, 4, 6, 7, 8, 12, 14, 16, 28, 48, 96, 192, 384};
-    ffb_batch_tuning_t best{256, 0};
-    double best_t = 1e30;
-    // the candidates tune the single-step kernel (with step pairs it runs the
-    // odd remainders); the trials time its two-step graph
-    struct PairsOff {
-        ffb_batch *b;
-        bool saved;
-        ~PairsOff() { b->pairs = saved; }
-    } pairs_off{B, B->pairs};
-    B->pairs = false;
+    // Candidates of the kernel the batch runs for consecutive steps, each timed
+    // by direct launches (no graph is captured until the winner is known):
+    //   step pairs: region shape (62 x 14 or 62 x 30 tiles; FP64 one shape) x
+    //     XCD group (4 .. 64 tiles) x grid (one workgroup per tile, or a
+    //     persistent grid of 1, 2 or 4 rounds of the workgroups a CU holds:
+    //     4 of the 8-wave shape, 2 of the 16-wave one)
+    //   one launch per step: workgroup size (64 .. 1024 threads) x persistent
+    //     grid (0: one workgroup per tile; 2 .. 384 workgroups per CU)
+    std::vector<ffb_batch_tuning_t> cands;
+    const ffb_batch_tuning_t base = B->tuning;
+    if (B->pairs) {
+        const int shapes[] = {2, 1};
+        const int groups[] = {4, 8, 16, 32, 64};
+        for (int sh : shapes) {
+            if (B->elem == 8 && sh == 1) continue;
+            const int resident = (sh == 1 ? 2 : 4) * B->cus;
+            const int grids[] = {0, resident, 2 * resident, 4 * resident};
+            for (int g : groups)
+                for (int gr : grids) {
+                    ffb_batch_tuning_t t = base;
+                    t.pair_shape = sh;
+                    t.pair_group = g;
+                    t.pair_grid = gr;
+                    cands.push_back(t);
+                }
+        }
+    } else {
+        const int tws[] = {64, 128, 256, 512, 1024};
+        const int per_cu[] = {0, 2, 4, 6, 7, 8, 12, 14, 16, 28, 48, 96, 192, 384};
+        for (int tw : tws)
+            for (int pc : per_cu) {
+                ffb_batch_tuning_t t = base;
+                t.tile_w = tw;
+                t.grid_blocks = pc ? pc * B->cus : 0;
+                cands.push_back(t);
+            }
+    }
     int rc = ffb_batch_init(B);
     if (rc) return rc;
-    for (int tw : tws) {
-        for (int pc : per_cu) {
-            ffb_batch_tuning_t t{tw, pc ? pc * cus : 0};
-            if ((rc = apply_tuning(B, t))) return rc;
-            // warm-up pair, then the timed trial
-            BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
-            BHIPCHK(hipEventRecord(B->ev_start, B->stream));
-            for (int k = 0; k < trial_steps / 2; ++k) BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
-            BHIPCHK(hipEventRecord(B->ev_stop, B->stream));
-            BHIPCHK(hipEventSynchronize(B->ev_stop));
-            float ms = 0.f;
-            BHIPCHK(hipEventElapsedTime(&ms, B->ev_start, B->ev_stop));
-            if (ms < best_t) {
-                best_t = ms;
-                best = t;
-            }
+    ffb_batch_tuning_t best = base;
+    double best_t = 1e30;
+    const int launches = B->pairs ? trial_steps / 2 : trial_steps;
+    for (const ffb_batch_tuning_t &t : cands) {
+        if ((rc = apply_tuning(B, t))) return rc;
+        // one warm-up launch, then the timed ones, alternating the pools
+        auto one = [&](int k) {
+            if (B->pairs) launch_pair(B, (B->cur + k) & 1, 0);
+            else launch_step(B, (B->cur + k) & 1, 0);
+        };
+        one(0);
+        BHIPCHK(hipEventRecord(B->ev_start, B->stream));
+        for (int k = 1; k <= launches; ++k) one(k);
+        BHIPCHK(hipGetLastError());
+        BHIPCHK(hipEventRecord(B->ev_stop, B->stream));
+        BHIPCHK(hipEventSynchronize(B->ev_stop));
+        B->cur = (B->cur + launches + 1) & 1;
+        float ms = 0.f;
+        BHIPCHK(hipEventElapsedTime(&ms, B->ev_start, B->ev_stop));
+        if (ms < best_t) {
+            best_t = ms;
+            best = t;
         }
     }
-    B->pairs = pairs_off.saved;
     if ((rc = apply_tuning(B, best))) return rc;
     if ((rc = ffb_batch_init(B))) return rc;  // trials advanced the state: start again
-    if (best_out) *best_out = best;
+    B->timing = false;
+    if (best_out) *best_out = B->tuning;
     if (best_ms) *best_ms = best_t / trial_steps;
     return FFB_OK;
 }

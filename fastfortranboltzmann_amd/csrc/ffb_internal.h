@@ -37,6 +37,7 @@ struct ffb_lattice {
     void *ux_row = nullptr;  // shear-wave initial profile
     double *mom_partial = nullptr;
     double *mom_out = nullptr;
+    unsigned *status = nullptr;  // density-check flags (StepArgs::status)
     int mom_blocks = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -49,6 +50,7 @@ struct ffb_lattice {
     int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
     int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows
     int pair_nt = 1;         // pair stores: 1 nontemporal, 0 default cache policy
+    int pair_grid = 0;       // pair workgroups: 0 = one per tile, else a persistent grid
     long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
     long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };

@@ -15,13 +15,14 @@ module ffb_c
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
     public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV, &
         FFB_PREC_E3M12_DEV
-    public :: FFB_ROUND_NEAREST, FFB_ROUND_STOCHASTIC
+    public :: FFB_ROUND_NEAREST, FFB_ROUND_STOCHASTIC, FFB_E_DENSITY
 
     integer(c_int32_t), parameter :: FFB_SHEAR_WAVE = 1, FFB_COUETTE_FLOW = 2, &
         FFB_POISEUILLE_FLOW = 3, FFB_SLIDING_LID = 4
     integer(c_int32_t), parameter :: FFB_PREC_FP32 = 0, FFB_PREC_FP64 = 1, &
         FFB_PREC_FP16_DEV = 2, FFB_PREC_BF16_DEV = 3, FFB_PREC_FP32_DEV = 4, FFB_PREC_E3M12_DEV = 5
     integer(c_int32_t), parameter :: FFB_ROUND_NEAREST = 0, FFB_ROUND_STOCHASTIC = 1
+    integer(c_int), parameter :: FFB_E_DENSITY = 4
 
     ! ffb_config_t, field for field
     type, bind(c) :: ffb_config_t
@@ -41,7 +42,7 @@ module ffb_c
         integer(c_int32_t) :: device
         real(c_double) :: body_force
         integer(c_int32_t) :: storage_rounding
-        integer(c_int32_t) :: reserved
+        integer(c_int32_t) :: density_checks
     end type ffb_config_t
 
     interface

@@ -25,6 +25,7 @@ program ffb_host
     real(c_float), allocatable, target :: rho32(:,:), ux32(:,:), uy32(:,:)
     logical :: fp64
     integer(int32) :: step, next_stop, chunk
+    integer(c_int) :: rc
     integer(int64) :: clock_start, clock_end, clock_rate
     real(real64) :: kernel_seconds, elapsed_seconds, other_seconds, ms, mlups, mass, mx, my, speed
     character(len=:), allocatable :: out_dir
@@ -46,6 +47,7 @@ program ffb_host
     cfg%rho_in = s%rho_in
     cfg%rho_out = s%rho_out
     cfg%storage_scale_log2 = s%storage_scale_log2
+    cfg%density_checks = merge(1_c_int32_t, 0_c_int32_t, s%density_checks)
     select case (s%storage_rounding)
     case ("stochastic")
         cfg%storage_rounding = FFB_ROUND_STOCHASTIC
@@ -116,7 +118,10 @@ program ffb_host
         call ffb_check(ffb_step(lat, int(chunk, c_int32_t), &
             merge(1_c_int32_t, 0_c_int32_t, s%export_macros .and. should_export_step(next_stop))), &
             "ffb_step")
-        call ffb_check(ffb_sync(lat), "ffb_sync")
+        rc = ffb_sync(lat)
+        ! FFB_DENSITY_CHECKS: the reference's own stop (simulation.f90:229-233)
+        if (rc == FFB_E_DENSITY) error stop ffb_error_message()
+        call ffb_check(rc, "ffb_sync")
         call ffb_check(ffb_elapsed_ms(lat, ms), "ffb_elapsed_ms")
         kernel_seconds = kernel_seconds + ms * 1.0e-3_real64
         step = next_stop

@@ -34,6 +34,7 @@ module ffb_settings
         character(len=:), allocatable :: precision
         integer(int32) :: storage_scale_log2
         character(len=:), allocatable :: storage_rounding
+        logical :: density_checks
     end type run_settings_t
 
 contains
@@ -91,6 +92,12 @@ contains
 #endif
         s%storage_scale_log2 = STORAGE_SCALE_DEFAULT
         s%storage_rounding = "stochastic"
+        ! FFB_DENSITY_CHECKS: stop at a density rho <= 0 (simulation.f90:229-233)
+#ifdef FFB_DENSITY_CHECKS
+        s%density_checks = .true.
+#else
+        s%density_checks = .false.
+#endif
 
         call env_int("FFB_N_X", s%n_x)
         call env_int("FFB_N_Y", s%n_y)
@@ -113,6 +120,7 @@ contains
         call env_string("FFB_PRECISION", s%precision)
         call env_int("FFB_STORAGE_SCALE_LOG2", s%storage_scale_log2)
         call env_string("FFB_STORAGE_ROUNDING", s%storage_rounding)
+        call env_logical("FFB_DENSITY_CHECKS", s%density_checks)
 
         if (s%n_x < 1 .or. s%n_y < 1) error stop "error: N_X and N_Y must be positive"
         if (s%n_steps < 0) error stop "error: N_STEPS must not be negative"

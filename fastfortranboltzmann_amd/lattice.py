@@ -59,6 +59,7 @@ class Settings:
     device: int = 0
     BODY_FORCE: float = 0.0          # body-force channel (mode 5) only
     storage_rounding: str = "stochastic"  # 16-bit forms: "stochastic" or "nearest"
+    DENSITY_CHECKS: bool = False     # FFB_DENSITY_CHECKS: sync reports rho <= 0 (settings.f90, simulation.f90:229)
     extra: dict = field(default_factory=dict)
 
     def config(self) -> FfbConfig:
@@ -73,7 +74,8 @@ class Settings:
             u_wall=float(self.U_WALL), u_lid=float(self.U_LID), rho_in=float(self.RHO_IN),
             rho_out=float(self.RHO_OUT), storage_scale_log2=int(self.storage_scale_log2),
             device=int(self.device), body_force=float(self.BODY_FORCE),
-            storage_rounding=ROUNDINGS[self.storage_rounding])
+            storage_rounding=ROUNDINGS[self.storage_rounding],
+            density_checks=1 if self.DENSITY_CHECKS else 0)
 
 
 def reynolds(settings: Settings) -> float:

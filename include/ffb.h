@@ -88,6 +88,8 @@ extern "C" {
 #define FFB_E_INVALID 1   /* bad argument or configuration */
 #define FFB_E_HIP 2       /* HIP runtime error (allocation, launch, copy) */
 #define FFB_E_STATE 3     /* call not valid in the handle's current state */
+#define FFB_E_DENSITY 4   /* density check (ffb_config_t.density_checks) failed: a cell had
+                             rho <= 0 or rho NaN in a collision (ffb_sync reports it) */
 
 /* Run configuration: the compile-time parameters of source/settings.f90:21-53
  * as run-time values. */
@@ -108,7 +110,9 @@ typedef struct ffb_config {
     int32_t device;       /* HIP device ordinal */
     double body_force;    /* FFB_BODY_FORCE_CHANNEL: force density along x; ignored otherwise */
     int32_t storage_rounding; /* 16-bit forms (FP16, BF16, 1-3-12): FFB_ROUND_STOCHASTIC or FFB_ROUND_NEAREST */
-    int32_t reserved;
+    int32_t density_checks;   /* FFB_DENSITY_CHECKS (simulation.f90:229-233, poiseuille_flow.f90:156-160):
+                                 non-zero: every collision and Poiseuille strip flags a density rho <= 0
+                                 (or NaN) and ffb_sync returns FFB_E_DENSITY */
 } ffb_config_t;
 
 typedef struct ffb_lattice ffb_lattice_t;
@@ -118,7 +122,8 @@ typedef struct ffb_lattice ffb_lattice_t;
 void ffb_config_default(ffb_config_t *cfg);
 
 /* Allocates the device domain.  Replaces the reference's allocate(f_a, f_b,
- * rho, u_x, u_y) and halo buffers (main.f90:96-105, exchange.f90:117-135). */
+ * rho, u_x, u_y) and halo buffers (main.f90:96-105, exchange.f90:117-135).
+ * Bounds: n_x <= 2^24 (row and byte offsets fit 32 bits), n_x * n_y <= 2^34. */
 int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out);
 int ffb_destroy(ffb_lattice_t *lat);
 
@@ -141,7 +146,12 @@ int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
  * them): omega, body-force density, Couette wall speed, lid speed. */
 int ffb_set_parameters(ffb_lattice_t *lat, double omega, double body_force, double u_wall, double u_lid);
 
-/* Waits for all work of the handle. */
+/* Waits for all work of the handle.  With cfg.density_checks set, returns
+ * FFB_E_DENSITY once a collision has met rho <= 0 (message: the reference's
+ * "error: density is zero in collision/streaming step (rho_val <= 0)") or a
+ * NaN density; the flag stays set until ffb_init.  Replaces the reference's
+ * `error stop` under FFB_DENSITY_CHECKS (simulation.f90:229-233, :133-137,
+ * modes/poiseuille_flow.f90:156-160). */
 int ffb_sync(ffb_lattice_t *lat);
 
 /* Copies the macro fields written by the last write_macro step to host
@@ -213,10 +223,15 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
 /* ------------------------------------------------------------------------
  * Batched form (no reference counterpart: the reference runs one domain per
  * process).  Many independent lattices of uneven size, each with its own
- * omega and boundary kind, advance together: one persistent kernel launch per
- * step covers all of them, and the launches of a ping-pong pair are captured
- * once in a hipGraph and replayed.  ffb_batch_autotune() picks the workgroup
- * size and the persistent grid by timing the candidates on the batch.
+ * omega and boundary kind, advance together: in the 4- and 8-byte forms
+ * consecutive steps run as step pairs, two steps per launch of one kernel over
+ * the tiles of all lattices, in the 16-bit forms one step per launch of one
+ * kernel over the row segments of all lattices (both bitwise the same;
+ * FFB_BATCH_PAIRS=0/1 at creation selects one or the other; either kernel
+ * runs one workgroup per tile or a persistent grid), and the launches of a
+ * ping-pong cycle are captured once in a hipGraph and replayed.  ffb_batch_autotune()
+ * picks the launch shape of the kernel the batch runs by timing the
+ * candidates on the batch.
  * ------------------------------------------------------------------------ */
 typedef struct ffb_batch_lattice {
     int32_t n_x, n_y;     /* n_x <= 2^24, n_x * n_y <= 2^30 */
@@ -232,16 +247,19 @@ typedef struct ffb_batch_lattice {
 } ffb_batch_lattice_t;
 
 typedef struct ffb_batch_tuning {
+    /* one launch per step (also an odd remainder of a paired batch) */
     int32_t tile_w;       /* threads per workgroup (64-cell row segments x 64): 64 .. 1024 */
     int32_t grid_blocks;  /* persistent grid size; 0 = one workgroup per tile */
+    /* step pairs */
+    int32_t pair_shape;   /* 2: 62 x 14 output tiles (8 waves), 1: 62 x 30 (16 waves); FP64 always 62 x 6 */
+    int32_t pair_group;   /* consecutive tiles per XCD (0: dispatch order) */
+    int32_t pair_grid;    /* persistent grid size (rounded up to a multiple of 8); 0 = one workgroup per tile */
+    int32_t reserved;
 } ffb_batch_tuning_t;
 
 typedef struct ffb_batch ffb_batch_t;
 
-/* FP32, FP64 and FP32-deviation batches advance in step pairs, two steps per
- * launch (bitwise the same; FFB_BATCH_PAIRS=0 at creation selects one launch
- * per step); the tuning below then applies to odd remainders.
- * precision: any FFB_PREC_* form.  The deviation forms use each lattice's
+/* precision: any FFB_PREC_* form.  The deviation forms use each lattice's
  * storage_scale_log2 (ffb_default_scale_log2 at the lattice's speed is a good
  * choice; 0 for FP32 deviations) and round the 16-bit stores stochastically,
  * with random bits from (cell of the lattice, step), as the single-domain form;
@@ -252,7 +270,8 @@ int ffb_batch_destroy(ffb_batch_t *batch);
 /* Equilibrium initial state of every lattice (shear wave: its sine profile;
  * the others at rest with rho_0). */
 int ffb_batch_init(ffb_batch_t *batch);
-/* Enqueues n_steps steps of every lattice (graph replays of step pairs). */
+/* Enqueues n_steps steps of every lattice (graph replays; the graph is
+ * captured again after a change of tuning). */
 int ffb_batch_step(ffb_batch_t *batch, int32_t n_steps, int32_t write_macro_last);
 int ffb_batch_sync(ffb_batch_t *batch);
 int ffb_batch_elapsed_ms(ffb_batch_t *batch, double *ms);
@@ -260,8 +279,10 @@ int ffb_batch_reset_timer(ffb_batch_t *batch);
 /* Total cells, number of tiles, and the tuning in use. */
 int ffb_batch_info(ffb_batch_t *batch, int64_t *cells, int32_t *tiles, ffb_batch_tuning_t *tuning);
 int ffb_batch_set_tuning(ffb_batch_t *batch, const ffb_batch_tuning_t *tuning);
-/* Times every candidate (tile_w x persistent grid) for trial_steps steps,
- * keeps the fastest, and re-initialises the batch (the trials advance it). */
+/* Times every candidate launch shape of the kernel the batch runs (step
+ * pairs: pair_shape x pair_group x pair_grid; one launch per step: tile_w x
+ * grid_blocks) for trial_steps steps, keeps the fastest, and re-initialises
+ * the batch (the trials advance it).  *best_ms_per_step is per step. */
 int ffb_batch_autotune(ffb_batch_t *batch, int32_t trial_steps, ffb_batch_tuning_t *best, double *best_ms_per_step);
 /* Macro fields of lattice `index` after a write_macro step (n_x*n_y each). */
 int ffb_batch_read_macros(ffb_batch_t *batch, int32_t index, void *rho, void *u_x, void *u_y);

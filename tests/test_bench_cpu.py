@@ -1,7 +1,9 @@
 """CPU checks of bench.py's contract pieces that need no GPU: the one-GPU
 guard, the selection of one launch per step or step pairs for the lattices a
-leg creates (and its restoration of the environment), and the traffic lookup
-from profiles/traffic.txt."""
+leg creates (and its restoration of the environment), the roofline
+arithmetic (bytes a launch must move, not per update), the traffic file's
+source stamp (a stale file is refused), and the segmentation of a profiled
+run's counter file into workloads."""
 import os
 import subprocess
 import sys
@@ -21,8 +23,7 @@ def test_more_than_one_gpu_exits_with_message():
 
 def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     monkeypatch.delenv("FFB_TWO_STEP", raising=False)
-    assert bench.uses_pairs("fp32") and bench.uses_pairs("fp64") and bench.uses_pairs("fp16dev")
-    assert not bench.uses_pairs("e3m12dev")
+    assert all(bench.uses_pairs(p) for p in ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev"))
     with bench.pass_mode(False):
         assert os.environ["FFB_TWO_STEP"] == "0" and not bench.uses_pairs("fp32")
     assert "FFB_TWO_STEP" not in os.environ
@@ -36,17 +37,56 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
 
 def test_batch_pair_selection(monkeypatch):
     monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
-    assert bench.batch_uses_pairs("fp32") and bench.batch_uses_pairs("fp64")
+    assert bench.batch_uses_pairs("fp32") and bench.batch_uses_pairs("fp64") and bench.batch_uses_pairs("fp32dev")
     assert not bench.batch_uses_pairs("e3m12dev") and not bench.batch_uses_pairs("fp16dev")
     monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
     assert not bench.batch_uses_pairs("fp32")
 
 
-def test_traffic_lines_for_both_paths():
-    for workload in ("sl4096_fp32", "sl4096_fp64", "sl4096_fp32_single_pass", "sl4096_fp64_single_pass"):
-        t = bench.traffic_from_profiles(workload)
-        assert t is not None and t > 1e9, workload
-    # the pair moves fewer bytes than two single steps, more than one step's minimum
-    pair, single = bench.traffic_from_profiles("sl4096_fp32"), bench.traffic_from_profiles("sl4096_fp32_single_pass")
-    assert 18 * 4 * 4096 * 4096 < pair < 2 * single
+def test_roofline_counts_one_read_and_write_per_launch():
+    """A pair launch of the FP32 4096^2 cavity must move 18*4*4096^2 bytes,
+    not twice that: its fraction is that over the launch time and 8 TB/s, at
+    most 1 for any launch not faster than the peak allows."""
+    must = 18 * 4 * 4096 * 4096
+    run = {"achieved_gbs": must / 0.2e-3 / 1e9, "bytes_per_launch": must, "steps_per_launch": 2,
+           "kernel_ms": 0.2}
+    r = bench.roofline(run, "no_such_workload")
+    assert abs(r["frac"] - must / 0.2e-3 / 8e12) < 1e-4 and r["frac"] < 1
+    rates = bench.hbm_rates(160000.0, "fp32", 2)
+    assert abs(rates["min_bytes_gbs"] - 160000e6 * 72 / 2 / 1e9) < 0.1
+    assert abs(rates["effective_gbs"] - 2 * rates["min_bytes_gbs"]) < 0.2
+
+
+def test_traffic_file_is_stamped_and_a_stale_one_refused(monkeypatch, tmp_path):
+    stamp = bench.kernel_source_hash()
+    good = tmp_path / "good.txt"
+    good.write_text(f"# kernel sources {stamp}\nsl4096_fp32 700000000 610000000 6 ffb::step2_kernel<float>\n")
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(good))
+    t = bench.traffic_from_profiles("sl4096_fp32")
+    assert t["bytes"] == 1.31e9 and t["kernel"] == "ffb::step2_kernel<float>"
     assert bench.traffic_from_profiles("no_such_workload") is None
+    stale = tmp_path / "stale.txt"
+    stale.write_text("# kernel sources 0123456789abcdef\nsl4096_fp32 700000000 610000000 6 k\n")
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(stale))
+    assert bench.traffic_from_profiles("sl4096_fp32") is None
+    assert "not used" in bench.traffic_status()
+
+
+def test_committed_traffic_file_matches_the_sources():
+    """profiles/traffic.txt is re-taken whenever a kernel source changes."""
+    stamp, table = bench._traffic_table()
+    assert stamp == bench.kernel_source_hash(), bench.traffic_status()
+    names = [n for n, _ in bench._profile_workloads()]
+    assert set(names) <= set(table)
+    pair, single = table["sl4096_fp32"]["bytes"], table["sl4096_fp32_single_pass"]["bytes"]
+    assert 18 * 4 * 4096 * 4096 < pair < 2 * single
+
+
+def test_profile_segments_follow_the_markers():
+    m1, m2 = bench.MARK_START // 16, bench.MARK_END // 16
+    disp = [("init_kernel", 100, 1.0), ("copy_kernel<0>", 2097152, 5), ("copy_kernel<1>", m1, 1),
+            ("copy_kernel<3>", m1, 1), ("step2_kernel<a>", 10, 100.0), ("step2_kernel<a>", 10, 102.0),
+            ("copy_kernel<1>", m2, 1), ("batch_pair_kernel", 9, 5.0), ("batch_pair_kernel", 9, 5.0),
+            ("copy_kernel<1>", m1, 1), ("batch_count_kernel", 1, 0.1), ("batch_pair_kernel", 9, 7.0),
+            ("copy_kernel<2>", m2, 1)]
+    assert bench._segments(disp) == [("step2_kernel<a>", 101.0, 2), ("batch_pair_kernel", 7.0, 1)]

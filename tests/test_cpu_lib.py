@@ -34,8 +34,9 @@ def test_config_defaults_are_reference_settings():
 
     c = _lib.FfbConfig()
     _lib.lib().ffb_config_default(ctypes.byref(c))
-    # source/settings.f90:27-63
+    # source/settings.f90:27-63; FFB_DENSITY_CHECKS is off unless defined
     assert (c.n_x, c.n_y, c.sim_mode) == (2520, 2520, 4)
+    assert c.density_checks == 0
     assert (c.omega, c.rho_0, c.u_max, c.n_sin) == (1.1, 1.0, 0.1, 2.0)
     assert (c.u_wall, c.u_lid, c.rho_in, c.rho_out) == (0.1, 0.1, 1.00005, 0.99995)
 
@@ -54,6 +55,14 @@ def test_invalid_config_is_rejected_with_message():
     c.n_x = 0
     assert L.ffb_create(ctypes.byref(c), ctypes.byref(h)) == 1
     assert h.value is None
+    # a row of more than 2^24 cells: its 9 direction rows and byte offsets
+    # would overflow 32 bits, so it is refused before any device call
+    c.n_x, c.n_y = (1 << 24) + 1, 1
+    assert L.ffb_create(ctypes.byref(c), ctypes.byref(h)) == 1
+    assert b"2^24" in L.ffb_last_error() and h.value is None
+    c.n_x = 1 << 24
+    assert L.ffb_create(ctypes.byref(c), ctypes.byref(h)) != 0  # no GPU here: fails later, at the device
+    assert b"2^24" not in L.ffb_last_error()
 
 
 def test_invalid_batch_is_rejected_before_any_gpu_call():

@@ -1,11 +1,14 @@
 """Batched form on the GPU: every lattice of a mixed batch, advanced by the
-persistent hipGraph-replayed launch, is bitwise equal to the oracle run of
+persistent hipGraph-replayed launches, is bitwise equal to the oracle run of
 that lattice alone (FP32, FP64 and the deviation forms at each lattice's own
 scale, the 16-bit ones with stochastic rounding, whose step index the graph
 replays advance on the device), across graph replays, odd/even step counts,
-re-tuning and the autotuner's re-initialisation; with step pairs in one
-launch (the default for FP32, FP64 and FP32 deviations) and with one launch
-per step."""
+re-tuning (the graph is captured again) and the autotuner's
+re-initialisation; with step pairs in one launch (the default) under several
+pair tunings, persistent grids smaller than the tile count included, and with
+one launch per step under several single-step tunings.  The 512-lattice mixed
+batch is checked against an FP64 batch of the same lattices, lattice by
+lattice."""
 import numpy as np
 import pytest
 
@@ -39,16 +42,29 @@ def oracle_of(L, dtype, steps, scale=0):
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-@pytest.mark.parametrize("pairs", ["default", "0"])
+# (pairs, tuning): step pairs with tile shape / XCD group / persistent grid,
+# or one launch per step with workgroup size / persistent grid
+TUNINGS = [
+    ("default", {"pair_shape": 2, "pair_group": 32, "pair_grid": 0}),
+    ("default", {"pair_shape": 1, "pair_group": 3, "pair_grid": 8}),
+    ("default", {"pair_shape": 2, "pair_group": 0, "pair_grid": 40}),
+    ("0", {"tile_w": 256, "grid_blocks": 0}),
+    ("0", {"tile_w": 128, "grid_blocks": 8}),
+    ("0", {"tile_w": 512, "grid_blocks": 3}),
+    ("0", {"tile_w": 1024, "grid_blocks": 0}),
+]
+
+
+@pytest.mark.parametrize("pairs,tuning", TUNINGS,
+                         ids=[f"pairs{p}-" + "-".join(f"{k}{v}" for k, v in t.items()) for p, t in TUNINGS])
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"])
-@pytest.mark.parametrize("tile_w,grid", [(256, 0), (128, 8), (512, 3), (1024, 0)])
-def test_batch_matches_oracle_per_lattice(monkeypatch, precision, tile_w, grid, pairs):
-    """pairs "default": step pairs in one launch for FP32, FP64 and FP32
-    deviations (the tuning then applies to the odd remainders), one launch per
-    step for the 16-bit forms; "0": one launch per step everywhere."""
+def test_batch_matches_oracle_per_lattice(monkeypatch, precision, pairs, tuning):
+    """pairs "default": step pairs in one launch (the tuning applies to the
+    pair kernel; an odd remainder takes one single step); "0": one launch per
+    step everywhere."""
     from fastfortranboltzmann_amd.batch import Batch
 
-    print(f"[gpu] batch {precision} tile_w={tile_w} grid={grid} pairs={pairs}", flush=True)
+    print(f"[gpu] batch {precision} pairs={pairs} tuning={tuning}", flush=True)
     if pairs == "0":
         monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
     else:
@@ -56,10 +72,10 @@ def test_batch_matches_oracle_per_lattice(monkeypatch, precision, tile_w, grid, 
     lats = small_batch()
     dt = {"fp64": np.float64, "fp32": np.float32}.get(precision, precision)
     with Batch(lats, precision) as b:
-        b.set_tuning(tile_w, grid)
+        b.set_tuning(**tuning)
         b.init()
         done = 0
-        for chunk in (5, 6, 1, 11):           # parities 1/0 at the chunk starts; graph pairs inside
+        for chunk in (5, 6, 1, 11, 8):       # parities 1/0 at the chunk starts; graph cycles inside
             b.step(chunk, write_macro_last=True)
             done += chunk
             for i, L in enumerate(lats):
@@ -69,6 +85,8 @@ def test_batch_matches_oracle_per_lattice(monkeypatch, precision, tile_w, grid, 
                 rho, ux, uy = b.macros(i)
                 np.testing.assert_array_equal(ux, o.u_x)
                 np.testing.assert_array_equal(rho, o.rho)
+            if done == 12:  # re-tune between graph replays: the graph is captured again
+                b.set_tuning(pair_group=5, grid_blocks=16)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "e3m12dev"])
@@ -83,6 +101,8 @@ def test_batch_autotune_reinitialises_and_keeps_results(precision):
     with Batch(lats, precision) as b:
         best = b.autotune(4)
         assert best["tile_w"] in (64, 128, 256, 512, 1024) and best["ms_per_step"] > 0
+        assert best["pair_shape"] in (1, 2) and best["pair_group"] >= 0 and best["pair_grid"] >= 0
+        assert b.info()["tuning"] == {k: v for k, v in best.items() if k != "ms_per_step"}
         b.step(9, write_macro_last=True)
         dt = np.float32 if precision == "fp32" else precision
         for i, L in enumerate(lats):
@@ -103,3 +123,20 @@ def test_single_lattice_body_force_channel_matches_oracle(precision):
         lat.init().step(40, write_macro_last=True)
         np.testing.assert_array_equal(lat.interior(), o.interior)
         np.testing.assert_array_equal(lat.macros()[1], o.u_x)
+
+
+@pytest.mark.parametrize("precision,bound", [("fp32dev", 1e-4), ("fp32", 1.5e-1)])
+def test_mixed_batch_512_against_fp64(precision, bound):
+    """The benchmark's 512-lattice mixed batch (seed 0), autotuned, 300 steps:
+    every lattice's relative L2 error of u against an FP64 batch of the same
+    lattices is below `bound` (FP32 deviations: the precision form's 1e-4
+    target; plain FP32: the body-force channels, whose force of 1e-6 moves
+    f ~ w by a few FP32 ulps per step, carry errors of several per cent)."""
+    from fastfortranboltzmann_amd.batch import batch_study, mixed_batch
+
+    print(f"[gpu] mixed batch 512 {precision} vs fp64", flush=True)
+    res = batch_study(mixed_batch(0, 512), 298, autotune=True, trial_steps=4, forms=(precision,),
+                      log=lambda m: print(m, flush=True))
+    err = res["forms"][precision]["rel_l2_u_vs_fp64"]
+    print(f"[gpu] mixed batch 512 {precision}: {err}", flush=True)
+    assert err["max"] < bound, err

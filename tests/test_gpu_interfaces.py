@@ -165,3 +165,49 @@ def test_deviation_extent_shows_clamping_of_the_1_3_12_form():
     assert 0.055 < dev[0] < lim[0] and np.all(dev < lim)
     dev, lim = res[7]
     assert dev[0] == lim[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16dev", "e3m12dev"])
+def test_density_check_reports_the_reference_stop(precision):
+    """FFB_DENSITY_CHECKS (simulation.f90:229-233): a state with one cell whose
+    pulled density is negative makes ffb_sync return FFB_E_DENSITY with the
+    reference's message; the flag stays until ffb_init; without the check the
+    same run syncs normally; a NaN density is reported as such."""
+    import numpy as np
+
+    from fastfortranboltzmann_amd import Lattice, Settings
+    from fastfortranboltzmann_amd._lib import E_DENSITY, FfbError
+
+    print(f"[gpu] density check {precision}", flush=True)
+    nx, ny = 40, 30
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    dt = np.float64 if precision == "fp64" else np.float32
+    f = np.broadcast_to(w[:, None, None], (9, ny + 2, nx + 2)).astype(dt).copy()
+    f[:, 11:14, 16:19] = -2.0 * w[:, None, None]  # cell (16, 11) pulls all nine from this block: rho = -2
+    for checks in (False, True):
+        s = Settings(N_X=nx, N_Y=ny, SIM_MODE=4, OMEGA=1.2, U_LID=0.05, precision=precision,
+                     storage_scale_log2=1 if precision.endswith("dev") else 0, DENSITY_CHECKS=checks)
+        with Lattice(s) as lat:
+            lat.init()
+            lat.set_populations(f)
+            lat.step(3)
+            if not checks:
+                lat.sync()
+                continue
+            with pytest.raises(FfbError) as err:
+                lat.sync()
+            assert err.value.code == E_DENSITY
+            assert "error: density is zero in collision/streaming step (rho_val <= 0)" in str(err.value)
+            with pytest.raises(FfbError):
+                lat.sync()  # sticky until init
+            lat.init().step(4)
+            lat.sync()  # a healthy state runs clean
+            if precision == "e3m12dev":
+                continue  # the 1-3-12 layout has no NaN: a NaN written in is stored as a finite value
+            g = np.broadcast_to(w[:, None, None], f.shape).astype(dt).copy()
+            g[:, 5, 5] = np.nan
+            lat.set_populations(g)
+            lat.step(1)
+            with pytest.raises(FfbError) as err:
+                lat.sync()
+            assert err.value.code == E_DENSITY and "NaN" in str(err.value)

@@ -2,10 +2,11 @@
 per step (step_kernel): bitwise equal populations and macro fields in every
 mode without a strip pass and every storage form, from random states that
 make every boundary rule carry a distinct value, on shapes smaller and larger
-than one 62 x 30 (FP64: 62 x 6) tile, with odd step counts (the last step of
-an odd count runs alone) and macro output on the last step.  The pair path
-is the default for the 4- and 8-byte forms, so the oracle parity tests cover
-it too; FFB_TWO_STEP selects the path when a lattice is created."""
+than one tile (62 x 14 and 62 x 30 for 4-byte and narrower elements, both
+region shapes; FP64 62 x 6), with odd step counts (the last step of an odd
+count runs alone) and macro output on the last step.  The pair path is the
+default for every form, so the oracle parity tests cover it too;
+FFB_TWO_STEP and FFB_PAIR_SHAPE select the path when a lattice is created."""
 import numpy as np
 import pytest
 
@@ -16,10 +17,12 @@ FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 10), ("bf16dev", 
 SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15), (70, 67)]
 
 
-def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps):
+def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps, env=None):
     from fastfortranboltzmann_amd import Lattice, Settings
 
     monkeypatch.setenv("FFB_TWO_STEP", "1" if pairs else "0")
+    for k, v in (env or {}).items():
+        monkeypatch.setenv(k, str(v))
     s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=0.83, U_MAX=0.02, N_SIN=1.0, U_WALL=0.05, U_LID=0.07,
                  BODY_FORCE=1e-5, precision=precision, storage_scale_log2=scale,
                  storage_rounding="stochastic")
@@ -30,19 +33,27 @@ def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps):
         return lat.populations(), lat.macros(), lat.steps
 
 
+def _random_state(seed, precision, nx, ny):
+    rng = np.random.default_rng(seed)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    dt = np.float64 if precision == "fp64" else np.float32
+    return (w[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dt)
+
+
+KINDS = {"shape8x2": {"FFB_PAIR_SHAPE": 2}, "shape16x2": {"FFB_PAIR_SHAPE": 1}}
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{a}x{b}" for a, b in SHAPES])
 @pytest.mark.parametrize("form", FORMS, ids=[f[0] for f in FORMS])
 @pytest.mark.parametrize("mode", list(MODES), ids=list(MODES.values()))
-def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape):
+def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape, kind):
     precision, scale = form
     nx, ny = shape
-    print(f"[gpu] two-step pairs mode {mode} {precision} {nx}x{ny}", flush=True)
-    rng = np.random.default_rng(100 * mode + nx + 7 * ny)
-    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
-    dt = np.float64 if precision == "fp64" else np.float32
-    f0 = (w[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dt)
+    print(f"[gpu] two-step pairs ({kind}) mode {mode} {precision} {nx}x{ny}", flush=True)
+    f0 = _random_state(100 * mode + nx + 7 * ny, precision, nx, ny)
     for steps in (2, 7):
-        fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps)
+        fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps, KINDS[kind])
         fb, mb, nb = _run(monkeypatch, False, mode, nx, ny, precision, scale, f0, steps)
         assert na == nb == steps
         np.testing.assert_array_equal(fa, fb)
