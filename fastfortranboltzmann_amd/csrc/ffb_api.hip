@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -413,6 +414,48 @@ __global__ void __launch_bounds__(256) copy_kernel(const v4f *__restrict__ src, 
     }
 }
 
+// Scratch of ffb_moments_noalloc: the FP64 partial sums of the moment kernel,
+// three per workgroup for at most kMomentBlocks workgroups (48 KiB per
+// device, held until the process ends).  In host code `scratch` names a
+// host-side stand-in; kernels get the device address from scratch_on().
+constexpr int kMomentBlocks = 2048;
+static __device__ __align__(256) double scratch[3 * kMomentBlocks];
+
+std::mutex g_scratch_mu;
+double *g_scratch[64] = {};  // device address of `scratch` per device, resolved once
+
+// Device address of `scratch` on `device` (current), resolved at the first call.
+int scratch_on(int device, double **out) {
+    if (device < 0 || device >= 64) return fail(FFB_E_INVALID, "ffb_moments_noalloc: device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FFB_E_HIP, std::string("ffb_moments_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch[device];
+    return FFB_OK;
+}
+
+template <typename S, typename C>
+void launch_moments_into(const ffb_config_t &c, const void *src, long long pitch, long long plane, int blocks,
+                         double *partial, double *out, hipStream_t stream) {
+    StepArgs<C> a{};
+    a.src = src;
+    a.nx = c.n_x;
+    a.ny = c.n_y;
+    a.pitch = (int)pitch;
+    a.plane = plane;
+    a.w[0] = (C)4 / (C)9;
+    for (int d = 1; d <= 4; ++d) a.w[d] = (C)1 / (C)9;
+    for (int d = 5; d <= 8; ++d) a.w[d] = (C)1 / (C)36;
+    a.inv_scale = (C)std::ldexp(1.0, -(is_deviation_form(c.precision) ? work_scale_log2(c) : 0));
+    hipLaunchKernelGGL((moments_partial_kernel<S, C>), dim3((unsigned)blocks), dim3(256), 0, stream, a, partial);
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, stream, partial, blocks, out);
+}
+
 // lattice weight in FP32, folded like the device's (C)4 / (C)9 etc.
 float weightf(int d) { return d == 0 ? 4.0f / 9.0f : (d <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f); }
 
@@ -570,7 +613,7 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     }
     if ((e = hipMalloc(&L->ux_row, (size_t)cfg->n_y * L->celem)) != hipSuccess) return cleanup_fail(e, "profile");
     long long cells = (long long)cfg->n_x * cfg->n_y;
-    L->mom_blocks = (int)std::min<long long>(2048, (cells + 255) / 256);
+    L->mom_blocks = (int)std::min<long long>(kMomentBlocks, (cells + 255) / 256);
     if ((e = hipMalloc(&L->mom_partial, (size_t)L->mom_blocks * 3 * sizeof(double))) != hipSuccess)
         return cleanup_fail(e, "moment partials");
     if ((e = hipMalloc(&L->mom_out, 3 * sizeof(double))) != hipSuccess) return cleanup_fail(e, "moments");
@@ -879,6 +922,36 @@ int ffb_moments(ffb_lattice_t *L, double *mass, double *mom_x, double *mom_y) {
     if (mass) *mass = h[0];
     if (mom_x) *mom_x = h[1];
     if (mom_y) *mom_y = h[2];
+    return FFB_OK;
+}
+
+int ffb_moments_noalloc(const ffb_config_t *cfg, const void *populations, int64_t pitch, int64_t plane,
+                        void *stream, double *out) {
+    if (!cfg || !populations || !out) return fail(FFB_E_INVALID, "ffb_moments_noalloc: null argument");
+    if (cfg->n_x < 1 || cfg->n_y < 1 || cfg->n_x > (1 << 24) || (long long)cfg->n_x * cfg->n_y > (1LL << 34))
+        return fail(FFB_E_INVALID, "ffb_moments_noalloc: n_x, n_y outside 1 .. 2^24, n_x * n_y above 2^34");
+    if (cfg->precision < FFB_PREC_FP32 || cfg->precision > FFB_PREC_E3M12_DEV)
+        return fail(FFB_E_INVALID, "ffb_moments_noalloc: unknown precision " + std::to_string(cfg->precision));
+    if (pitch < cfg->n_x || pitch > INT_MAX || plane < cfg->n_x)
+        return fail(FFB_E_INVALID, "ffb_moments_noalloc: pitch or plane smaller than n_x (or pitch above 2^31-1)");
+    HIPCHK(hipSetDevice(cfg->device));
+    double *partial = nullptr;
+    if (int rc = scratch_on(cfg->device, &partial)) return rc;
+    // the workgroup count of ffb_moments (same partial sums, same order)
+    const long long cells = (long long)cfg->n_x * cfg->n_y;
+    const int blocks = (int)std::min<long long>(kMomentBlocks, (cells + 255) / 256);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (cfg->precision) {
+    case FFB_PREC_FP64: launch_moments_into<double, double>(*cfg, populations, pitch, plane, blocks, partial, out, st); break;
+    case FFB_PREC_FP16_DEV: launch_moments_into<__half, float>(*cfg, populations, pitch, plane, blocks, partial, out, st); break;
+    case FFB_PREC_BF16_DEV:
+        launch_moments_into<__hip_bfloat16, float>(*cfg, populations, pitch, plane, blocks, partial, out, st);
+        break;
+    case FFB_PREC_FP32_DEV: launch_moments_into<f32dev, float>(*cfg, populations, pitch, plane, blocks, partial, out, st); break;
+    case FFB_PREC_E3M12_DEV: launch_moments_into<e3m12, float>(*cfg, populations, pitch, plane, blocks, partial, out, st); break;
+    default: launch_moments_into<float, float>(*cfg, populations, pitch, plane, blocks, partial, out, st); break;
+    }
+    HIPCHK(hipGetLastError());
     return FFB_OK;
 }
 

@@ -206,6 +206,23 @@ int ffb_deviation_extent(ffb_lattice_t *lat, double max_dev[3], double limit[3])
  * counterpart. */
 int32_t ffb_default_scale_log2(int32_t precision, double speed);
 
+/* ffb_moments without allocating: the FP64 sums mass, mom_x, mom_y of the
+ * populations at `populations` (device memory, the layout ffb_device_info
+ * reports: element d * plane + y * pitch + x, stored form of cfg->precision and
+ * cfg->storage_scale_log2, cfg->n_x by cfg->n_y cells), written to out[0..2]
+ * (device memory) by kernels enqueued on `stream` (a hipStream_t; NULL: the
+ * null stream) of device cfg->device; returns without waiting.  The same
+ * kernels and partial-sum order as ffb_moments, so the sums are bitwise
+ * equal.  No counterpart in the reference; for callers that cannot allocate
+ * device memory (code under hipGraph capture, serving loops, harnesses that
+ * forbid hipMalloc): the partial sums live in a 48 KiB array of the library
+ * on each device, so the call makes no allocation.  Limit: n_x <= 2^24,
+ * n_x * n_y <= 2^34 (at most 2048 partial sums, any domain size within that).
+ * Calls that overlap on one device (other streams, other threads) share that
+ * array and race; each call overwrites every partial sum it reads. */
+int ffb_moments_noalloc(const ffb_config_t *cfg, const void *populations, int64_t pitch, int64_t plane,
+                        void *stream, double *out);
+
 /* Device pointers for zero-copy interop (torch, hipGraph capture):
  * *cur = populations the next step reads, *stream = the handle's hipStream_t;
  * population d of cell (x, y) is element d * plane + y * pitch + x of *cur
