@@ -21,7 +21,7 @@ EXPORTS = (
     "ffb_config_default", "ffb_create", "ffb_destroy", "ffb_init", "ffb_step", "ffb_sync",
     "ffb_read_macros", "ffb_read_populations", "ffb_write_populations", "ffb_read_pf_strips",
     "ffb_elapsed_ms", "ffb_reset_timer", "ffb_step_count", "ffb_moments", "ffb_moments_noalloc",
-    "ffb_deviation_extent",
+    "ffb_deviation_extent", "ffb_device_properties", "ffb_memory_usage",
     "ffb_default_scale_log2",
     "ffb_device_info",
     "ffb_bench_copy", "ffb_last_error", "ffb_version",
@@ -104,6 +104,10 @@ def lib() -> ctypes.CDLL:
         "ffb_reset_timer": ([vp], ctypes.c_int),
         "ffb_step_count": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "ffb_moments": ([vp, dp, dp, dp], ctypes.c_int),
+        "ffb_device_properties": ([i32, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "ffb_memory_usage": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                              ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "ffb_moments_noalloc": ([ctypes.POINTER(FfbConfig), vp, ctypes.c_int64, ctypes.c_int64, vp, vp],
                                 ctypes.c_int),
         "ffb_deviation_extent": ([vp, dp, dp], ctypes.c_int),

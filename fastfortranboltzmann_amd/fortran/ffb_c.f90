@@ -11,7 +11,8 @@ module ffb_c
 
     public :: ffb_config_t, ffb_config_default, ffb_create, ffb_destroy, ffb_init, ffb_step, &
         ffb_sync, ffb_read_macros, ffb_read_populations, ffb_elapsed_ms, ffb_reset_timer, &
-        ffb_moments, ffb_deviation_extent, ffb_default_scale_log2, ffb_check, ffb_error_message
+        ffb_moments, ffb_deviation_extent, ffb_default_scale_log2, ffb_check, ffb_error_message, &
+        ffb_device_properties, ffb_memory_usage
     public :: FFB_SHEAR_WAVE, FFB_COUETTE_FLOW, FFB_POISEUILLE_FLOW, FFB_SLIDING_LID
     public :: FFB_PREC_FP32, FFB_PREC_FP64, FFB_PREC_FP16_DEV, FFB_PREC_BF16_DEV, FFB_PREC_FP32_DEV, &
         FFB_PREC_E3M12_DEV
@@ -126,6 +127,25 @@ module ffb_c
             integer(c_int32_t), value :: precision
             real(c_double), value :: speed
         end function ffb_default_scale_log2
+
+        ! name, compute units and memory of a HIP device (hardware report)
+        integer(c_int) function ffb_device_properties(device, name, name_len, compute_units, memory_bytes) &
+                bind(c, name="ffb_device_properties")
+            import :: c_int, c_int32_t, c_int64_t, c_char
+            integer(c_int32_t), value :: device
+            character(kind=c_char), intent(out) :: name(*)
+            integer(c_int32_t), value :: name_len
+            integer(c_int32_t), intent(out) :: compute_units
+            integer(c_int64_t), intent(out) :: memory_bytes
+        end function ffb_device_properties
+
+        ! device bytes of the handle: population sets, macro fields, the rest
+        integer(c_int) function ffb_memory_usage(lattice, population_bytes, macro_bytes, other_bytes) &
+                bind(c, name="ffb_memory_usage")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: lattice
+            integer(c_int64_t), intent(out) :: population_bytes, macro_bytes, other_bytes
+        end function ffb_memory_usage
 
         type(c_ptr) function c_ffb_last_error() bind(c, name="ffb_last_error")
             import :: c_ptr

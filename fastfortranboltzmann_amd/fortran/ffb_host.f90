@@ -12,8 +12,8 @@
 ! accepted (domain.f90:73-101) but must describe one image, and the halo
 ! exchange is part of the device kernel, so the halo timing rows read 0.
 program ffb_host
-    use iso_fortran_env, only: int32, int64, real32, real64, output_unit
-    use iso_c_binding, only: c_ptr, c_null_ptr, c_loc, c_int32_t, c_double, c_float, c_int
+    use iso_fortran_env, only: int32, int64, real32, real64, output_unit, compiler_version, compiler_options
+    use iso_c_binding, only: c_ptr, c_null_ptr, c_loc, c_int32_t, c_int64_t, c_double, c_float, c_int
     use ffb_c
     use ffb_settings
     implicit none
@@ -29,6 +29,11 @@ program ffb_host
     integer(int64) :: clock_start, clock_end, clock_rate
     real(real64) :: kernel_seconds, elapsed_seconds, other_seconds, ms, mlups, mass, mx, my, speed
     character(len=:), allocatable :: out_dir
+    ! hardware summary (hardware_info.f90:13-29, 132-145): host CPU, threads,
+    ! nodes, compiler, flags, commit, plus the GPU the domain lives on
+    character(len=:), allocatable :: hw_cpu, hw_compiler, hw_flags, hw_commit, hw_gpu
+    integer(int32) :: hw_threads, hw_nodes, hw_cus
+    integer(int64) :: hw_gpu_bytes
 
     call load_settings(s)
     call check_single_domain()
@@ -98,6 +103,8 @@ program ffb_host
     call ffb_check(ffb_create(cfg, lat), "ffb_create")
     call ffb_check(ffb_init(lat), "ffb_init")
 
+    call collect_hardware()
+    call print_hardware_summary()
     call print_run_summary()
     out_dir = "output/" // trim(s%export_num)
     if (s%export_macros) call export_metadata()
@@ -210,14 +217,78 @@ contains
         flush(output_unit)
     end subroutine print_progress
 
+    ! host CPU model and logical threads from /proc/cpuinfo, nodes from
+    ! SLURM_NNODES (1 without it), the compiler and its options from the
+    ! compiler itself, the commit from FFB_GIT_COMMIT at build time, and the
+    ! GPU from the HIP runtime
+    subroutine collect_hardware()
+        integer :: u, ios, k, n, st
+        character(len=512) :: line
+        character(len=64) :: buf
+        character(len=256) :: gname
+        hw_cpu = "unknown"
+        hw_threads = 0
+        open(newunit=u, file="/proc/cpuinfo", status="old", action="read", iostat=ios)
+        if (ios == 0) then
+            do
+                read(u, '(A)', iostat=ios) line
+                if (ios /= 0) exit
+                if (index(line, "processor") == 1) hw_threads = hw_threads + 1
+                if (index(line, "model name") == 1 .and. hw_cpu == "unknown") then
+                    k = index(line, ":")
+                    if (k > 0) hw_cpu = trim(adjustl(line(k + 1:)))
+                end if
+            end do
+            close(u)
+        end if
+        hw_nodes = 1
+        call get_environment_variable("SLURM_NNODES", buf, n, st)
+        if (st == 0 .and. n > 0) then
+            read(buf(1:n), *, iostat=ios) k
+            if (ios == 0) hw_nodes = k
+        end if
+        hw_compiler = trim(compiler_version())
+        hw_flags = trim(compiler_options())
+        if (len(hw_flags) == 0) hw_flags = "none"
+#ifdef FFB_GIT_COMMIT
+        hw_commit = FFB_GIT_COMMIT
+#else
+        hw_commit = "unknown"
+#endif
+        gname = ""
+        hw_cus = 0
+        hw_gpu_bytes = 0
+        if (ffb_device_properties(0_c_int32_t, gname, int(len(gname), c_int32_t), hw_cus, hw_gpu_bytes) == 0) then
+            k = index(gname, char(0))
+            if (k > 0) gname(k:) = ""
+            hw_gpu = trim(gname)
+        else
+            hw_gpu = "unknown"
+        end if
+    end subroutine collect_hardware
+
+    ! the reference's block (hardware_info.f90:138-144) plus the GPU line
+    subroutine print_hardware_summary()
+        character(len=32) :: t
+        print '(A)', ""
+        print '(A)', "--- [ hardware ] ---------------------------------------------------------------"
+        print '(A,T27,A,A)', "cpu model", "= ", hw_cpu
+        write(t, '(I0)') hw_threads
+        print '(A,T27,A,A)', "logical threads/node", "= ", trim(t)
+        write(t, '(I0)') hw_nodes
+        print '(A,T27,A,A)', "compute nodes", "= ", trim(t)
+        print '(A,T27,A,A)', "compiler", "= ", hw_compiler
+        print '(A,T27,A,A)', "flags", "= ", hw_flags
+        print '(A,T27,A,A)', "latest commit", "= ", hw_commit
+        write(t, '(I0,A,F0.1,A)') hw_cus, " CUs, ", real(hw_gpu_bytes, real64) * 1.0e-9_real64, " GB"
+        print '(A,T27,A,A)', "gpu", "= ", hw_gpu // ", " // trim(t) // ", one device-resident domain"
+    end subroutine print_hardware_summary
+
     subroutine print_run_summary()
         real(real64) :: bytes_pop, gb
         bytes_pop = merge(8.0_real64, merge(4.0_real64, 2.0_real64, &
                           s%precision == "fp32" .or. s%precision == "fp32dev"), fp64)
         gb = 2.0_real64 * 9.0_real64 * bytes_pop * real(s%n_x, real64) * real(s%n_y, real64) * 1.0e-9_real64
-        print '(A)', ""
-        print '(A)', "--- [ hardware ] ---------------------------------------------------------------"
-        print '(A,T27,A)', "device", "= AMD Instinct MI355X (gfx950), one device-resident domain"
         print '(A)', ""
         print '(A)', "--- [ simulation parameters ] --------------------------------------------------"
         print '(A,T27,A,A)', "SIM_MODE", "= ", mode_name(s%sim_mode)
@@ -295,7 +366,18 @@ contains
         call execute_command_line("mkdir -p " // out_dir)
         open(newunit=u, file=out_dir // "/config.json", form="formatted", status="replace", action="write")
         write(u, '(A)') "{"
-        write(u, '(A)') '  "device": "AMD Instinct MI355X (gfx950)",'
+        ! hardware block (hardware_info.f90:257-272), plus the GPU
+        write(u, '(A)') '  "hardware": {'
+        write(u, '(A,A,A)') '    "cpu_model": "', json_escape(hw_cpu), '",'
+        write(u, '(A,I0,A)') '    "logical_threads_per_node": ', hw_threads, ','
+        write(u, '(A,I0,A)') '    "compute_nodes": ', hw_nodes, ','
+        write(u, '(A,A,A)') '    "compiler": "', json_escape(hw_compiler), '",'
+        write(u, '(A,A,A)') '    "compiler_flags": "', json_escape(hw_flags), '",'
+        write(u, '(A,A,A)') '    "gpu": "', json_escape(hw_gpu), '",'
+        write(u, '(A,I0,A)') '    "gpu_compute_units": ', hw_cus, ','
+        write(u, '(A,ES24.16)') '    "gpu_memory_GB": ', real(hw_gpu_bytes, real64) * 1.0e-9_real64
+        write(u, '(A)') '  },'
+        write(u, '(A)') '  "device": "' // json_escape(hw_gpu) // '",'
         write(u, '(A,A,A)') '  "SIM_MODE": "', mode_name(s%sim_mode), '",'
         write(u, '(A,ES24.16,A)') '  "rho_0": ', s%rho_0, ','
         write(u, '(A,ES24.16,A)') '  "omega": ', s%omega, ','
@@ -320,7 +402,8 @@ contains
         write(u, '(A)') '  "N_DIRS": 9,'
         write(u, '(A)') '  "dist_func_layout": "xyd",'
         write(u, '(A)') '  "use_unrolled_kernels": true,'
-        write(u, '(A,A,A)') '  "FFB_FP64": ', merge("true ", "false", fp64), ','
+        write(u, '(A,A,A)') '  "FFB_FP64": ', trim(merge("true ", "false", fp64)), ','
+        write(u, '(A,A,A)') '  "FFB_DENSITY_CHECKS": ', trim(merge("true ", "false", s%density_checks)), ','
         write(u, '(A,A,A)') '  "precision": "', s%precision, '",'
         write(u, '(A,A,A)') '  "export_macros": ', merge("true ", "false", s%export_macros), ','
         write(u, '(A,A,A)') '  "export_endpoint_states": ', merge("true ", "false", s%export_endpoint_states), ','
@@ -330,13 +413,56 @@ contains
         write(u, '(A)') '    "image_grid_x": 1,'
         write(u, '(A)') '    "image_grid_y": 1,'
         write(u, '(A,I0,A)') '    "local_n_x": ', s%n_x, ','
-        write(u, '(A,I0)') '    "local_n_y": ', s%n_y
+        write(u, '(A,I0,A)') '    "local_n_y": ', s%n_y, ','
+        ! the device domain holds no halo cells (boundary values are computed
+        ! where they are read); the reference's layout holds
+        ! (n_x + 2)(n_y + 2) - n_x n_y of them (export.f90:154-157)
+        write(u, '(A)') '    "halo_cells_percent": 0.0'
         write(u, '(A)') '  },'
+        call write_memory_usage(u)
         write(u, '(A,A,A)') '  "output_dir": "', out_dir, '",'
         write(u, '(A,A,A)') '  "file_dtype": "', merge("real64", "real32", fp64), '"'
         write(u, '(A)') "}"
         close(u)
     end subroutine export_metadata
+
+    ! memory_usage block (export.f90:237-247): device population sets and macro
+    ! fields, plus the host arrays the export reads the fields into
+    subroutine write_memory_usage(u)
+        integer, intent(in) :: u
+        integer(c_int64_t) :: pop_b, mac_b, oth_b
+        real(real64) :: cells, host_b, total
+        call ffb_check(ffb_memory_usage(lat, pop_b, mac_b, oth_b), "ffb_memory_usage")
+        cells = real(s%n_x, real64) * real(s%n_y, real64)
+        host_b = 3.0_real64 * cells * merge(8.0_real64, 4.0_real64, fp64)
+        total = real(pop_b, real64) + real(mac_b, real64) + real(oth_b, real64) + host_b
+        write(u, '(A)') '  "memory_usage": {'
+        write(u, '(A,I0,A)') '    "dist_function_buffers_per_cell_B": ', nint(real(pop_b, real64) / cells, int64), ','
+        write(u, '(A,ES24.16,A)') '    "dist_function_buffers_total_GB": ', real(pop_b, real64) * 1.0e-9_real64, ','
+        write(u, '(A,I0,A)') '    "macro_field_buffers_per_cell_B": ', &
+            nint((real(mac_b, real64) + host_b) / cells, int64), ','
+        write(u, '(A,ES24.16,A)') '    "macro_field_buffers_total_GB": ', &
+            (real(mac_b, real64) + host_b) * 1.0e-9_real64, ','
+        write(u, '(A,I0,A)') '    "total_per_cell_B": ', nint(total / cells, int64), ','
+        write(u, '(A,ES24.16)') '    "total_GB": ', total * 1.0e-9_real64
+        write(u, '(A)') '  },'
+    end subroutine write_memory_usage
+
+    ! JSON string escape of backslash and double quote
+    function json_escape(v) result(o)
+        character(len=*), intent(in) :: v
+        character(len=:), allocatable :: o
+        integer :: k
+        o = ""
+        do k = 1, len_trim(v)
+            select case (v(k:k))
+            case ('"', '\')
+                o = o // '\' // v(k:k)
+            case default
+                o = o // v(k:k)
+            end select
+        end do
+    end function json_escape
 
     subroutine export_fields(st)
         integer(int32), intent(in) :: st

@@ -14,9 +14,18 @@
 ! FFB_STORAGE_SCALE_LOG2 the scale is the library's default for the form and
 ! the mode's speed (ffb_default_scale_log2).
 module ffb_settings
-    use iso_fortran_env, only: int32, int64, real64
+    use iso_fortran_env, only: int32, int64, real32, real64
     implicit none
     private
+
+    ! kind of the compile-time settings literals (1.1_FP, ...), as the
+    ! reference's FP (settings.f90:7-13): real32 unless FFB_FP64, so the values
+    ! a settings build passes are the reference's constants exactly
+#ifdef FFB_FP64
+    integer, parameter :: FP = real64
+#else
+    integer, parameter :: FP = real32
+#endif
 
     public :: run_settings_t, load_settings, mode_name, reynolds_number
     ! storage_scale_log2 not given: the library's default for the form and speed

@@ -330,6 +330,18 @@ int ffb_adjoint_backward(ffb_adjoint_t *adj, const void *grad_u_x, const void *g
 /* Device time of the last forward and backward calls, in milliseconds. */
 int ffb_adjoint_timing(ffb_adjoint_t *adj, double *forward_ms, double *backward_ms);
 
+/* Name ("<marketing name> (<gfx arch>)", truncated to name_len - 1 characters),
+ * compute units and memory of HIP device `device`, for the hardware report the
+ * reference prints from hardware_info.f90:132-145 (any pointer may be NULL). */
+int ffb_device_properties(int32_t device, char *name, int32_t name_len, int32_t *compute_units,
+                          int64_t *memory_bytes);
+
+/* Device bytes the handle holds: both population sets (row padding
+ * included), the macro fields rho, u_x, u_y, and the rest (Poiseuille strips,
+ * profile row, partial sums, status word).  Counterpart of the reference's
+ * memory_usage report (main.f90:109-119, export.f90:237-247). */
+int ffb_memory_usage(ffb_lattice_t *lat, int64_t *population_bytes, int64_t *macro_bytes, int64_t *other_bytes);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *ffb_last_error(void);
 

@@ -142,4 +142,41 @@ def test_adjoint_cost_against_forward():
         a.backward(np.ones_like(ux), np.zeros_like(uy))
         f_ms, b_ms = a.timing()
     print(f"  forward {f_ms:.2f} ms, backward {b_ms:.2f} ms, ratio {b_ms / f_ms:.2f}", flush=True)
-    assert b_ms / f_ms < 4.0
+    assert b_ms / f_ms <= 3.0  # the differentiation form's cost target
+
+
+def test_inverse_channel_512_gradients():
+    """The inverse channel problem at its full size: 512^2 body-force channel,
+    5000 steps, a checkpoint every 100 steps, loss ||u - u_target||^2 with the
+    target from the forward solver at seeded true (omega, g) (seed 0: omega
+    1.43696, g 3.464e-7) and the gradient taken at the perturbed start.  The
+    FP64 adjoint is within 1e-6 of central finite differences of the FP64
+    forward (h = 1e-4 relative); the FP32-deviation adjoint within 5e-3 of the
+    FP64 gradient (measured 9e-4 / 2e-4); plain FP32, whose forward misses the
+    force's few-ulp changes of f ~ w, within 0.2 (measured 7e-2 / 5e-2)."""
+    from fastfortranboltzmann_amd.adjoint import gradient_check
+
+    print("[gpu] inverse channel 512^2 gradient check", flush=True)
+    rng = np.random.Generator(np.random.PCG64(0))
+    t_om, t_g = float(rng.uniform(0.8, 1.8)), float(10 ** rng.uniform(-7, -5))
+    gc = gradient_check(512, 5000, 100, t_om * 0.85 + 0.1, t_g * 1.6, t_om, t_g, rel_h=1e-4)
+    print({k: v for k, v in gc.items() if k.startswith("rel_err")}, flush=True)
+    for k in ("d_omega", "d_force"):
+        assert gc[f"rel_err_fp64_vs_fd_{k}"] < 1e-6
+        assert gc[f"rel_err_fp32dev_vs_fp64_{k}"] < 5e-3
+        assert gc[f"rel_err_fp32_vs_fp64_{k}"] < 0.2
+
+
+def test_inverse_channel_512_adam_recovers_omega_and_force():
+    """The 50 Adam iterations of the inverse channel problem (512^2, 5000 steps,
+    checkpoint every 100, FP32-deviation forward; seed 0, start 0.85 omega + 0.1
+    and 1.6 g) recover omega and g within 5 % (measured 1.3 % and 0.7 %; the
+    loss is nearly flat along omega, whose first Adam steps move it away) and
+    lower the loss by three orders of magnitude."""
+    from fastfortranboltzmann_amd.adjoint import inverse_channel
+
+    print("[gpu] inverse channel 512^2 adam", flush=True)
+    r = inverse_channel(512, 5000, 100, seed=0, iters=50, precision="fp32dev", log=lambda m: print(m, flush=True))
+    print({k: v for k, v in r.items() if k != "history"}, flush=True)
+    assert r["rel_err_omega"] < 0.05 and r["rel_err_g"] < 0.05
+    assert r["loss_final"] < 1e-3 * r["loss_start"]
