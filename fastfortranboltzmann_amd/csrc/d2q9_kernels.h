@@ -282,16 +282,22 @@ template <typename C> struct StepArgs {
     int xcd_group;         // G > 0: G consecutive tiles per XCD (tile_of_block)
     int check_rho;         // FFB_DENSITY_CHECKS: flag densities rho <= 0 (or NaN) in *status
     unsigned *status;      // bit 0: rho <= 0 met, bit 1: rho NaN met
+    int march_rows;        // march2_kernel: output rows per workgroup (one chunk of a strip)
+    int march_inner;       // march2_kernel: 1 = strips away from the x edges pull without edge rules
+    int xcd_rows;          // step2_kernel: H > 1: blocks of xcd_group x H tiles per XCD
+    int n_tiles_y;         // step2_kernel: tile rows
 };
 
 // FFB_DENSITY_CHECKS (simulation.f90:229-233, :133-137,
 // modes/poiseuille_flow.f90:156-160): the reference stops at a density
 // rho <= 0 before dividing by it; here the cell sets bit 0 of the status word
 // (bit 1 for a NaN density, which the reference's test lets pass) and the
-// host reports it (ffb_sync).  A branch on a kernel argument when off.
+// host reports it (ffb_sync).  A branch on a kernel argument when off.  `act`
+// false: a lane that computes a cell only to keep its wave's stores
+// unconditional (march2_kernel), whose result is dropped.
 template <typename C>
-__device__ __forceinline__ void density_check(const StepArgs<C> &a, C rho) {
-    if (a.check_rho) {
+__device__ __forceinline__ void density_check(const StepArgs<C> &a, C rho, bool act = true) {
+    if (a.check_rho && act) {
         if (rho <= (C)0) atomicOr(a.status, 1u);
         else if (rho != rho) atomicOr(a.status, 2u);
     }
@@ -431,16 +437,16 @@ __device__ __forceinline__ void pull_edge_cell(const StepArgs<C> &a, const typen
 // (source/layouts/xyd/simulation.f90:224-293), same operation order.  f[]
 // holds the pulled populations on entry and the post-collision ones on exit.
 template <typename C>
-__device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, int y) {
+__device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, int y, bool act = true) {
     const C f1 = f[0], f2 = f[1], f3 = f[2], f4 = f[3], f5 = f[4], f6 = f[5], f7 = f[6], f8 = f[7], f9 = f[8];
     C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
-    density_check(a, rho);
+    density_check(a, rho, act);
     ux = ux / rho;
     uy = uy / rho;
     const C usq = ux * ux + uy * uy;
-    if (a.write_macro) {
+    if (a.write_macro && act) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
         a.rho_o[m] = rho;
         a.ux_o[m] = ux;
@@ -472,17 +478,17 @@ __device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, 
 // fields written are rho and this u.  No reference counterpart (the
 // reference's Poiseuille flow is pressure-driven).
 template <typename C>
-__device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int x, int y) {
+__device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int x, int y, bool act = true) {
     const C f1 = f[0], f2 = f[1], f3 = f[2], f4 = f[3], f5 = f[4], f6 = f[5], f7 = f[6], f8 = f[7], f9 = f[8];
     const C F = a.force;
     C rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
-    density_check(a, rho);
+    density_check(a, rho, act);
     ux = (ux + (C)0.5 * F) / rho;
     uy = uy / rho;
     const C usq = ux * ux + uy * uy;
-    if (a.write_macro) {
+    if (a.write_macro && act) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
         a.rho_o[m] = rho;
         a.ux_o[m] = ux;
@@ -518,7 +524,7 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
 // The operation order here is the definition of the deviation forms (the oracle
 // restates it, ffbo_q_step).  No reference counterpart.
 template <typename C, int MODE>
-__device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int x, int y) {
+__device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int x, int y, bool act = true) {
     const C inv = a.inv_scale;
     const C a13 = D[1] + D[3], a24 = D[2] + D[4], a57 = D[5] + D[7], a68 = D[6] + D[8];
     const C b13 = D[1] - D[3], b24 = D[2] - D[4], b57 = D[5] - D[7], b68 = D[6] - D[8];
@@ -527,11 +533,11 @@ __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int
     const C jy = (b24 + b57) + b68;
     const C drho = m0 * inv;
     const C rho = (C)1 + drho;
-    density_check(a, rho);
+    density_check(a, rho, act);
     const C r = (C)1 / rho;
     const C ux = (MODE == BF ? __builtin_fmaf(jx, inv, a.half_force) : jx * inv) * r;
     const C uy = (jy * inv) * r;
-    if (a.write_macro) {
+    if (a.write_macro && act) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
         a.rho_o[m] = rho;
         a.ux_o[m] = ux;
@@ -578,10 +584,10 @@ __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int
 }
 
 template <typename S, typename C, int MODE>
-__device__ __forceinline__ void collide_mode(const StepArgs<C> &a, C (&f)[9], int x, int y) {
-    if constexpr (Store<S>::dev) collide_dev<C, MODE>(a, f, x, y);
-    else if (MODE == BF) collide_guo<C>(a, f, x, y);
-    else collide<C>(a, f, x, y);
+__device__ __forceinline__ void collide_mode(const StepArgs<C> &a, C (&f)[9], int x, int y, bool act = true) {
+    if constexpr (Store<S>::dev) collide_dev<C, MODE>(a, f, x, y, act);
+    else if (MODE == BF) collide_guo<C>(a, f, x, y, act);
+    else collide<C>(a, f, x, y, act);
 }
 
 // x offset and y offset of the pull source of direction d: f_d(x - c_d).
@@ -642,6 +648,16 @@ __device__ __forceinline__ void buf_st(R v, __amdgpu_buffer_rsrc_t r, unsigned x
         typedef unsigned u2 __attribute__((ext_vector_type(2)));
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, xb, 0, 0);
     }
+}
+
+// Row descriptor with range checking: num_records 2^31 - 1, so a lane whose
+// byte offset is kDropOffset lies out of range and the hardware drops its store
+// (the range check of raw buffer accesses: offset >= num_records).  Every
+// offset of a real element is below 2^31 (n_x <= 2^24 elements of at most 8
+// bytes).  Lets a wave store unconditionally with some lanes masked.
+constexpr unsigned kDropOffset = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc_checked(const void *row) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), 0, 0x7fffffff, 0x00020000);
 }
 
 // Element xb/sizeof(R) of a row: raw buffer load for 4- and 8-byte elements,
@@ -778,6 +794,44 @@ __device__ __forceinline__ void collide_store(const StepArgs<C> &a, C (&v)[9], t
     }
 #pragma unroll
     for (int d = 0; d < 9; ++d) store_pop<R, ONE_DESC, NT>(pop_out<S>(v[d], d), gr, P, xb, d);
+}
+
+// collide_store for every lane of a wave, without a branch around the stores:
+// lanes with act false compute on whatever they hold, flag no density and
+// write no macro field, and their population stores go to kDropOffset, which
+// the hardware drops.  The stores are then unconditional in the program, so
+// the compiler's count of outstanding memory operations includes them on
+// every path (a store under a branch is counted as possibly absent, and a
+// later wait for loads issued before it then also waits for the stores).
+// y is wave-uniform (the row addresses are scalar); it may lie outside the
+// domain when no lane is active.
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void collide_store_masked(const StepArgs<C> &a, C (&v)[9], typename Store<S>::R *__restrict__ g,
+                                                     int x, int y, bool act) {
+    using R = typename Store<S>::R;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    R *gr = g + (ptrdiff_t)y * W;
+    const unsigned xb = act ? (unsigned)x * (unsigned)sizeof(R) : kDropOffset;
+    collide_mode<S, C, MODE>(a, v, x, y, act);
+    R o[9];
+    encode_cell<S, C>(a, v, x, y, o);
+    // one descriptor for the row, direction d at the scalar offset d P
+    // elements (the nine direction rows of a row lie within 9 plane elements,
+    // below 2^31 bytes for the row-blocked layout; the plane layout keeps a
+    // descriptor per direction)
+    const bool one = a.pitch > a.plane;
+    const __amdgpu_buffer_rsrc_t r0 = row_rsrc_checked(gr);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        const __amdgpu_buffer_rsrc_t r = one ? r0 : row_rsrc_checked(gr + d * P);
+        const unsigned so = one ? (unsigned)(d * P * (ptrdiff_t)sizeof(R)) : 0u;
+        if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, o[d]), r, xb, so, 0);
+        else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, o[d]), r, xb, so, 0);
+        else {
+            typedef unsigned u2 __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, o[d]), r, xb, so, 0);
+        }
+    }
 }
 
 // One cell, any position: pull, collide, store.
@@ -977,11 +1031,295 @@ template <typename S, typename C, int MODE, int WAVES, int RPT, bool NT>
 __global__ void __launch_bounds__(64 * WAVES) step2_kernel(StepArgs<C> a) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
     for (int p = (int)blockIdx.x; p < a.n_blocks; p += (int)gridDim.x) {
-        const int t = tile_of_block(p, a.n_blocks, a.xcd_group);
-        const int ty = t / a.n_tiles_x;
-        pair_tile<S, C, MODE, WAVES, RPT, NT>(a, t - ty * a.n_tiles_x, ty, lds);
+        int tx, ty;
+        if (a.xcd_rows > 1) {
+            // blocks of G x H tiles per XCD: workgroups p of one chunk of 8 G H
+            // map to a band of 8 G tiles by H tile rows, XCD j taking the
+            // G x H block j (row by row), so that the tiles above and below
+            // a tile, which read the same ring rows, share its L2 as the
+            // x-neighbours do; chunks cover the tile grid row-major, tiles
+            // past its edge are skipped
+            const int G = a.xcd_group, H = a.xcd_rows, per = 8 * G * H;
+            const int c = p / per, r = p - c * per, k = r >> 3;
+            const int cx = (a.n_tiles_x + 8 * G - 1) / (8 * G);
+            tx = (c % cx) * 8 * G + (r & 7) * G + k % G;
+            ty = (c / cx) * H + k / G;
+            if (tx >= a.n_tiles_x || ty >= a.n_tiles_y) continue;
+        } else {
+            const int t = tile_of_block(p, a.n_blocks, a.xcd_group);
+            ty = t / a.n_tiles_x;
+            tx = t - ty * a.n_tiles_x;
+        }
+        pair_tile<S, C, MODE, WAVES, RPT, NT>(a, tx, ty, lds);
         if (p + (int)gridDim.x < a.n_blocks) __syncthreads();  // the next tile rewrites the LDS
     }
+}
+
+// Step-1 pulls of a wave of the column march (row-blocked layout) in a strip
+// away from the x edges (every lane's cell has 1 <= x <= nx - 2): nine unconditional loads
+// through row addresses chosen per wave -- the source row of each direction,
+// wrapped on a periodic y axis, or, beyond a wall, the cell's own row of the
+// opposite direction (bounce-back; the moving-wall terms follow in
+// raw_pops) -- so no branch surrounds them (march_pull).  0 <= gy < ny,
+// wave-uniform; xb = x * element size.
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void march_pull_inner(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f,
+                                                 unsigned xb, int gy, RawPop<S> (&raw)[9]) {
+    using R = typename Store<S>::R;
+    const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+    const int ny = a.ny;
+    if constexpr (sizeof(R) >= 4) {
+        // one descriptor per source row (one element before its direction-0
+        // row, so that every offset is >= 0), the direction and the x shift
+        // of the pull in the scalar offset: (d P + 1 - c_x) elements, or, for
+        // a bounce-back, (opp(d) P + 1) elements of the cell's own row
+        __amdgpu_buffer_rsrc_t rs[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int sy = gy + k - 1;
+            if (MODE == SW) sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+            else sy = sy < 0 ? 0 : (sy >= ny ? ny - 1 : sy);  // out of range: not read
+            rs[k] = row_rsrc(f + (ptrdiff_t)sy * W - 1);
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const int sy = gy - cy_of(d);
+            const bool out = MODE != SW && ((sy < 0) | (sy >= ny));
+            const unsigned so = (unsigned)((out ? (ptrdiff_t)opp_of(d) * P + 1 : (ptrdiff_t)d * P + 1 - cx_of(d)) *
+                                           (ptrdiff_t)sizeof(R));
+            const __amdgpu_buffer_rsrc_t r = out ? rs[1] : rs[1 - cy_of(d)];
+            if constexpr (sizeof(R) == 4)
+                raw[d] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, xb, so, 0));
+            else
+                raw[d] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, xb, so, 0));
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            int sy = gy - cy_of(d);
+            const R *row;
+            if (MODE == SW) {
+                sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+                row = f + d * P + (ptrdiff_t)sy * W + pull_dx(d);
+            } else {
+                const bool out = (sy < 0) | (sy >= ny);
+                row = out ? f + opp_of(d) * P + (ptrdiff_t)gy * W : f + d * P + (ptrdiff_t)sy * W + pull_dx(d);
+            }
+            raw[d] = ld_row(row, xb);
+        }
+    }
+}
+
+// The nine stored elements step 1 of the column march pulls into region cell
+// (gx, gy), issued unconditionally: every lane of the wave loads nine elements
+// on either side of one wave-uniform branch, so the compiler counts them on
+// every path (a load under a branch that may be skipped is counted as
+// possibly absent, and a later wait for older loads then also waits for
+// everything issued since).  A wave holding a domain-edge cell, or whose row
+// is not needed (row_act false), takes the edge rule in every lane
+// (pull_edge_cell, which gives the plain pull for an interior cell); lanes
+// that compute nothing (lane_act false) and rows not needed read cell
+// (0, 0), whose values are not used.  Otherwise every lane pulls through the
+// row addresses.  gy and row_act are wave-uniform.
+template <typename S, typename C, int MODE>
+__device__ __forceinline__ void march_pull(const StepArgs<C> &a, const typename Store<S>::R *__restrict__ f, int gx,
+                                           int gy, bool lane_act, bool row_act, RawPop<S> (&raw)[9]) {
+    using R = typename Store<S>::R;
+    const bool edge = lane_act & ((gx == 0) | (gx == a.nx - 1) | (gy == 0) | (gy == a.ny - 1));
+    if (!row_act | (__builtin_amdgcn_ballot_w64(edge) != 0)) {
+        pull_edge_cell<S, C, MODE>(a, f, row_act & lane_act ? gx : 0, row_act ? gy : 0, raw);
+    } else {
+        const ptrdiff_t P = (ptrdiff_t)a.plane, W = (ptrdiff_t)a.pitch;
+        const R *fr = f + (ptrdiff_t)gy * W;  // row gy, direction 0 (wave-uniform); 1 <= gy <= ny - 2 here
+        const unsigned xb = lane_act ? (unsigned)gx * (unsigned)sizeof(R) : (unsigned)sizeof(R);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) raw[d] = ld_row(fr + d * P + pull_dy(d) * W + pull_dx(d), xb);
+    }
+}
+
+// Two steps in one pass by a column march (the modes without strip passes).
+// A workgroup owns a strip of 62 output columns (64 region columns with the
+// one-cell ring in x) and a chunk of a.march_rows output rows [Y0, Y1), and
+// walks down it in bands of B = WAVES RPT rows.  Iteration `it` computes step
+// 1 of the region rows Y0 - 1 + it B + [0, B) into a ring of NR = 2B + 2 LDS
+// rows, then, after one barrier, step 2 of the rows one below them
+// (Y0 - 2 + it B + [0, B), within [Y0, Y1)), whose three step-1 source rows
+// are in the ring: each step-1 row is computed once per chunk (the tile
+// pairs recompute a ring row above and below every tile), and the ring of
+// 2B + 2 rows keeps the rows that step 2 of band `it` reads apart from those
+// step 1 of band it + 1 writes, so one barrier per band suffices.  The HBM
+// loads of band it + 2's step 1 are issued as soon as step 1 of band it has
+// run (two register sets, the loop unrolled by two), so they are in flight
+// while the next one and a half bands compute.
+// Workgroup p takes strip (p / 8) mod n_strips and chunk 8 ((p / 8) /
+// n_strips) + p mod 8: the eight XCDs (workgroup p runs on XCD p mod 8) each
+// take one row chunk of every strip, so x-neighbouring strips, which read
+// each other's edge columns, share an L2.  Boundary rules, random bits and
+// arithmetic are those of pair_tile, so the result is bitwise that of two
+// step_kernel launches.
+// One-row bands are held to 64 VGPRs (8 waves per SIMD): two 16-wave
+// workgroups per CU then fit for FP32 (2 x 78.3 KB of LDS).
+template <typename S, typename C, int MODE, int WAVES, int RPT>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(RPT == 1 ? 8 : 1)))
+march2_kernel(StepArgs<C> a) {
+    static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
+    using R = typename Store<S>::R;
+    constexpr int B = WAVES * RPT, NR = 2 * B + 2;
+    constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
+    // the ring, flat: direction d, ring row k, region column c at
+    // 1 + (d NR + k) 64 + c; one spare element at either end keeps the pulls
+    // of lanes 0 and 63 (columns -1 and 64, results dropped) inside the array
+    __shared__ R ring[9 * NR * 64 + 2];
+    auto lds = [&](int d, int k, int c) -> R & { return ring[1 + (d * NR + k) * 64 + c]; };
+    const int p = (int)blockIdx.x, q = p >> 3;
+    const int sx = q % a.n_tiles_x;
+    const int Y0 = ((q / a.n_tiles_x) * 8 + (p & 7)) * a.march_rows;
+    if (Y0 >= a.ny) return;  // chunks past the last row (grid rounded up to 8 chunks per strip)
+    const int Y1 = min(Y0 + a.march_rows, a.ny);
+    const int lane = (int)threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int nx = a.nx, ny = a.ny;
+    const int ux = sx * 62 - 1 + lane;  // unwrapped region column
+    int gx = ux;
+    bool xin = (ux >= -1) & (ux <= nx);
+    if (PX) gx = ux < 0 ? ux + nx : (ux >= nx ? ux - nx : ux);
+    else xin &= (ux >= 0) & (ux < nx);
+    const bool out_x = (lane >= 1) & (lane <= 62) & (ux < nx);
+    const bool inner_x = (sx * 62 >= 1) & (sx * 62 + 62 <= nx - 1);
+    const R *__restrict__ f = static_cast<const R *>(a.src);
+    R *__restrict__ g = static_cast<R *>(a.dst);
+    StepArgs<C> a1 = a;
+    a1.write_macro = 0;
+    StepArgs<C> a2 = a;
+    a2.step_seed = a.step_seed + 1u;
+    const int last = Y1 - Y0 + 1;                  // step-1 rows l = 0 .. last (row Y0 - 1 + l)
+    const int n_it = (last + 1 + B - 1) / B;
+    // step-1 row of band `it`, band row j of this wave: wrapped row and whether the lane computes it
+    auto row1 = [&](int it, int j, int &gy) {
+        const int l = it * B + w + WAVES * j, uy = Y0 - 1 + l;
+        bool act = xin & (l <= last) & (uy >= -1) & (uy <= ny);
+        gy = uy;
+        if (PY) gy = uy < 0 ? uy + ny : (uy >= ny ? uy - ny : uy);
+        else act &= (uy >= 0) & (uy < ny);
+        return act;
+    };
+    // whether any lane of the wave computes step-1 row (it, j) (wave-uniform)
+    auto row_on = [&](int it, int j) {
+        const int l = it * B + w + WAVES * j, uy = Y0 - 1 + l;
+        return (l <= last) & (PY ? (uy >= -1) & (uy <= ny) : (uy >= 0) & (uy < ny));
+    };
+    // strips whose region lies in 1 <= x <= nx - 2 pull without edge rules in x
+    // (march_pull_inner): one copy of the march per kind, chosen per workgroup
+    const bool inner_strip = (sx >= 1) & (sx * 62 + 64 <= nx) & (a.march_inner != 0);
+    const unsigned xb1 = (unsigned)gx * (unsigned)sizeof(R);
+    auto run = [&](auto inner_tag) {
+        constexpr bool INNER = decltype(inner_tag)::value;
+        // two register sets of raw step-1 inputs, bands alternating between them:
+        // a set is reloaded with band it + 2 as soon as step 1 of band it has
+        // consumed it, so the loads are in flight for one and a half bands, and
+        // the set is never free for the compiler to reuse (a load into registers
+        // that held the data of a store still in flight waits for the store)
+        RawPop<S> raw_a[RPT][9], raw_b[RPT][9];
+        auto load_band = [&](int it, RawPop<S>(&dst)[RPT][9]) {
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                int gy;
+                const bool act = row1(it, j, gy);
+                const bool on = (it < n_it) & row_on(it, j);
+                if constexpr (INNER) march_pull_inner<S, C, MODE>(a1, f, xb1, on ? gy : 0, dst[j]);
+                else march_pull<S, C, MODE>(a1, f, gx, gy, act, on, dst[j]);
+            }
+        };
+        // scheduling barriers keep the loads, the dropped stores and each band's
+        // stores in program order: the counts above assume that order, and the
+        // scheduler would otherwise move stores ahead of loads and loads across
+        // the loop entry
+        load_band(0, raw_a);
+        __builtin_amdgcn_sched_barrier(0);
+        load_band(1, raw_b);
+        __builtin_amdgcn_sched_barrier(0);
+        // Dropped stores after the first bands' loads, as many as step 2 issues
+        // after every later band's loads: the compiler counts outstanding memory
+        // operations at the loop head as the fewest over the paths into it, and
+        // without these its wait for a band's loads would also wait for the
+        // stores issued after them.  Each at its own out-of-range offset
+        // (identical stores would be merged into one).
+        {
+            const __amdgpu_buffer_rsrc_t r = row_rsrc_checked(g);
+#pragma unroll
+            for (int k = 0; k < 9 * RPT; ++k) {
+                const unsigned o = kDropOffset + 256u * (unsigned)k;
+                if constexpr (sizeof(R) == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)0, r, o, 0, 0);
+                else if constexpr (sizeof(R) == 4) __builtin_amdgcn_raw_buffer_store_b32(0u, r, o, 0, 0);
+                else {
+                    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+                    __builtin_amdgcn_raw_buffer_store_b64(u2{0u, 0u}, r, o, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // band `it`: step 1 from cur into the ring, loads of band it + 2 into cur,
+        // barrier, step 2
+        auto band = [&](int it, RawPop<S>(&cur)[RPT][9]) {
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                int gy;
+                if (row1(it, j, gy)) {
+                    C v[9];
+                    raw_pops<S, C, MODE>(a1, cur[j], gy, v);
+                    collide_mode<S, C, MODE>(a1, v, gx, gy);
+                    R o[9];
+                    encode_cell<S, C>(a1, v, gx, gy, o);
+                    const int slot = (it * B + w + WAVES * j) % NR;
+#pragma unroll
+                    for (int d = 0; d < 9; ++d) lds(d, slot, lane) = o[d];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            load_band(it + 2, cur);
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+            // step 2: every lane of every band row computes and stores (collide_store_masked);
+            // the cells outside [Y0, Y1) x [0, nx) and lanes 0, 63 are dropped
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int l = it * B + w + WAVES * j - 1, uy = Y0 - 1 + l;  // step-2 row, ring row l >= -1
+                const bool act = out_x & (uy >= Y0) & (uy < Y1);
+                const int r0 = (l + NR) % NR, rm = (l + NR - 1) % NR, rp = (l + 1) % NR;
+                auto src = [&](int d) { return cy_of(d) > 0 ? rm : (cy_of(d) < 0 ? rp : r0); };
+                C v[9];
+                if (inner_x & (uy >= 1) & (uy <= ny - 2)) {
+#pragma unroll
+                    for (int d = 0; d < 9; ++d) v[d] = pop_in<S>(lds(d, src(d), lane - cx_of(d)), d);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 9; ++d) {
+                        const int sxs = ux - cx_of(d), sys = uy - cy_of(d);
+                        const bool xo = (sxs < 0) | (sxs >= nx), yo = (sys < 0) | (sys >= ny);
+                        const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
+                        const R rw = bb ? lds(opp_of(d), r0, lane) : lds(d, src(d), lane - cx_of(d));
+                        v[d] = pop_in<S>(rw, d);  // opp(d) stores with the offset of d
+                    }
+                    if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
+                        if (uy == ny - 1) {
+                            v[7] = v[7] - a.corr8;
+                            v[8] = v[8] + a.corr9;
+                        }
+                    }
+                }
+                collide_store_masked<S, C, MODE>(a2, v, g, ux, uy, act);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        int it = 0;
+        for (; it + 1 < n_it; it += 2) {
+            band(it, raw_a);
+            band(it + 1, raw_b);
+        }
+        if (it < n_it) band(it, raw_a);
+
+    };
+    if (inner_strip) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 // Poiseuille strip pass (update_poiseuille_flow_macro_strips_PF,

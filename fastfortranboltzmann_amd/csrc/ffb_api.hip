@@ -210,8 +210,16 @@ int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) 
 template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_shape(ffb_lattice *L, StepArgs<C> a) {
     constexpr int ROWS = WAVES * RPT;
     a.n_tiles_x = (a.nx + 61) / 62;
-    a.n_blocks = a.n_tiles_x * ((a.ny + ROWS - 3) / (ROWS - 2));
+    a.n_tiles_y = (a.ny + ROWS - 3) / (ROWS - 2);
+    a.n_blocks = a.n_tiles_x * a.n_tiles_y;
     a.xcd_group = L->pair_xcd_group;
+    a.xcd_rows = a.xcd_group > 0 ? L->pair_xcd_rows : 1;
+    if (a.xcd_rows > 1) {  // whole chunks of 8 G x H tiles
+        const long long G = a.xcd_group, H = a.xcd_rows;
+        const long long n = ((a.n_tiles_x + 8 * G - 1) / (8 * G)) * ((a.n_tiles_y + H - 1) / H) * 8 * G * H;
+        if (n > 0x7fffffffLL) return fail(FFB_E_INVALID, "two-step pass: too many tiles for FFB_PAIR_XCD_ROWS");
+        a.n_blocks = (int)n;
+    }
     // persistent grid of L->pair_grid workgroups (rounded up to a multiple of 8), or one per tile
     const int g = L->pair_grid > 0 ? std::min(a.n_blocks, ((L->pair_grid + 7) / 8) * 8) : a.n_blocks;
     const dim3 grid((unsigned)g), block(64 * WAVES);
@@ -226,6 +234,38 @@ template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_
     return FFB_OK;
 }
 
+// Two steps in one launch by column march (march2_kernel): strips of 62
+// columns, chunks of L->march_rows rows, eight chunks of every strip per
+// group of eight workgroups (one chunk per XCD).
+template <typename S, typename C, int WAVES, int RPT> int launch_march_shape(ffb_lattice *L, StepArgs<C> a) {
+    a.n_tiles_x = (a.nx + 61) / 62;
+    a.march_rows = L->march_rows;
+    a.march_inner = L->march_inner;
+    const long long chunks = (a.ny + a.march_rows - 1) / a.march_rows;
+    const long long g = (long long)a.n_tiles_x * (((chunks + 7) / 8) * 8);
+    if (g > 0x7fffffffLL) return fail(FFB_E_INVALID, "column march: grid too large (raise FFB_MARCH_ROWS)");
+    a.n_blocks = (int)g;
+    const dim3 grid((unsigned)g), block(64 * WAVES);
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((march2_kernel<S, C, SW, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((march2_kernel<S, C, CF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((march2_kernel<S, C, BF, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    case FFB_SLIDING_LID: hipLaunchKernelGGL((march2_kernel<S, C, SL, WAVES, RPT>), grid, block, 0, L->stream, a); break;
+    default: return fail(FFB_E_INVALID, "two-step pass: Poiseuille flow is not supported");
+    }
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+template <typename S, typename C> int launch_march(ffb_lattice *L, const StepArgs<C> &a) {
+    switch (L->march_shape) {
+    case 2: return launch_march_shape<S, C, 8, 2>(L, a);
+    case 3: return launch_march_shape<S, C, 16, 1>(L, a);
+    case 4: return launch_march_shape<S, C, 4, 2>(L, a);
+    default: return launch_march_shape<S, C, 8, 1>(L, a);
+    }
+}
+
 // Region shapes, measured on the 4096^2 and 8192^2 cavities with 4 tiles per
 // XCD group: 4-byte and smaller elements 16 waves x 2 rows (64 x 32 cells, LDS
 // 73.7 KB for FP32; FP32 4096^2 150.5-152.1 us per step against 153.4-154.2
@@ -233,6 +273,7 @@ template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_
 // 219-220 us per step, against 263 with 16 waves x 1 row on another box).
 template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_macro) {
     const StepArgs<C> a = make_args<C>(L, write_macro);
+    if (L->pair_kernel == 1 && L->layout == 1) return launch_march<S, C>(L, a);  // row-blocked layout only
     const bool nt = L->pair_nt != 0;
     if constexpr (sizeof(typename Store<S>::R) <= 4) {
         if (L->pair_shape == 2)
@@ -579,10 +620,18 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // BF16 deviations 8192^2 380 -> 341-342 (8-wave regions), the 1-3-12 form
     // 390 -> 371.  FFB_TWO_STEP=0/1 overrides.
     L->two_step = env_ll("FFB_TWO_STEP", 1) ? 1 : 0;
-    // pair tiles per XCD group: FP32 4096^2 at groups of 2/4/8/16/32/67 tiles
-    // 147.2/132.5/125.3/123.6/125.3/129.5 us per step, 8192^2 groups of 16/32
-    // 493/488; FP64 (default-policy stores) 223.2/220.8/219.6 at 4/16/32
-    L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", 32));
+    // pair tiles per XCD group, by the width of the domain in 62-cell tiles:
+    // 8 up to 96 tiles per row, else 32.  Measured on one box, us per pair
+    // launch: FP32 4096^2 (67 tiles per row) 241 at 8, 243 at 12, 247-249 at
+    // 16 and 32, 250 at 6, 330 without groups; FP64 4096^2 431 at 8, 445-448
+    // at 4, 16, 32; FP32 8192^2 (133 tiles per row) 969 at 32, 984 at 16, 996
+    // at 12, 1018 at 8.  Blocks of G x H tiles per XCD (FFB_PAIR_XCD_ROWS = H)
+    // were slower in every shape tried (FP32 4096^2 317-423 us for 2x4 .. 16x2).
+    {
+        const int tiles_x = (cfg->n_x + 61) / 62;
+        L->pair_xcd_group = (int)std::max(0LL, env_ll("FFB_PAIR_XCD_GROUP", tiles_x <= 96 ? 8 : 32));
+    }
+    L->pair_xcd_rows = (int)std::max(1LL, std::min(64LL, env_ll("FFB_PAIR_XCD_ROWS", 1)));
     // pair region of 4-byte and narrower elements: 2 = 8 waves x 2 rows (64 x 16
     // cells, 62 x 14 outputs), 1 = 16 waves x 2 rows (64 x 32, 62 x 30).  Shape 2
     // for FP32 (4096^2: 248-254 against 259-268 us per pair; 8192^2 995 against
@@ -592,6 +641,18 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->pair_nt = (int)env_ll("FFB_PAIR_NT", 0);
     // pair workgroups: 0 = one per tile, else a persistent grid of this size
     L->pair_grid = (int)std::max(0LL, env_ll("FFB_PAIR_GRID", 0));
+    // step pairs by tiles (step2_kernel, the default) or by column march
+    // (march2_kernel, FFB_PAIR_KERNEL=1): the march computes no ring rows and
+    // keeps the next bands' loads in flight, but its workgroups spread over many
+    // more rows at once and it ran slower on MI355X (FP32 4096^2 292-373 us per
+    // pair launch against 241-247 for the tiles, FP16 8192^2 723-889 against
+    // 765-801); with the collision removed both ran as fast as with it (tiles
+    // 246 against 244 us, march 296 against 292): the pairs are bound by their
+    // memory access pattern, not by the arithmetic or its overlap with loads
+    L->pair_kernel = (int)env_ll("FFB_PAIR_KERNEL", 0) == 1 ? 1 : 0;
+    L->march_shape = (int)env_ll("FFB_MARCH_SHAPE", 1);
+    L->march_rows = (int)std::max(1LL, std::min(1LL << 20, env_ll("FFB_MARCH_ROWS", 126)));
+    L->march_inner = (int)env_ll("FFB_MARCH_INNER", 1) ? 1 : 0;
 
     auto cleanup_fail = [&](hipError_t e, const char *what) {
         ffb_destroy(L);

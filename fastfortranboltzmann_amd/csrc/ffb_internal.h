@@ -48,9 +48,14 @@ struct ffb_lattice {
     int xcd_group = 0;       // G consecutive tiles per XCD, 0: plain order (tile_of_block)
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
     int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
+    int pair_xcd_rows = 1;   // H > 1: blocks of G x H pair tiles per XCD
     int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows
     int pair_nt = 1;         // pair stores: 1 nontemporal, 0 default cache policy
     int pair_grid = 0;       // pair workgroups: 0 = one per tile, else a persistent grid
+    int pair_kernel = 0;     // step pairs: 0 = tiles (step2_kernel), 1 = column march (march2_kernel)
+    int march_shape = 1;     // march band: 1 = 8 waves x 1 row, 2 = 8 x 2, 3 = 16 x 1, 4 = 4 x 2
+    int march_rows = 126;    // march: output rows per workgroup
+    int march_inner = 1;     // march: strips away from the x edges pull without edge rules
     long long plane_pad = 0; // elements added to each plane stride (multiple of 64)
     long long buf_pad = 0;   // elements between the two population sets (multiple of 64)
 };

@@ -6,7 +6,9 @@ than one tile (62 x 14 and 62 x 30 for 4-byte and narrower elements, both
 region shapes; FP64 62 x 6), with odd step counts (the last step of an odd
 count runs alone) and macro output on the last step.  The pair path is the
 default for every form, so the oracle parity tests cover it too;
-FFB_TWO_STEP and FFB_PAIR_SHAPE select the path when a lattice is created."""
+FFB_TWO_STEP, FFB_PAIR_KERNEL (tiles or column march), FFB_PAIR_SHAPE,
+FFB_MARCH_SHAPE and FFB_MARCH_ROWS select the path when a lattice is
+created."""
 import numpy as np
 import pytest
 
@@ -40,7 +42,17 @@ def _random_state(seed, precision, nx, ny):
     return (w[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dt)
 
 
-KINDS = {"shape8x2": {"FFB_PAIR_SHAPE": 2}, "shape16x2": {"FFB_PAIR_SHAPE": 1}}
+KINDS = {"shape8x2": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2},
+         "shape16x2": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 1},
+         # XCD blocks of 2 x 3 tiles (partial chunks at the grid's edges)
+         "xcd2x3": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2, "FFB_PAIR_XCD_GROUP": 2, "FFB_PAIR_XCD_ROWS": 3},
+         # column march: bands of 8 x 1, 8 x 2, 16 x 1 and 4 x 2 rows, chunks of
+         # 5, 16, 40 and 3 rows (several chunks per strip, partial last chunks,
+         # ring rows reused many times, chunks shorter than a band)
+         "march8x1": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 1, "FFB_MARCH_ROWS": 5},
+         "march8x2": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 2, "FFB_MARCH_ROWS": 16},
+         "march16x1": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 3, "FFB_MARCH_ROWS": 40},
+         "march4x2": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 4, "FFB_MARCH_ROWS": 3}}
 
 
 @pytest.mark.parametrize("kind", list(KINDS))
