@@ -4,9 +4,10 @@ Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
-whole domain; the library runs consecutive steps as step pairs, two updates
-per kernel launch (temporal blocking, bitwise the same result), and the line
-also reports one launch per step ("single_pass").  Inputs are resident in
+whole domain; the library runs consecutive steps as step triples, three
+updates per kernel launch (temporal blocking, bitwise the same result; a pair
+or a single step for a remainder), and the line also reports step pairs
+("pairs") and one launch per step ("single_pass").  Inputs are resident in
 HBM before timing.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64|...]
@@ -16,8 +17,9 @@ and the HBM roofline of the kernel that runs the workload.  `roofline.achieved`
 is the bytes one launch must move -- one read and one write of the stored
 state, 18*s bytes per cell, whatever the number of updates the launch
 performs -- over the launch's average duration (HIP events on the library's
-stream); `effective_gbs` is the per-update rate 18*s*cells*steps/time, above
-the peak for step pairs, and not a roofline figure.  `roofline.traffic` is the
+stream, over whole launches); `effective_gbs` is the per-update rate
+18*s*cells*steps/time, above the peak for step pairs and triples, and not a
+roofline figure.  `roofline.traffic` is the
 HBM traffic per launch measured by rocprofv3 (profiles/traffic.txt), reported
 only when that file was written from the kernel sources being benched.  The
 line also carries an FP64 run of the workload, a copy-kernel ceiling, and the
@@ -71,46 +73,74 @@ def kernel_source_hash() -> str:
 
 
 class pass_mode:
-    """Lattices created inside run step pairs in one launch (pairs=True), one
-    launch per step (False), or the library's default (None)."""
+    """Lattices created inside run k steps per launch: 3 (step triples), 2
+    (step pairs), 1 (one launch per step), or the library's default (None)."""
 
-    def __init__(self, pairs: bool | None):
-        self.pairs = pairs
+    KEYS = ("FFB_STEPS_PER_PASS", "FFB_TWO_STEP")
+
+    def __init__(self, k: int | None):
+        self.k = k
 
     def __enter__(self):
-        self.saved = os.environ.get("FFB_TWO_STEP")
-        if self.pairs is not None:
-            os.environ["FFB_TWO_STEP"] = "1" if self.pairs else "0"
+        self.saved = {key: os.environ.get(key) for key in self.KEYS}
+        if self.k is not None:
+            os.environ["FFB_STEPS_PER_PASS"] = str(int(self.k))
+            os.environ["FFB_TWO_STEP"] = "1" if self.k >= 2 else "0"
         return self
 
     def __exit__(self, *exc):
-        if self.saved is None:
-            os.environ.pop("FFB_TWO_STEP", None)
-        else:
-            os.environ["FFB_TWO_STEP"] = self.saved
+        for key, val in self.saved.items():
+            if val is None:
+                os.environ.pop(key, None)
+            else:
+                os.environ[key] = val
 
 
-def uses_pairs(precision: str) -> bool:
-    """Whether the library runs step pairs for this form (FFB_TWO_STEP, default on)."""
+DEFAULT_STEPS_PER_PASS = 3  # the library's default for every form (FFB_STEPS_PER_PASS)
+
+
+def steps_per_pass(precision: str) -> int:
+    """Steps the library computes per launch for this form: FFB_STEPS_PER_PASS
+    (1..3), 1 where FFB_TWO_STEP=0, else the library's default."""
     env = os.environ.get("FFB_TWO_STEP")
     if env is not None:
         try:
-            return int(env) != 0
+            if int(env) == 0:
+                return 1
         except ValueError:
-            return False
-    return precision in BYTES_PER_POP
-
-
-def batch_uses_pairs(precision: str) -> bool:
-    """Whether a batch runs step pairs (FFB_BATCH_PAIRS, default: the 4- and
-    8-byte forms; the 16-bit forms take one launch per step)."""
-    env = os.environ.get("FFB_BATCH_PAIRS")
-    if env:
+            return 1
+    env = os.environ.get("FFB_STEPS_PER_PASS")
+    if env is not None:
         try:
-            return int(env) != 0
+            return max(1, min(3, int(env)))
+        except ValueError:
+            return 1
+    return DEFAULT_STEPS_PER_PASS if precision in BYTES_PER_POP else 1
+
+
+def batch_steps_per_pass(precision: str) -> int:
+    """Steps a batch computes per launch, as ffb_batch_create chooses them:
+    multi-step launches for the 4- and 8-byte forms unless FFB_BATCH_PAIRS
+    says otherwise, three steps each unless FFB_BATCH_STEPS_PER_PASS (1..3)
+    says otherwise (which alone, without FFB_BATCH_PAIRS, also selects
+    multi-step launches for any form); 1 = one launch per step."""
+    pairs_env = os.environ.get("FFB_BATCH_PAIRS")
+    k_env = os.environ.get("FFB_BATCH_STEPS_PER_PASS")
+    pairs = BYTES_PER_POP.get(precision, 0) >= 4
+    if pairs_env:
+        try:
+            pairs = int(pairs_env) != 0
+        except ValueError:
+            pairs = False
+    k = 3 if pairs else 1
+    if k_env:
+        try:
+            k = int(k_env)
         except ValueError:
             pass
-    return BYTES_PER_POP.get(precision, 0) >= 4
+        if not pairs_env:
+            pairs = k >= 2
+    return max(2, min(3, k)) if pairs else 1
 
 
 def hbm_rates(mlups: float, precision: str, per_launch: int) -> dict:
@@ -124,38 +154,44 @@ def hbm_rates(mlups: float, precision: str, per_launch: int) -> dict:
 
 
 def run_gpu(n: int, precision: str, steps: int, warmup: int, scale_log2: int = 0,
-            pairs: bool | None = None) -> dict:
+            per_pass: int | None = None) -> dict:
+    """The cavity n^2 for `steps` steps after `warmup`: MLUPS from the wall
+    clock around the timed steps (inputs resident, synchronised on both
+    sides), and the kernel's average launch duration from HIP events on the
+    library stream over a run of whole launches (a multiple of the steps per
+    launch, so that no remainder launch of another kernel is mixed in)."""
     from fastfortranboltzmann_amd import Lattice, Settings
 
     s = Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=cavity_omega(n=n), U_LID=0.1, RHO_0=1.0,
                  precision=precision, storage_scale_log2=scale_log2)
-    with pass_mode(pairs):
-        paired = uses_pairs(precision)
+    with pass_mode(per_pass):
+        k = steps_per_pass(precision)
         lat = Lattice(s)
     with lat:
         lat.init()
         lat.step(warmup)
         lat.sync()
-        lat.reset_timer()
         t0 = time.perf_counter()
         lat.step(steps)
         lat.sync()
         wall = time.perf_counter() - t0
+        ev_steps = max(k, (steps // k) * k)
+        lat.reset_timer()
+        lat.step(ev_steps)
+        lat.sync()
         ev_ms = lat.elapsed_ms()
     cells = n * n
-    # launches: step pairs (and a single step for an odd count), or one per step
-    launches = steps // 2 + steps % 2 if paired else steps
-    per_launch = 2 if paired and steps % 2 == 0 else 1
+    launches = ev_steps // k
     kernel_ms = ev_ms / launches  # the kernel's average launch duration
     must_move = 18 * BYTES_PER_POP[precision] * cells  # one read + one write of the state
     return {
         "mlups": cells * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms": kernel_ms,
-        "steps_per_launch": per_launch,
+        "steps_per_launch": k,
         "bytes_per_launch": must_move,
         "achieved_gbs": must_move / (kernel_ms * 1e-3) / 1e9,
-        "effective_gbs": must_move * steps / (ev_ms * 1e-3) / 1e9,
+        "effective_gbs": must_move * ev_steps / (ev_ms * 1e-3) / 1e9,
     }
 
 
@@ -194,8 +230,10 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
            "error_case_big": f"cavity {n}^2 Re 5000, {big_err_steps} steps"}
     scale = {f: (default_scale_log2(f) if f.endswith("dev") else 0) for f in forms}
     rates = {f: run_form(cavity_settings(n, 5000.0, 0.1, f, scale[f]), steps, chunk=steps)["mlups"] for f in forms}
-    with pass_mode(False):  # FP32 with one launch per step
+    with pass_mode(1):  # FP32 with one launch per step
         fp32_single = run_form(cavity_settings(n, 5000.0, 0.1, "fp32", 0), steps, chunk=steps)["mlups"]
+    with pass_mode(2):  # FP32 in step pairs
+        fp32_pair = run_form(cavity_settings(n, 5000.0, 0.1, "fp32", 0), steps, chunk=steps)["mlups"]
     ref = run_form(cavity_settings(err_n, 1000.0, 0.1, "fp64"), err_steps, chunk=err_steps)
     ref_big = run_form(cavity_settings(n, 5000.0, 0.1, "fp64"), big_err_steps, chunk=big_err_steps)
     for form in forms:
@@ -204,8 +242,9 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
         rb = run_form(cavity_settings(n, 5000.0, 0.1, form, scale[form]), big_err_steps, chunk=big_err_steps)
         cb = compare(ref_big, rb)
         sw = shear_wave_viscosity(form)
-        per_launch = 2 if uses_pairs(form) else 1
+        per_launch = steps_per_pass(form)
         out[form] = {"mlups": round(rates[form], 1), "speedup_vs_fp32": round(rates[form] / rates["fp32"], 3),
+                     "speedup_vs_fp32_pairs": round(rates[form] / fp32_pair, 3),
                      "speedup_vs_fp32_single_pass": round(rates[form] / fp32_single, 3),
                      "steps_per_launch": per_launch, **hbm_rates(rates[form], form, per_launch),
                      "scale_log2": scale[form], "rel_l2_u_vs_fp64": c["rel_l2_u"],
@@ -217,6 +256,8 @@ def reduced_storage(n: int, steps: int, err_n: int = 1024, err_steps: int = 2000
                      "rounding": "stochastic" if form in ("fp16dev", "bf16dev", "e3m12dev") else "nearest"}
     out["fp64_mass_drift"] = ref["mass_drift"]
     out["fp32_mlups"] = round(rates["fp32"], 1)
+    out["fp32_steps_per_launch"] = steps_per_pass("fp32")
+    out["fp32_pairs_mlups"] = round(fp32_pair, 1)
     out["fp32_single_pass_mlups"] = round(fp32_single, 1)
     return out
 
@@ -251,9 +292,9 @@ def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
             "amplitude_ratio": amps[-1] / 0.05}
 
 
-def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int = 200) -> dict:
+def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int = 202) -> dict:
     """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
-    Couette / body-force channel), autotuned: FP32 (step pairs), and the
+    Couette / body-force channel), autotuned: FP32 (step triples), and the
     1-3-12 and FP16 deviation forms (one launch per step), each with the
     kernel and launch shape that ran, and every lattice's relative L2
     velocity error against an FP64 batch of the same lattices after the same
@@ -262,20 +303,20 @@ def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int 
 
     lats = mixed_batch(0, 512)
     forms = ("fp32", "e3m12dev", "fp16dev")
-    res = batch_study(lats, steps, autotune=True, trial_steps=10, forms=forms, log=lambda m: None)
+    res = batch_study(lats, steps, autotune=True, trial_steps=30, forms=forms, log=lambda m: None)
     out = {"lattices": len(lats), "cells": res["cells"], "steps": res["steps"]}
     for form in forms:
         rec = res["forms"][form]
-        pairs = batch_uses_pairs(form)
+        k = batch_steps_per_pass(form)
         tun = rec["tuning"]
-        if pairs:
+        if k > 1:
             launch = {"kernel": "batch_pair_kernel", "pair_shape": tun["pair_shape"],
                       "pair_group": tun["pair_group"], "pair_grid": tun["pair_grid"]}
         else:
             launch = {"kernel": "batch_step_kernel", "tile_w": tun["tile_w"], "grid_blocks": tun["grid_blocks"]}
         err = rec["rel_l2_u_vs_fp64"]
-        out[form] = {"mlups": round(rec["mlups"], 1), "steps_per_launch": 2 if pairs else 1,
-                     **hbm_rates(rec["mlups"], form, 2 if pairs else 1), "launch": launch,
+        out[form] = {"mlups": round(rec["mlups"], 1), "steps_per_launch": k,
+                     **hbm_rates(rec["mlups"], form, k), "launch": launch,
                      "rel_l2_u_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()}}
     out["mlups"] = out["fp32"]["mlups"]
     out["vs_single_4096_fp32"] = round(out["fp32"]["mlups"] / single_fp32_mlups, 3)
@@ -314,18 +355,19 @@ def copy_ceiling(nbytes: int = 1 << 30, iters: int = 20) -> float | None:
 # measured HBM traffic (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
 # ---------------------------------------------------------------------------
 
-def _profile_lattice(n: int, precision: str, pairs: bool, launches: int, scale: int = 0):
+def _profile_lattice(n: int, precision: str, per_pass: int | None, launches: int, scale: int = 0):
     from fastfortranboltzmann_amd import Lattice, Settings
 
     s = Settings(N_X=n, N_Y=n, SIM_MODE=4, OMEGA=cavity_omega(re=1000.0 if n == 4096 else 5000.0, n=n),
                  U_LID=0.1, RHO_0=1.0, precision=precision, storage_scale_log2=scale)
-    with pass_mode(pairs):
+    with pass_mode(per_pass):
+        k = steps_per_pass(precision)
         lat = Lattice(s)
     with lat:
         lat.init()
         lat.sync()
         copy_ceiling(MARK_START, 1)  # the dispatches up to the end marker are this workload's
-        lat.step(launches * (2 if pairs else 1))
+        lat.step(launches * k)
         lat.sync()
         copy_ceiling(MARK_END, 1)
 
@@ -333,11 +375,13 @@ def _profile_lattice(n: int, precision: str, pairs: bool, launches: int, scale: 
 def _profile_batch(precision: str, launches: int):
     from fastfortranboltzmann_amd.batch import Batch, mixed_batch
 
+    # the default tuning (an autotuned shape may differ between the two
+    # counter passes, whose segments must run the same kernel)
     with Batch(mixed_batch(0, 512), precision) as b:
-        b.autotune(10)
+        b.init()
         b.sync()
         copy_ceiling(MARK_START, 1)
-        b.step(launches * (2 if batch_uses_pairs(precision) else 1))
+        b.step(launches * batch_steps_per_pass(precision))
         b.sync()
         copy_ceiling(MARK_END, 1)
 
@@ -347,14 +391,16 @@ def _profile_workloads():
     from fastfortranboltzmann_amd.precision import default_scale_log2
 
     k = 6
-    out = [("sl4096_fp32", lambda: _profile_lattice(4096, "fp32", True, k)),
-           ("sl4096_fp32_single_pass", lambda: _profile_lattice(4096, "fp32", False, k)),
-           ("sl4096_fp64", lambda: _profile_lattice(4096, "fp64", True, k)),
-           ("sl4096_fp64_single_pass", lambda: _profile_lattice(4096, "fp64", False, k))]
+    out = [("sl4096_fp32", lambda: _profile_lattice(4096, "fp32", None, k)),
+           ("sl4096_fp32_pair", lambda: _profile_lattice(4096, "fp32", 2, k)),
+           ("sl4096_fp32_single_pass", lambda: _profile_lattice(4096, "fp32", 1, k)),
+           ("sl4096_fp64", lambda: _profile_lattice(4096, "fp64", None, k)),
+           ("sl4096_fp64_pair", lambda: _profile_lattice(4096, "fp64", 2, k)),
+           ("sl4096_fp64_single_pass", lambda: _profile_lattice(4096, "fp64", 1, k))]
     for form in ("fp32dev", "fp16dev", "bf16dev", "e3m12dev"):
         s = default_scale_log2(form)
-        out.append((f"sl8192_{form}", lambda form=form, s=s: _profile_lattice(8192, form, True, k, s)))
-    out.append(("sl8192_fp32", lambda: _profile_lattice(8192, "fp32", True, k)))
+        out.append((f"sl8192_{form}", lambda form=form, s=s: _profile_lattice(8192, form, None, k, s)))
+    out.append(("sl8192_fp32", lambda: _profile_lattice(8192, "fp32", None, k)))
     for form in ("fp32", "e3m12dev", "fp16dev"):
         out.append((f"batch512_{form}", lambda form=form: _profile_batch(form, k)))
     return out
@@ -599,17 +645,25 @@ def main() -> int:
           file=sys.stderr, flush=True)
     main_run = run_gpu(n, args.precision, args.steps, args.warmup)
     print("bench: one launch per step", file=sys.stderr, flush=True)
-    single = run_gpu(n, args.precision, args.steps, args.warmup, pairs=False)
+    single = run_gpu(n, args.precision, args.steps, args.warmup, per_pass=1)
     workload = f"sl{n}_{args.precision}"
     extra = {"single_pass": {"mlups": round(single["mlups"], 1), "kernel_ms": round(single["kernel_ms"], 5),
                              "roofline": roofline(single, workload + "_single_pass")}}
+    if main_run["steps_per_launch"] != 2:
+        print("bench: step pairs", file=sys.stderr, flush=True)
+        pair = run_gpu(n, args.precision, args.steps, args.warmup, per_pass=2)
+        extra["pairs"] = {"mlups": round(pair["mlups"], 1), "kernel_ms": round(pair["kernel_ms"], 5),
+                          "roofline": roofline(pair, workload + "_pair")}
     if not args.no_fp64 and args.precision != "fp64":
         print("bench: fp64 run", file=sys.stderr, flush=True)
         r64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup)
-        s64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup, pairs=False)
+        s64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup, per_pass=1)
+        p64 = run_gpu(n, "fp64", max(2, args.steps // 2), args.warmup, per_pass=2)
         extra["fp64"] = {"mlups": round(r64["mlups"], 1), "kernel_ms": round(r64["kernel_ms"], 5),
                          "effective_gbs": round(r64["effective_gbs"], 1),
                          "roofline": roofline(r64, f"sl{n}_fp64"),
+                         "pairs": {"mlups": round(p64["mlups"], 1), "kernel_ms": round(p64["kernel_ms"], 5),
+                                   "roofline": roofline(p64, f"sl{n}_fp64_pair")},
                          "single_pass": {"mlups": round(s64["mlups"], 1), "kernel_ms": round(s64["kernel_ms"], 5),
                                          "roofline": roofline(s64, f"sl{n}_fp64_single_pass")}}
     if not args.no_extras and args.precision == "fp32":

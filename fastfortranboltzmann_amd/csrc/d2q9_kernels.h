@@ -1024,6 +1024,155 @@ __device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
     }
 }
 
+// K steps in one pass (K = 3: step triples), the modes without strip
+// passes.  A workgroup of WAVES waves owns an output tile of
+// (64 - 2(K-1)) x (WAVES RPT - 2(K-1)) cells and computes step 1 on the tile
+// plus a ring of K-1 cells (a region of 64 x WAVES RPT cells; wave w takes
+// region rows w, w + WAVES, ...), step k on the tile plus a ring of K-k
+// cells, and step K on the tile.  Step 1 pulls from HBM into LDS (stored
+// representation, as pair_tile); every later step pulls from LDS, with the
+// boundary rules of pull_edge for sources beyond a wall (the cell's own
+// opposite population, plus the moving-wall term on the top row); on a
+// periodic axis the ring cells are the wrapped cells, whose values at every
+// step are those the reference's halo copies would hold.  An intermediate
+// step keeps its results in registers until every wave has read the LDS
+// (barrier), then writes them over its sources (barrier).  Step k runs with
+// a.step_seed + k - 1; step K writes the macro fields when asked.  Every cell
+// update is the single step's arithmetic, so the result is bitwise that of K
+// step_kernel launches.
+template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
+__device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
+                                           typename Store<S>::R (&lds)[9][WAVES * RPT][64]) {
+    static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
+    static_assert(K >= 2 && K <= 3, "two or three steps per pass");
+    using R = typename Store<S>::R;
+    constexpr int ROWS = WAVES * RPT, RING = K - 1, OW = 64 - 2 * RING, OH = ROWS - 2 * RING;
+    constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
+    const int lane = (int)threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int nx = a.nx, ny = a.ny;
+    const int ux = tx * OW - RING + lane;  // unwrapped region column
+    const int uy0 = ty * OH - RING;        // unwrapped first region row
+    const typename Store<S>::R *__restrict__ f = static_cast<const R *>(a.src);
+    R *__restrict__ g = static_cast<R *>(a.dst);
+    // wrap a coordinate at most RING cells outside [0, n) (n >= 1 may be smaller than RING)
+    auto wrap = [](int u, int n) {
+#pragma unroll
+        for (int k = 0; k < RING; ++k) u = u < 0 ? u + n : (u >= n ? u - n : u);
+        return u;
+    };
+    // whether region cell (u, v) takes part in step k: within K - k of the tile's
+    // cells in the domain, and inside the domain on a wall axis
+    auto live = [&](int u, int v, int k) {
+        const int e = K - k;
+        bool in = PX ? (u >= -e) & (u <= nx - 1 + e) : (u >= 0) & (u < nx);
+        in &= PY ? (v >= -e) & (v <= ny - 1 + e) : (v >= 0) & (v < ny);
+        return in;
+    };
+    const int gx = PX ? wrap(ux, nx) : ux;
+    // step 1 on the region
+    {
+        StepArgs<C> a1 = a;
+        a1.write_macro = 0;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = w + WAVES * j, uy = uy0 + r;
+            if (live(ux, uy, 1)) {
+                const int gy = PY ? wrap(uy, ny) : uy;
+                C v[9];
+                pull_pops<S, C, MODE>(a1, f, gx, gy, v);
+                collide_mode<S, C, MODE>(a1, v, gx, gy);
+                R o[9];
+                encode_cell<S, C>(a1, v, gx, gy, o);
+#pragma unroll
+                for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[d];
+            }
+        }
+    }
+    __syncthreads();
+    // A region inside the domain on every wall axis (most tiles) has no
+    // later-step pull that crosses a wall: every source of a step-k cell lies
+    // in the region, and a cell on a wall row is on the region's edge, which
+    // no later step computes.  Such a tile pulls without the boundary rules (a
+    // branch uniform over the workgroup); on a periodic axis the ring cells
+    // hold the wrapped values, so no rule is needed there either.
+    const int x0 = tx * OW - RING;
+    const bool inner = (PX || ((x0 >= 0) & (x0 + 63 <= nx - 1))) && (PY || ((uy0 >= 0) & (uy0 + ROWS - 1 <= ny - 1)));
+    // the values a later step pulls into cell (lane, r) at (ux, uy) from the LDS
+    auto pull_lds = [&](int r, int uy, C (&v)[9]) {
+        if (inner) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) v[d] = pop_in<S>(lds[d][r - cy_of(d)][lane - cx_of(d)], d);
+            return;
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const int sx = ux - cx_of(d), sy = uy - cy_of(d);
+            const bool xo = (sx < 0) | (sx >= nx), yo = (sy < 0) | (sy >= ny);
+            const bool bb = MODE == SW ? false : (MODE == SL ? (xo | yo) : yo);
+            const R raw = bb ? lds[opp_of(d)][r][lane] : lds[d][r - cy_of(d)][lane - cx_of(d)];
+            v[d] = pop_in<S>(raw, d);  // opp(d) stores with the offset of d
+        }
+        if (MODE == CF || MODE == SL) {  // moving top wall (pull_edge)
+            if (uy == ny - 1) {
+                v[7] = v[7] - a.corr8;
+                v[8] = v[8] + a.corr9;
+            }
+        }
+    };
+    // intermediate steps 2 .. K-1, in place in the LDS
+#pragma unroll
+    for (int k = 2; k < K; ++k) {
+        StepArgs<C> ak = a;
+        ak.write_macro = 0;
+        ak.step_seed = a.step_seed + (unsigned)(k - 1);
+        R o[RPT][9];
+        bool on[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = w + WAVES * j, uy = uy0 + r;
+            on[j] = (lane >= k - 1) & (lane <= 64 - k) & (r >= k - 1) & (r <= ROWS - k) & live(ux, uy, k);
+            if (on[j]) {
+                const int gy = PY ? wrap(uy, ny) : uy;
+                C v[9];
+                pull_lds(r, uy, v);
+                collide_mode<S, C, MODE>(ak, v, gx, gy);
+                encode_cell<S, C>(ak, v, gx, gy, o[j]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = w + WAVES * j;
+            if (on[j]) {
+#pragma unroll
+                for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[j][d];
+            }
+        }
+        __syncthreads();
+    }
+    // step K on the tile, to HBM
+    StepArgs<C> aK = a;
+    aK.step_seed = a.step_seed + (unsigned)(K - 1);
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int r = w + WAVES * j, uy = uy0 + r;
+        if ((lane >= RING) & (lane <= 63 - RING) & (r >= RING) & (r <= ROWS - 1 - RING) & (ux < nx) & (uy < ny)) {
+            C v[9];
+            pull_lds(r, uy, v);
+            collide_store<S, C, MODE, false, false>(aK, v, g, ux, uy);
+        }
+    }
+}
+
+// K steps per launch over tiles (multi_tile), XCD-grouped as step2_kernel.
+template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
+__global__ void __launch_bounds__(64 * WAVES) stepk_kernel(StepArgs<C> a) {
+    __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
+    const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
+    const int ty = t / a.n_tiles_x;
+    multi_tile<S, C, MODE, WAVES, RPT, K>(a, t - ty * a.n_tiles_x, ty, lds);
+}
+
 // a.n_blocks tiles; a grid smaller than that walks them (persistent
 // workgroups, grid a multiple of 8 so that every tile of a block lies on the
 // block's XCD in the grouped order).

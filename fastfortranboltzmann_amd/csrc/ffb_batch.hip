@@ -1,18 +1,18 @@
 // ffb_batch.hip -- batched form: many independent lattices of uneven size,
 // each with its own omega and boundary kind, advanced together.  Consecutive
-// steps run as step pairs, two steps per launch of one persistent kernel over
-// the tiles of every lattice (one launch per step is the option, and runs an
-// odd remainder); the launches of a ping-pong cycle are captured once in a
-// hipGraph and replayed.  A project-written autotuner picks the launch shape
+// steps run as step triples (or pairs), three (two) steps per launch of one
+// kernel over the tiles of every lattice (one launch per step is the option,
+// and runs a remainder); the launches of a ping-pong cycle are captured once
+// in a hipGraph and replayed.  A project-written autotuner picks the launch shape
 // of the kernel the batch runs by timing the candidates on the batch.  No
 // reference counterpart (the reference runs one domain per process).
 //
 // Layout: every lattice keeps the single-domain row-blocked layout (per row
 // the nine direction rows, each n_x rounded up to 64 elements; no halo) inside
 // two pools (ping-pong); a descriptor table holds each lattice's offsets and
-// parameters.  Step pairs: a tile table names (lattice, tile) for every
-// 62 x 14 (or 62 x 30; FP64 62 x 6) output tile, lattice by lattice, grouped by
-// kind.  One launch per step: work is cut into row segments of 64 cells (one
+// parameters.  Multi-step launches: a tile table names (lattice, tile) for
+// every output tile (pairs 62 x 14 or 62 x 30, FP64 62 x 6; triples 60 x 12 or
+// 60 x 28, FP64 60 x 12), lattice by lattice, grouped by kind.  One launch per step: work is cut into row segments of 64 cells (one
 // wave each, so every load of a wave is one aligned stretch of one row);
 // segs[t] names the lattice and row of segment t, and a workgroup of TW
 // threads takes TW/64 consecutive segments per iteration of its persistent
@@ -180,11 +180,12 @@ __global__ void __launch_bounds__(TW)
     }
 }
 
-// Step pairs of every lattice: workgroups walk the tile table (pair_tile:
-// step 1 on the tile and its one-cell ring into LDS, step 2 of the tile from
-// LDS), a persistent grid (a multiple of 8 workgroups) or one workgroup per
+// K steps (pairs, K = 2, or triples, K = 3) of every lattice: workgroups walk
+// the tile table (pair_tile: step 1 on the tile and its one-cell ring into
+// LDS, step 2 of the tile from LDS; multi_tile for three steps, a two-cell
+// ring), a persistent grid (a multiple of 8 workgroups) or one workgroup per
 // tile, the tiles mapped to workgroups in XCD groups of G (tile_of_block).
-template <typename S, typename C, int WAVES, int RPT, bool NT>
+template <typename S, typename C, int WAVES, int RPT, bool NT, int K>
 __global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, const PTile *tiles, int n, int G) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
     for (int p = (int)blockIdx.x; p < n; p += (int)gridDim.x) {
@@ -192,11 +193,20 @@ __global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, 
         const PTile pt = load_uniform(tiles + t);
         const BatchLat<C> L = load_uniform(b.lats + pt.lat);
         StepArgs<C> a = lattice_args<S, C>(b, L);
-        switch (L.kind) {  // uniform per workgroup
-        case SW: pair_tile<S, C, SW, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-        case CF: pair_tile<S, C, CF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-        case SL: pair_tile<S, C, SL, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
-        default: pair_tile<S, C, BF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+        if constexpr (K == 2) {
+            switch (L.kind) {  // uniform per workgroup
+            case SW: pair_tile<S, C, SW, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+            case CF: pair_tile<S, C, CF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+            case SL: pair_tile<S, C, SL, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+            default: pair_tile<S, C, BF, WAVES, RPT, NT>(a, pt.tx, pt.ty, lds); break;
+            }
+        } else {
+            switch (L.kind) {
+            case SW: multi_tile<S, C, SW, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
+            case CF: multi_tile<S, C, CF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
+            case SL: multi_tile<S, C, SL, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
+            default: multi_tile<S, C, BF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
+            }
         }
         if (p + (int)gridDim.x < n) __syncthreads();  // the next tile rewrites the LDS
     }
@@ -259,10 +269,11 @@ struct ffb_batch {
     void *lats = nullptr;     // BatchLat<C>[count]
     void *segs = nullptr;     // Seg per row segment: (lattice, row)
     int n_tiles = 0;
-    bool pairs = false;       // step pairs in one launch (batch_pair_kernel)
-    void *ptiles = nullptr;   // PTile per step-pair tile (capacity ptile_cap)
+    bool pairs = false;       // two or three steps per launch (batch_pair_kernel)
+    int per_pass = 1;         // steps per launch: 1, 2 (pairs) or 3 (triples)
+    void *ptiles = nullptr;   // PTile per multi-step tile (capacity ptile_cap)
     int n_ptiles = 0, ptile_cap = 0;
-    int ptile_shape = 0;      // pair_shape the tile table holds
+    int ptile_shape = 0;      // 4 pair_shape + steps per launch of the tile table
     int cus = 0;              // compute units of the device
     ffb_batch_tuning_t tuning{};
     hipStream_t stream = nullptr;
@@ -270,7 +281,7 @@ struct ffb_batch {
     bool timing = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    bool graph_pairs = false; // B->pairs when the graph was captured
+    int graph_pass = 0;       // B->per_pass when the graph was captured
     long long steps = 0;
 };
 
@@ -284,9 +295,13 @@ int grid_for(const ffb_batch *B) {
     return (g <= 0 || g > all) ? all : g;
 }
 
-// output rows of a step-pair tile for a region shape: 62 x 14 (8 waves x 2
-// rows), 62 x 30 (16 waves x 2 rows); FP64 62 x 6 (8 waves x 1 row)
-int pair_out_rows(const ffb_batch *B, int shape) { return B->elem == 8 ? 6 : (shape == 1 ? 30 : 14); }
+// region rows of a multi-step tile: 16 (8 waves x 2 rows), 32 (16 waves x 2
+// rows, shape 1); FP64 pairs 8 (8 waves x 1 row), FP64 triples 16.  Output
+// tiles of K steps are (64 - 2(K-1)) x (rows - 2(K-1)): pairs 62 x 14 / 62 x 30
+// (FP64 62 x 6), triples 60 x 12 / 60 x 28 (FP64 60 x 12).
+int region_rows(const ffb_batch *B, int shape, int k) {
+    return B->elem == 8 ? (k == 3 ? 16 : 8) : (shape == 1 ? 32 : 16);
+}
 
 // lattices in table order: grouped by kind (stable), so resident waves share a code path
 std::vector<int> lattice_order(const ffb_batch *B) {
@@ -298,13 +313,13 @@ std::vector<int> lattice_order(const ffb_batch *B) {
 }
 
 // Step-pair tiles of a region shape, lattice by lattice in table order.
-std::vector<PTile> pair_tiles(const ffb_batch *B, int shape) {
+std::vector<PTile> pair_tiles(const ffb_batch *B, int shape, int k) {
     std::vector<PTile> ptiles;
-    const int out_rows = pair_out_rows(B, shape);
+    const int out_rows = region_rows(B, shape, k) - 2 * (k - 1), out_w = 64 - 2 * (k - 1);
     for (int i : lattice_order(B)) {
         const ffb_batch_lattice_t &d = B->desc[(size_t)i];
         for (int ty = 0; ty < (d.n_y + out_rows - 1) / out_rows; ++ty)
-            for (int tx = 0; tx < (d.n_x + 61) / 62; ++tx) ptiles.push_back(PTile{i, tx, ty, 0});
+            for (int tx = 0; tx < (d.n_x + out_w - 1) / out_w; ++tx) ptiles.push_back(PTile{i, tx, ty, 0});
     }
     return ptiles;
 }
@@ -357,7 +372,10 @@ template <typename C> int upload_tables(ffb_batch *B) {
         }
         L.half_force = (C)0.5 * L.force;
     }
-    B->ptile_cap = (int)std::max(pair_tiles(B, 1).size(), pair_tiles(B, 2).size());
+    size_t cap = 0;
+    for (int k = 2; k <= 3; ++k)
+        for (int sh = 1; sh <= 2; ++sh) cap = std::max(cap, pair_tiles(B, sh, k).size());
+    B->ptile_cap = (int)cap;
     BHIPCHK(hipMalloc(&B->ptiles, (size_t)B->ptile_cap * sizeof(PTile)));
     BHIPCHK(hipMalloc(&B->lats, lats.size() * sizeof(BatchLat<C>)));
     BHIPCHK(hipMalloc(&B->segs, tiles.size() * sizeof(Seg)));
@@ -379,15 +397,15 @@ int drop_graph(ffb_batch *B) {
 // Writes the tile list of B->tuning.pair_shape into the tile table (in place,
 // synchronously; the graph, whose launches read the table, is dropped first).
 int write_pair_tiles(ffb_batch *B) {
-    const int shape = B->tuning.pair_shape;
-    if (B->ptile_shape == shape) return FFB_OK;
+    const int shape = B->tuning.pair_shape, k = std::max(2, B->per_pass);
+    if (B->ptile_shape == 4 * shape + k) return FFB_OK;
     drop_graph(B);
     BHIPCHK(hipStreamSynchronize(B->stream));
-    const std::vector<PTile> pt = pair_tiles(B, shape);
+    const std::vector<PTile> pt = pair_tiles(B, shape, k);
     if ((int)pt.size() > B->ptile_cap) return fail(FFB_E_STATE, "ffb_batch: tile table too small");
     BHIPCHK(hipMemcpy(B->ptiles, pt.data(), pt.size() * sizeof(PTile), hipMemcpyHostToDevice));
     B->n_ptiles = (int)pt.size();
-    B->ptile_shape = shape;
+    B->ptile_shape = 4 * shape + k;
     return FFB_OK;
 }
 
@@ -443,37 +461,46 @@ void launch_step(const ffb_batch *B, int parity, int write_macro) {
     count_steps(B, 1u);
 }
 
-// One launch of step pairs (two steps) of every lattice.
-template <typename S, typename C, int WAVES, int RPT, bool NT>
+// One launch of K steps (pairs or triples) of every lattice.
+template <typename S, typename C, int WAVES, int RPT, bool NT, int K>
 void launch_pair_c(const ffb_batch *B, int parity, int write_macro) {
     const BatchArgs<C> b = batch_args<C>(B, parity, write_macro);
     const int n = B->n_ptiles, pg = B->tuning.pair_grid;
     const int grid = pg > 0 ? std::min(n, ((pg + 7) / 8) * 8) : n;
-    hipLaunchKernelGGL((batch_pair_kernel<S, C, WAVES, RPT, NT>), dim3((unsigned)grid), dim3(64 * WAVES), 0, B->stream,
-                       b, static_cast<const PTile *>(B->ptiles), n, B->tuning.pair_group);
+    hipLaunchKernelGGL((batch_pair_kernel<S, C, WAVES, RPT, NT, K>), dim3((unsigned)grid), dim3(64 * WAVES), 0,
+                       B->stream, b, static_cast<const PTile *>(B->ptiles), n, B->tuning.pair_group);
 }
 
-template <typename S, typename C> void launch_pair_shape(const ffb_batch *B, int parity, int write_macro) {
-    if (B->tuning.pair_shape == 1) launch_pair_c<S, C, 16, 2, false>(B, parity, write_macro);
-    else launch_pair_c<S, C, 8, 2, false>(B, parity, write_macro);
+template <typename S, typename C, int K> void launch_pair_shape(const ffb_batch *B, int parity, int write_macro) {
+    if (B->tuning.pair_shape == 1) launch_pair_c<S, C, 16, 2, false, K>(B, parity, write_macro);
+    else launch_pair_c<S, C, 8, 2, false, K>(B, parity, write_macro);
 }
 
-// Default-policy stores as the single domain's pairs (FP64: nontemporal);
-// regions 8 waves x 2 rows or 16 waves x 2 rows, FP64 8 waves x 1 row.
-void launch_pair(const ffb_batch *B, int parity, int write_macro) {
+template <int K> void launch_pair_k(const ffb_batch *B, int parity, int write_macro) {
     switch (B->precision) {
-    case FFB_PREC_FP64: launch_pair_c<double, double, 8, 1, true>(B, parity, write_macro); break;
-    case FFB_PREC_FP32_DEV: launch_pair_shape<f32dev, float>(B, parity, write_macro); break;
-    case FFB_PREC_FP16_DEV: launch_pair_shape<__half, float>(B, parity, write_macro); break;
-    case FFB_PREC_BF16_DEV: launch_pair_shape<__hip_bfloat16, float>(B, parity, write_macro); break;
-    case FFB_PREC_E3M12_DEV: launch_pair_shape<e3m12, float>(B, parity, write_macro); break;
-    default: launch_pair_shape<float, float>(B, parity, write_macro); break;
+    case FFB_PREC_FP64:
+        if constexpr (K == 2) launch_pair_c<double, double, 8, 1, true, 2>(B, parity, write_macro);
+        else launch_pair_c<double, double, 8, 2, false, K>(B, parity, write_macro);
+        break;
+    case FFB_PREC_FP32_DEV: launch_pair_shape<f32dev, float, K>(B, parity, write_macro); break;
+    case FFB_PREC_FP16_DEV: launch_pair_shape<__half, float, K>(B, parity, write_macro); break;
+    case FFB_PREC_BF16_DEV: launch_pair_shape<__hip_bfloat16, float, K>(B, parity, write_macro); break;
+    case FFB_PREC_E3M12_DEV: launch_pair_shape<e3m12, float, K>(B, parity, write_macro); break;
+    default: launch_pair_shape<float, float, K>(B, parity, write_macro); break;
     }
-    count_steps(B, 2u);
 }
 
-// Captures the launches of one ping-pong cycle (parity 0 then 1): two pair
-// launches (four steps) or two single steps.
+// One launch of B->per_pass (2 or 3) steps.  Default-policy stores as the
+// single domain's pairs (FP64 pairs: nontemporal); regions 8 waves x 2 rows or
+// 16 waves x 2 rows, FP64 pairs 8 waves x 1 row, FP64 triples 8 waves x 2 rows.
+void launch_pair(const ffb_batch *B, int parity, int write_macro) {
+    if (B->per_pass >= 3) launch_pair_k<3>(B, parity, write_macro);
+    else launch_pair_k<2>(B, parity, write_macro);
+    count_steps(B, (unsigned)B->per_pass);
+}
+
+// Captures the launches of one ping-pong cycle (parity 0 then 1): two
+// multi-step launches (four or six steps) or two single steps.
 int build_graph(ffb_batch *B) {
     drop_graph(B);
     BHIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -489,13 +516,13 @@ int build_graph(ffb_batch *B) {
     if (e != hipSuccess) return fail(FFB_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     B->graph = g;
     BHIPCHK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
-    B->graph_pairs = B->pairs;
+    B->graph_pass = B->per_pass;
     return FFB_OK;
 }
 
 // The graph for the current tuning and pass mode, captured if missing or stale.
 int ensure_graph(ffb_batch *B) {
-    if (B->graph_exec && B->graph_pairs == B->pairs) return FFB_OK;
+    if (B->graph_exec && B->graph_pass == B->per_pass) return FFB_OK;
     return build_graph(B);
 }
 
@@ -601,9 +628,16 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     // in pairs against 83,200; FP32 deviations 127,000 / 83,700; FP64 63,000 /
     // 39,100; FP16 127,000 in pairs against 144,100, BF16 129,000 / 141,900,
     // 1-3-12 120,500 / 138,200); FFB_BATCH_PAIRS=0/1 at creation overrides
+    // and step triples (FFB_BATCH_STEPS_PER_PASS = 1, 2, 3; default 3 for the
+    // 4- and 8-byte forms when pairs are on)
     {
         const char *e = std::getenv("FFB_BATCH_PAIRS");
         B->pairs = e && *e ? std::atoi(e) != 0 : B->elem >= 4;
+        const char *k = std::getenv("FFB_BATCH_STEPS_PER_PASS");
+        int kk = k && *k ? std::atoi(k) : (B->pairs ? 3 : 1);
+        if (!(e && *e) && k && *k) B->pairs = kk >= 2;  // the steps per pass alone selects
+        kk = std::max(1, std::min(3, kk));
+        B->per_pass = B->pairs ? std::max(2, kk) : 1;
     }
     B->desc.assign(lattices, lattices + count);
     long long off = 0, moff = 0, poff = 0;
@@ -724,26 +758,27 @@ int ffb_batch_step(ffb_batch_t *B, int32_t n_steps, int32_t write_macro_last) {
     int left = n_steps;
     // the last step runs outside the graph when it writes the macro fields
     const int tail = write_macro_last ? 1 : 0;
-    if (left - tail >= (B->pairs ? 4 : 2)) {
+    const int k = B->per_pass;
+    if (left - tail >= 2 * k) {
         int rc = ensure_graph(B);
         if (rc) return rc;
     }
     if (B->pairs) {
-        // pair launches (two steps each); the captured graph holds two of them
-        // (parity 0 then 1, four steps)
-        if (B->cur == 1 && left - tail >= 2) {
+        // launches of k steps each; the captured graph holds two of them
+        // (parity 0 then 1, 2k steps); a remainder below k runs as single steps
+        if (B->cur == 1 && left - tail >= k) {
             launch_pair(B, 1, 0);
             B->cur = 0;
-            left -= 2;
+            left -= k;
         }
-        while (left - tail >= 4) {
+        while (left - tail >= 2 * k) {
             BHIPCHK(hipGraphLaunch(B->graph_exec, B->stream));
-            left -= 4;
+            left -= 2 * k;
         }
-        while (left - tail >= 2) {
+        while (left - tail >= k) {
             launch_pair(B, B->cur, 0);
             B->cur = 1 - B->cur;
-            left -= 2;
+            left -= k;
         }
     } else {
         // align to an even parity so the captured pair (0 then 1) applies
@@ -808,12 +843,12 @@ int ffb_batch_set_tuning(ffb_batch_t *B, const ffb_batch_tuning_t *t) {
 int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *best_out, double *best_ms) {
     if (!B) return fail(FFB_E_INVALID, "ffb_batch_autotune: null handle");
     if (trial_steps < 2) trial_steps = 2;
-    trial_steps += trial_steps & 1;  // whole ping-pong cycles of single steps; pairs: trial_steps / 2 launches
+    trial_steps += trial_steps & 1;  // whole ping-pong cycles of single steps; multi-step: trial_steps / k launches
     BHIPCHK(hipSetDevice(B->device));
     // Candidates of the kernel the batch runs for consecutive steps, each timed
     // by direct launches (no graph is captured until the winner is known):
     //   step pairs: region shape (62 x 14 or 62 x 30 tiles; FP64 one shape) x
-    //     XCD group (4 .. 64 tiles) x grid (one workgroup per tile, or a
+    //     XCD group (4, 6, 8, 12, 16, 32, 64 tiles) x grid (one workgroup per tile, or a
     //     persistent grid of 1, 2 or 4 rounds of the workgroups a CU holds:
     //     4 of the 8-wave shape, 2 of the 16-wave one)
     //   one launch per step: workgroup size (64 .. 1024 threads) x persistent
@@ -822,7 +857,7 @@ int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *
     const ffb_batch_tuning_t base = B->tuning;
     if (B->pairs) {
         const int shapes[] = {2, 1};
-        const int groups[] = {4, 8, 16, 32, 64};
+        const int groups[] = {4, 6, 8, 12, 16, 32, 64};
         for (int sh : shapes) {
             if (B->elem == 8 && sh == 1) continue;
             const int resident = (sh == 1 ? 2 : 4) * B->cus;
@@ -851,7 +886,7 @@ int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *
     if (rc) return rc;
     ffb_batch_tuning_t best = base;
     double best_t = 1e30;
-    const int launches = B->pairs ? trial_steps / 2 : trial_steps;
+    const int launches = B->pairs ? std::max(1, trial_steps / B->per_pass) : trial_steps;
     for (const ffb_batch_tuning_t &t : cands) {
         if ((rc = apply_tuning(B, t))) return rc;
         // one warm-up launch, then the timed ones, alternating the pools

@@ -136,10 +136,11 @@ int ffb_init(ffb_lattice_t *lat);
  * exchange_halos and the Poiseuille pre-pass, then execute_local_sim_step,
  * simulation.f90:18-73).  When write_macro_last != 0 the last step also
  * writes rho, u_x, u_y (the reference's write_macro_fields).  In every form
- * but the 1-3-12 layout (and not in Poiseuille flow) consecutive steps run in
- * pairs, two updates per kernel launch; the results are bitwise those of one
- * launch per step (FFB_TWO_STEP=0 at ffb_create selects that, =1 pairs for any
- * form). */
+ * (not in Poiseuille flow) consecutive steps run in triples, three updates per
+ * kernel launch, with a pair or a single step for a remainder; the results are
+ * bitwise those of one launch per step (FFB_STEPS_PER_PASS=1/2/3 at
+ * ffb_create selects one launch per step, pairs or triples; FFB_TWO_STEP=0 one
+ * launch per step). */
 int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
 
 /* Changes the physical parameters of an existing handle (the next steps use
@@ -241,10 +242,11 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
  * Batched form (no reference counterpart: the reference runs one domain per
  * process).  Many independent lattices of uneven size, each with its own
  * omega and boundary kind, advance together: in the 4- and 8-byte forms
- * consecutive steps run as step pairs, two steps per launch of one kernel over
- * the tiles of all lattices, in the 16-bit forms one step per launch of one
- * kernel over the row segments of all lattices (both bitwise the same;
- * FFB_BATCH_PAIRS=0/1 at creation selects one or the other; either kernel
+ * consecutive steps run as step triples, three steps per launch of one kernel
+ * over the tiles of all lattices (FFB_BATCH_STEPS_PER_PASS=2 at creation: step
+ * pairs), in the 16-bit forms one step per launch of one kernel over the row
+ * segments of all lattices (all bitwise the same; FFB_BATCH_PAIRS=0/1 at
+ * creation selects one launch per step or multi-step launches; either kernel
  * runs one workgroup per tile or a persistent grid), and the launches of a
  * ping-pong cycle are captured once in a hipGraph and replayed.  ffb_batch_autotune()
  * picks the launch shape of the kernel the batch runs by timing the
@@ -264,11 +266,12 @@ typedef struct ffb_batch_lattice {
 } ffb_batch_lattice_t;
 
 typedef struct ffb_batch_tuning {
-    /* one launch per step (also an odd remainder of a paired batch) */
+    /* one launch per step (also a remainder of a multi-step batch) */
     int32_t tile_w;       /* threads per workgroup (64-cell row segments x 64): 64 .. 1024 */
     int32_t grid_blocks;  /* persistent grid size; 0 = one workgroup per tile */
-    /* step pairs */
-    int32_t pair_shape;   /* 2: 62 x 14 output tiles (8 waves), 1: 62 x 30 (16 waves); FP64 always 62 x 6 */
+    /* step pairs and triples */
+    int32_t pair_shape;   /* 2: 8-wave regions (pairs 62 x 14, triples 60 x 12 output tiles), 1: 16 waves
+                             (62 x 30, 60 x 28); FP64 one shape (62 x 6, 60 x 12) */
     int32_t pair_group;   /* consecutive tiles per XCD (0: dispatch order) */
     int32_t pair_grid;    /* persistent grid size (rounded up to a multiple of 8); 0 = one workgroup per tile */
     int32_t reserved;
@@ -297,7 +300,7 @@ int ffb_batch_reset_timer(ffb_batch_t *batch);
 int ffb_batch_info(ffb_batch_t *batch, int64_t *cells, int32_t *tiles, ffb_batch_tuning_t *tuning);
 int ffb_batch_set_tuning(ffb_batch_t *batch, const ffb_batch_tuning_t *tuning);
 /* Times every candidate launch shape of the kernel the batch runs (step
- * pairs: pair_shape x pair_group x pair_grid; one launch per step: tile_w x
+ * pairs or triples: pair_shape x pair_group x pair_grid; one launch per step: tile_w x
  * grid_blocks) for trial_steps steps, keeps the fastest, and re-initialises
  * the batch (the trials advance it).  *best_ms_per_step is per step. */
 int ffb_batch_autotune(ffb_batch_t *batch, int32_t trial_steps, ffb_batch_tuning_t *best, double *best_ms_per_step);

@@ -1,5 +1,5 @@
 """CPU checks of bench.py's contract pieces that need no GPU: the one-GPU
-guard, the selection of one launch per step or step pairs for the lattices a
+guard, the selection of one, two or three steps per launch for the lattices a
 leg creates (and its restoration of the environment), the roofline
 arithmetic (bytes a launch must move, not per update), the traffic file's
 source stamp (a stale file is refused), and the segmentation of a profiled
@@ -23,24 +23,34 @@ def test_more_than_one_gpu_exits_with_message():
 
 def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     monkeypatch.delenv("FFB_TWO_STEP", raising=False)
-    assert all(bench.uses_pairs(p) for p in ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev"))
-    with bench.pass_mode(False):
-        assert os.environ["FFB_TWO_STEP"] == "0" and not bench.uses_pairs("fp32")
-    assert "FFB_TWO_STEP" not in os.environ
+    monkeypatch.delenv("FFB_STEPS_PER_PASS", raising=False)
+    forms = ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev")
+    assert all(bench.steps_per_pass(p) == bench.DEFAULT_STEPS_PER_PASS for p in forms)
+    with bench.pass_mode(1):
+        assert os.environ["FFB_TWO_STEP"] == "0" and bench.steps_per_pass("fp32") == 1
+    assert "FFB_TWO_STEP" not in os.environ and "FFB_STEPS_PER_PASS" not in os.environ
+    with bench.pass_mode(2):
+        assert os.environ["FFB_STEPS_PER_PASS"] == "2" and bench.steps_per_pass("fp64") == 2
     monkeypatch.setenv("FFB_TWO_STEP", "1")
     with bench.pass_mode(None):
-        assert bench.uses_pairs("e3m12dev")
-    with bench.pass_mode(False):
+        assert bench.steps_per_pass("e3m12dev") == bench.DEFAULT_STEPS_PER_PASS
+    with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0"
-    assert os.environ["FFB_TWO_STEP"] == "1"
+    assert os.environ["FFB_TWO_STEP"] == "1" and "FFB_STEPS_PER_PASS" not in os.environ
+    monkeypatch.setenv("FFB_TWO_STEP", "0")  # one launch per step wins over the steps per pass
+    monkeypatch.setenv("FFB_STEPS_PER_PASS", "3")
+    assert bench.steps_per_pass("fp32") == 1
 
 
-def test_batch_pair_selection(monkeypatch):
+def test_batch_steps_per_pass_selection(monkeypatch):
     monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
-    assert bench.batch_uses_pairs("fp32") and bench.batch_uses_pairs("fp64") and bench.batch_uses_pairs("fp32dev")
-    assert not bench.batch_uses_pairs("e3m12dev") and not bench.batch_uses_pairs("fp16dev")
+    monkeypatch.delenv("FFB_BATCH_STEPS_PER_PASS", raising=False)
+    assert all(bench.batch_steps_per_pass(p) == 3 for p in ("fp32", "fp64", "fp32dev"))
+    assert bench.batch_steps_per_pass("e3m12dev") == 1 and bench.batch_steps_per_pass("fp16dev") == 1
+    monkeypatch.setenv("FFB_BATCH_STEPS_PER_PASS", "2")
+    assert bench.batch_steps_per_pass("fp32") == 2 and bench.batch_steps_per_pass("fp16dev") == 2
     monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
-    assert not bench.batch_uses_pairs("fp32")
+    assert bench.batch_steps_per_pass("fp32") == 1
 
 
 def test_roofline_counts_one_read_and_write_per_launch():

@@ -4,9 +4,10 @@ that lattice alone (FP32, FP64 and the deviation forms at each lattice's own
 scale, the 16-bit ones with stochastic rounding, whose step index the graph
 replays advance on the device), across graph replays, odd/even step counts,
 re-tuning (the graph is captured again) and the autotuner's
-re-initialisation; with step pairs in one launch (the default) under several
-pair tunings, persistent grids smaller than the tile count included, and with
-one launch per step under several single-step tunings.  The 512-lattice mixed
+re-initialisation; with step triples (the default for the 4- and 8-byte forms)
+and step pairs in one launch under several tunings, persistent grids smaller
+than the tile count included, and with one launch per step under several
+single-step tunings.  The 512-lattice mixed
 batch is checked against an FP64 batch of the same lattices, lattice by
 lattice."""
 import numpy as np
@@ -42,12 +43,13 @@ def oracle_of(L, dtype, steps, scale=0):
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-# (pairs, tuning): step pairs with tile shape / XCD group / persistent grid,
-# or one launch per step with workgroup size / persistent grid
-TUNINGS = [
-    ("default", {"pair_shape": 2, "pair_group": 32, "pair_grid": 0}),
-    ("default", {"pair_shape": 1, "pair_group": 3, "pair_grid": 8}),
-    ("default", {"pair_shape": 2, "pair_group": 0, "pair_grid": 40}),
+# (mode, tuning): "3" step triples, "2" step pairs, with tile shape / XCD
+# group / persistent grid (forced for every form), or "0" one launch per step
+# with workgroup size / persistent grid
+MULTI = [{"pair_shape": 2, "pair_group": 32, "pair_grid": 0},
+         {"pair_shape": 1, "pair_group": 3, "pair_grid": 8},
+         {"pair_shape": 2, "pair_group": 0, "pair_grid": 40}]
+TUNINGS = [("3", t) for t in MULTI] + [("2", t) for t in MULTI] + [
     ("0", {"tile_w": 256, "grid_blocks": 0}),
     ("0", {"tile_w": 128, "grid_blocks": 8}),
     ("0", {"tile_w": 512, "grid_blocks": 3}),
@@ -55,20 +57,22 @@ TUNINGS = [
 ]
 
 
-@pytest.mark.parametrize("pairs,tuning", TUNINGS,
-                         ids=[f"pairs{p}-" + "-".join(f"{k}{v}" for k, v in t.items()) for p, t in TUNINGS])
+@pytest.mark.parametrize("mode,tuning", TUNINGS,
+                         ids=[f"k{p}-" + "-".join(f"{k}{v}" for k, v in t.items()) for p, t in TUNINGS])
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"])
-def test_batch_matches_oracle_per_lattice(monkeypatch, precision, pairs, tuning):
-    """pairs "default": step pairs in one launch (the tuning applies to the
-    pair kernel; an odd remainder takes one single step); "0": one launch per
+def test_batch_matches_oracle_per_lattice(monkeypatch, precision, mode, tuning):
+    """mode "3"/"2": three or two steps per launch (the tuning applies to the
+    multi-step kernel; a remainder takes single steps); "0": one launch per
     step everywhere."""
     from fastfortranboltzmann_amd.batch import Batch
 
-    print(f"[gpu] batch {precision} pairs={pairs} tuning={tuning}", flush=True)
-    if pairs == "0":
+    print(f"[gpu] batch {precision} steps per launch {mode} tuning={tuning}", flush=True)
+    if mode == "0":
         monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
+        monkeypatch.delenv("FFB_BATCH_STEPS_PER_PASS", raising=False)
     else:
-        monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
+        monkeypatch.setenv("FFB_BATCH_PAIRS", "1")
+        monkeypatch.setenv("FFB_BATCH_STEPS_PER_PASS", mode)
     lats = small_batch()
     dt = {"fp64": np.float64, "fp32": np.float32}.get(precision, precision)
     with Batch(lats, precision) as b:

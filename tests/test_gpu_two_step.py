@@ -1,14 +1,16 @@
-"""Step pairs in one pass (temporal blocking, step2_kernel) against one pass
-per step (step_kernel): bitwise equal populations and macro fields in every
+"""Step pairs and triples in one pass (temporal blocking: step2_kernel,
+march2_kernel, stepk_kernel) against one pass per step (step_kernel):
+bitwise equal populations and macro fields in every
 mode without a strip pass and every storage form, from random states that
 make every boundary rule carry a distinct value, on shapes smaller and larger
 than one tile (62 x 14 and 62 x 30 for 4-byte and narrower elements, both
 region shapes; FP64 62 x 6), with odd step counts (the last step of an odd
-count runs alone) and macro output on the last step.  The pair path is the
-default for every form, so the oracle parity tests cover it too;
-FFB_TWO_STEP, FFB_PAIR_KERNEL (tiles or column march), FFB_PAIR_SHAPE,
-FFB_MARCH_SHAPE and FFB_MARCH_ROWS select the path when a lattice is
-created."""
+count runs alone) and macro output on the last step.  Triples (a pair or a
+single step for a remainder) are the default for every form, so the oracle
+parity tests cover them too;
+FFB_TWO_STEP, FFB_STEPS_PER_PASS, FFB_PAIR_KERNEL (tiles or column march),
+FFB_PAIR_SHAPE, FFB_TRIPLE_SHAPE, FFB_MARCH_SHAPE and FFB_MARCH_ROWS select
+the path when a lattice is created."""
 import numpy as np
 import pytest
 
@@ -42,17 +44,21 @@ def _random_state(seed, precision, nx, ny):
     return (w[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny + 2, nx + 2)))).astype(dt)
 
 
-KINDS = {"shape8x2": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2},
-         "shape16x2": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 1},
+PAIRS = {"FFB_STEPS_PER_PASS": 2}
+KINDS = {"shape8x2": {**PAIRS, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2},
+         "shape16x2": {**PAIRS, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 1},
          # XCD blocks of 2 x 3 tiles (partial chunks at the grid's edges)
-         "xcd2x3": {"FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2, "FFB_PAIR_XCD_GROUP": 2, "FFB_PAIR_XCD_ROWS": 3},
+         "xcd2x3": {**PAIRS, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2, "FFB_PAIR_XCD_GROUP": 2, "FFB_PAIR_XCD_ROWS": 3},
          # column march: bands of 8 x 1, 8 x 2, 16 x 1 and 4 x 2 rows, chunks of
          # 5, 16, 40 and 3 rows (several chunks per strip, partial last chunks,
          # ring rows reused many times, chunks shorter than a band)
-         "march8x1": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 1, "FFB_MARCH_ROWS": 5},
-         "march8x2": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 2, "FFB_MARCH_ROWS": 16},
-         "march16x1": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 3, "FFB_MARCH_ROWS": 40},
-         "march4x2": {"FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 4, "FFB_MARCH_ROWS": 3}}
+         "march8x1": {**PAIRS, "FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 1, "FFB_MARCH_ROWS": 5},
+         "march8x2": {**PAIRS, "FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 2, "FFB_MARCH_ROWS": 16},
+         "march16x1": {**PAIRS, "FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 3, "FFB_MARCH_ROWS": 40},
+         "march4x2": {**PAIRS, "FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 4, "FFB_MARCH_ROWS": 3},
+         # step triples (stepk_kernel): regions 64 x 16 and 64 x 32, outputs 60 x 12 and 60 x 28
+         "triple8x2": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 2},
+         "triple16x2": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 1}}
 
 
 @pytest.mark.parametrize("kind", list(KINDS))
@@ -64,7 +70,8 @@ def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape, kind):
     nx, ny = shape
     print(f"[gpu] two-step pairs ({kind}) mode {mode} {precision} {nx}x{ny}", flush=True)
     f0 = _random_state(100 * mode + nx + 7 * ny, precision, nx, ny)
-    for steps in (2, 7):
+    # triples: 3 (one triple), 8 (two triples and a pair), 7 (two triples and a single step)
+    for steps in ((3, 8, 7) if kind.startswith("triple") else (2, 7)):
         fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps, KINDS[kind])
         fb, mb, nb = _run(monkeypatch, False, mode, nx, ny, precision, scale, f0, steps)
         assert na == nb == steps
