@@ -4,10 +4,10 @@ Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
-whole domain; the library runs consecutive steps as step triples, three
-updates per kernel launch (temporal blocking, bitwise the same result; a pair
-or a single step for a remainder), and the line also reports step pairs
-("pairs") and one launch per step ("single_pass").  Inputs are resident in
+whole domain; the library runs consecutive steps four per kernel launch
+(FP32; three for FP64; temporal blocking, bitwise the same result; fewer for
+a remainder), and the line also reports step pairs ("pairs") and one launch
+per step ("single_pass").  Inputs are resident in
 HBM before timing.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64|...]
@@ -96,7 +96,10 @@ class pass_mode:
                 os.environ[key] = val
 
 
-DEFAULT_STEPS_PER_PASS = 3  # the library's default for every form (FFB_STEPS_PER_PASS)
+def default_steps_per_pass(precision: str) -> int:
+    """The library's default steps per launch (FFB_STEPS_PER_PASS unset): four
+    for FP32 and FP32 deviations, three for FP64 and the 16-bit forms."""
+    return 4 if precision in ("fp32", "fp32dev") else 3
 
 
 def steps_per_pass(precision: str) -> int:
@@ -112,16 +115,19 @@ def steps_per_pass(precision: str) -> int:
     env = os.environ.get("FFB_STEPS_PER_PASS")
     if env is not None:
         try:
-            return max(1, min(3, int(env)))
+            k = max(1, min(4, int(env)))
         except ValueError:
             return 1
-    return DEFAULT_STEPS_PER_PASS if precision in BYTES_PER_POP else 1
+        four = precision in ("fp32", "fp64", "fp32dev", "fp16dev")  # forms with a four-step kernel
+        return k if (k < 4 or four) else 3
+    return default_steps_per_pass(precision) if precision in BYTES_PER_POP else 1
 
 
 def batch_steps_per_pass(precision: str) -> int:
     """Steps a batch computes per launch, as ffb_batch_create chooses them:
     multi-step launches for the 4- and 8-byte forms unless FFB_BATCH_PAIRS
-    says otherwise, three steps each unless FFB_BATCH_STEPS_PER_PASS (1..3)
+    says otherwise, four steps each for the 4-byte forms and three for FP64
+    unless FFB_BATCH_STEPS_PER_PASS (1..4; four only for the 4-byte forms)
     says otherwise (which alone, without FFB_BATCH_PAIRS, also selects
     multi-step launches for any form); 1 = one launch per step."""
     pairs_env = os.environ.get("FFB_BATCH_PAIRS")
@@ -132,7 +138,8 @@ def batch_steps_per_pass(precision: str) -> int:
             pairs = int(pairs_env) != 0
         except ValueError:
             pairs = False
-    k = 3 if pairs else 1
+    four = BYTES_PER_POP.get(precision, 0) == 4
+    k = (4 if four else 3) if pairs else 1
     if k_env:
         try:
             k = int(k_env)
@@ -140,7 +147,7 @@ def batch_steps_per_pass(precision: str) -> int:
             pass
         if not pairs_env:
             pairs = k >= 2
-    return max(2, min(3, k)) if pairs else 1
+    return max(2, min(4 if four else 3, k)) if pairs else 1
 
 
 def hbm_rates(mlups: float, precision: str, per_launch: int) -> dict:
@@ -321,7 +328,38 @@ def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int 
     out["mlups"] = out["fp32"]["mlups"]
     out["vs_single_4096_fp32"] = round(out["fp32"]["mlups"] / single_fp32_mlups, 3)
     out["vs_single_pass_4096_fp32"] = round(out["fp32"]["mlups"] / single_pass_fp32_mlups, 3)
+    mix = large_grid_mix(lats)
+    out["large_grids_same_kinds"] = mix
+    out["vs_large_grids_same_kinds"] = round(out["fp32"]["mlups"] / mix["mlups"], 3)
     return out
+
+
+def large_grid_mix(lats, n: int = 4096, steps: int = 240, warmup: int = 24) -> dict:
+    """The per-cell rate of large single grids of the batch's own lattice kinds:
+    FP32 n^2 shear wave, Couette flow and body-force channel (each with the
+    library's default steps per launch, HIP events), combined as the batch
+    combines them -- time per cell weighted by each kind's share of the
+    batch's cells."""
+    from fastfortranboltzmann_amd import Lattice, Settings
+    from fastfortranboltzmann_amd.batch import BODY_FORCE_CHANNEL, COUETTE_FLOW, SHEAR_WAVE
+
+    total = sum(L.n_x * L.n_y for L in lats)
+    rates, share = {}, {}
+    for kind, name, kw in ((SHEAR_WAVE, "shear_wave", dict(U_MAX=0.01, N_SIN=1.0)),
+                           (COUETTE_FLOW, "couette", dict(U_WALL=0.05)),
+                           (BODY_FORCE_CHANNEL, "body_force_channel", dict(BODY_FORCE=1e-6))):
+        share[name] = sum(L.n_x * L.n_y for L in lats if L.kind == kind) / total
+        with Lattice(Settings(N_X=n, N_Y=n, SIM_MODE=kind, OMEGA=1.25, precision="fp32", **kw)) as lat:
+            lat.init()
+            lat.step(warmup)
+            lat.sync()
+            lat.reset_timer()
+            lat.step(steps)
+            lat.sync()
+            rates[name] = n * n * steps / (lat.elapsed_ms() * 1e-3) / 1e6
+    per_cell = sum(share[k] / rates[k] for k in rates)
+    return {"size": n, "mlups": round(1.0 / per_cell, 1), "by_kind": {k: round(v, 1) for k, v in rates.items()},
+            "cell_share": {k: round(v, 4) for k, v in share.items()}}
 
 
 def adjoint_cost(n: int = 512, steps: int = 2000) -> dict:

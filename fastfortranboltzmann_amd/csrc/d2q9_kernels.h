@@ -1044,7 +1044,7 @@ template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
 __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
                                            typename Store<S>::R (&lds)[9][WAVES * RPT][64]) {
     static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
-    static_assert(K >= 2 && K <= 3, "two or three steps per pass");
+    static_assert(K >= 2 && K <= 4, "two to four steps per pass");
     using R = typename Store<S>::R;
     constexpr int ROWS = WAVES * RPT, RING = K - 1, OW = 64 - 2 * RING, OH = ROWS - 2 * RING;
     constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes

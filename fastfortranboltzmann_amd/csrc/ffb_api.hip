@@ -311,6 +311,43 @@ template <typename S, typename C> int launch_step3(ffb_lattice *L, int write_mac
     return launch_step3_shape<S, C, 8, 2>(L, a);
 }
 
+// Four steps in one launch (stepk_kernel): tiles of 58 x (WAVES RPT - 6) cells;
+// regions 2 = 12 waves x 2 rows (64 x 24, 58 x 18 tiles, the default), 1 = 16
+// waves x 2 rows, 3 = 8 waves x 2 rows.
+template <typename S, typename C, int WAVES, int RPT> int launch_step4_shape(ffb_lattice *L, StepArgs<C> a) {
+    constexpr int ROWS = WAVES * RPT;
+    a.n_tiles_x = (a.nx + 57) / 58;
+    a.n_tiles_y = (a.ny + ROWS - 7) / (ROWS - 6);
+    a.n_blocks = a.n_tiles_x * a.n_tiles_y;
+    a.xcd_group = L->pair_xcd_group;
+    const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((stepk_kernel<S, C, SW, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
+    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((stepk_kernel<S, C, CF, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
+    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((stepk_kernel<S, C, BF, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
+    case FFB_SLIDING_LID: hipLaunchKernelGGL((stepk_kernel<S, C, SL, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
+    default: return fail(FFB_E_INVALID, "four-step pass: Poiseuille flow is not supported");
+    }
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+template <typename S, typename C> int launch_step4(ffb_lattice *L, int write_macro) {
+    const StepArgs<C> a = make_args<C>(L, write_macro);
+    if (L->quad_shape == 1) return launch_step4_shape<S, C, 16, 2>(L, a);
+    if (L->quad_shape == 3) return launch_step4_shape<S, C, 8, 2>(L, a);
+    return launch_step4_shape<S, C, 12, 2>(L, a);
+}
+
+int dispatch_step4(ffb_lattice *L, int write_macro) {
+    switch (L->cfg.precision) {
+    case FFB_PREC_FP64: return launch_step4<double, double>(L, write_macro);
+    case FFB_PREC_FP16_DEV: return launch_step4<__half, float>(L, write_macro);
+    case FFB_PREC_FP32_DEV: return launch_step4<f32dev, float>(L, write_macro);
+    default: return launch_step4<float, float>(L, write_macro);
+    }
+}
+
 int dispatch_step3(ffb_lattice *L, int write_macro) {
     switch (L->cfg.precision) {
     case FFB_PREC_FP64: return launch_step3<double, double>(L, write_macro);
@@ -658,14 +695,22 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // BF16 deviations 8192^2 380 -> 341-342 (8-wave regions), the 1-3-12 form
     // 390 -> 371.  FFB_TWO_STEP=0/1 overrides.
     L->two_step = env_ll("FFB_TWO_STEP", 1) ? 1 : 0;
-    // steps per pass where the mode allows: 3 = triples (stepk_kernel, the
-    // default) and a pair or single step for a remainder, 2 = pairs, 1 = one
-    // launch per step (FFB_STEPS_PER_PASS).  Triples against pairs, us per
+    // steps per pass where the mode allows: 4 or 3 (stepk_kernel) and fewer
+    // for a remainder, 2 = pairs, 1 = one launch per step (FFB_STEPS_PER_PASS).  Triples against pairs, us per
     // three steps on one box: FP32 4096^2 264 against 363 (190,800 against
     // 138,700 MLUPS), FP64 4096^2 476 against 644, FP32 8192^2 1039 against
     // 1458, FP32 deviations 8192^2 997 against 1403, FP16 8192^2 1149
     // against 1199 (64 x 16 regions; 64 x 32 regions 9-19 % slower)
-    L->steps_per_pass = (int)std::max(1LL, std::min(3LL, env_ll("FFB_STEPS_PER_PASS", 3)));
+    // Four steps per pass (64 x 24 regions, 58 x 18 tiles) for FP32 and FP32
+    // deviations: FP32 4096^2 205,200 against 191,900 MLUPS in triples, FP32
+    // deviations 8192^2 223,900 against 199,100; FP64 110,600 against 107,900
+    // with 64 x 16 regions (its 64 x 24 regions hold one workgroup per CU:
+    // 92,200), so FP64 and the 16-bit forms keep triples.
+    {
+        const bool four = cfg->precision == FFB_PREC_FP32 || cfg->precision == FFB_PREC_FP32_DEV;
+        L->steps_per_pass = (int)std::max(1LL, std::min(4LL, env_ll("FFB_STEPS_PER_PASS", four ? 4 : 3)));
+    }
+    L->quad_shape = (int)env_ll("FFB_QUAD_SHAPE", 2);
     if (L->steps_per_pass == 1) L->two_step = 0;
     L->triple_shape = (int)env_ll("FFB_TRIPLE_SHAPE", 2);
     // pair tiles per XCD group, by the width of the domain in 62-cell tiles:
@@ -779,7 +824,19 @@ int ffb_step(ffb_lattice_t *L, int32_t n_steps, int32_t write_macro_last) {
     }
     const bool pairs = L->two_step && L->cfg.sim_mode != FFB_POISEUILLE_FLOW;
     const bool triples = pairs && L->steps_per_pass >= 3;
+    const bool quads = pairs && L->steps_per_pass >= 4 &&
+                       (L->cfg.precision == FFB_PREC_FP32 || L->cfg.precision == FFB_PREC_FP64 ||
+                        L->cfg.precision == FFB_PREC_FP32_DEV || L->cfg.precision == FFB_PREC_FP16_DEV);
     for (int32_t k = 0; k < n_steps;) {
+        if (quads && n_steps - k >= 4) {  // steps k+1 .. k+4 in one pass
+            const int wm = (write_macro_last && k + 4 == n_steps) ? 1 : 0;
+            int rc = dispatch_step4(L, wm);
+            if (rc) return rc;
+            L->cur = 1 - L->cur;
+            L->steps += 4;
+            k += 4;
+            continue;
+        }
         if (triples && n_steps - k >= 3) {  // steps k+1 .. k+3 in one pass
             const int wm = (write_macro_last && k + 3 == n_steps) ? 1 : 0;
             int rc = dispatch_step3(L, wm);

@@ -296,11 +296,14 @@ int grid_for(const ffb_batch *B) {
 }
 
 // region rows of a multi-step tile: 16 (8 waves x 2 rows), 32 (16 waves x 2
-// rows, shape 1); FP64 pairs 8 (8 waves x 1 row), FP64 triples 16.  Output
-// tiles of K steps are (64 - 2(K-1)) x (rows - 2(K-1)): pairs 62 x 14 / 62 x 30
-// (FP64 62 x 6), triples 60 x 12 / 60 x 28 (FP64 60 x 12).
+// rows, shape 1); four steps 24 (12 waves x 2 rows) or 32; FP64 pairs 8 (8
+// waves x 1 row), FP64 triples 16.  Output tiles of K steps are
+// (64 - 2(K-1)) x (rows - 2(K-1)): pairs 62 x 14 / 62 x 30 (FP64 62 x 6),
+// triples 60 x 12 / 60 x 28 (FP64 60 x 12), four steps 58 x 18 / 58 x 26.
 int region_rows(const ffb_batch *B, int shape, int k) {
-    return B->elem == 8 ? (k == 3 ? 16 : 8) : (shape == 1 ? 32 : 16);
+    if (B->elem == 8) return k >= 3 ? 16 : 8;
+    if (k == 4) return shape == 1 ? 32 : 24;
+    return shape == 1 ? 32 : 16;
 }
 
 // lattices in table order: grouped by kind (stable), so resident waves share a code path
@@ -373,7 +376,7 @@ template <typename C> int upload_tables(ffb_batch *B) {
         L.half_force = (C)0.5 * L.force;
     }
     size_t cap = 0;
-    for (int k = 2; k <= 3; ++k)
+    for (int k = 2; k <= 4; ++k)
         for (int sh = 1; sh <= 2; ++sh) cap = std::max(cap, pair_tiles(B, sh, k).size());
     B->ptile_cap = (int)cap;
     BHIPCHK(hipMalloc(&B->ptiles, (size_t)B->ptile_cap * sizeof(PTile)));
@@ -473,10 +476,16 @@ void launch_pair_c(const ffb_batch *B, int parity, int write_macro) {
 
 template <typename S, typename C, int K> void launch_pair_shape(const ffb_batch *B, int parity, int write_macro) {
     if (B->tuning.pair_shape == 1) launch_pair_c<S, C, 16, 2, false, K>(B, parity, write_macro);
+    else if constexpr (K == 4) launch_pair_c<S, C, 12, 2, false, K>(B, parity, write_macro);
     else launch_pair_c<S, C, 8, 2, false, K>(B, parity, write_macro);
 }
 
 template <int K> void launch_pair_k(const ffb_batch *B, int parity, int write_macro) {
+    if constexpr (K == 4) {  // 4-byte forms only (per_pass is at most 3 for the others)
+        if (B->precision == FFB_PREC_FP32_DEV) launch_pair_shape<f32dev, float, 4>(B, parity, write_macro);
+        else launch_pair_shape<float, float, 4>(B, parity, write_macro);
+        return;
+    }
     switch (B->precision) {
     case FFB_PREC_FP64:
         if constexpr (K == 2) launch_pair_c<double, double, 8, 1, true, 2>(B, parity, write_macro);
@@ -494,7 +503,8 @@ template <int K> void launch_pair_k(const ffb_batch *B, int parity, int write_ma
 // single domain's pairs (FP64 pairs: nontemporal); regions 8 waves x 2 rows or
 // 16 waves x 2 rows, FP64 pairs 8 waves x 1 row, FP64 triples 8 waves x 2 rows.
 void launch_pair(const ffb_batch *B, int parity, int write_macro) {
-    if (B->per_pass >= 3) launch_pair_k<3>(B, parity, write_macro);
+    if (B->per_pass >= 4) launch_pair_k<4>(B, parity, write_macro);
+    else if (B->per_pass == 3) launch_pair_k<3>(B, parity, write_macro);
     else launch_pair_k<2>(B, parity, write_macro);
     count_steps(B, (unsigned)B->per_pass);
 }
@@ -628,15 +638,16 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     // in pairs against 83,200; FP32 deviations 127,000 / 83,700; FP64 63,000 /
     // 39,100; FP16 127,000 in pairs against 144,100, BF16 129,000 / 141,900,
     // 1-3-12 120,500 / 138,200); FFB_BATCH_PAIRS=0/1 at creation overrides
-    // and step triples (FFB_BATCH_STEPS_PER_PASS = 1, 2, 3; default 3 for the
-    // 4- and 8-byte forms when pairs are on)
+    // and multi-step launches of 2..4 steps (FFB_BATCH_STEPS_PER_PASS; default
+    // 4 for the 4-byte forms, 3 for FP64 when multi-step launches are on; four
+    // steps only for the 4-byte forms)
     {
         const char *e = std::getenv("FFB_BATCH_PAIRS");
         B->pairs = e && *e ? std::atoi(e) != 0 : B->elem >= 4;
         const char *k = std::getenv("FFB_BATCH_STEPS_PER_PASS");
-        int kk = k && *k ? std::atoi(k) : (B->pairs ? 3 : 1);
+        int kk = k && *k ? std::atoi(k) : (B->pairs ? (B->elem == 4 ? 4 : 3) : 1);
         if (!(e && *e) && k && *k) B->pairs = kk >= 2;  // the steps per pass alone selects
-        kk = std::max(1, std::min(3, kk));
+        kk = std::max(1, std::min(B->elem == 4 ? 4 : 3, kk));
         B->per_pass = B->pairs ? std::max(2, kk) : 1;
     }
     B->desc.assign(lattices, lattices + count);

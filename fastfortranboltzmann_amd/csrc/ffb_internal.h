@@ -49,6 +49,7 @@ struct ffb_lattice {
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
     int steps_per_pass = 2;  // 3: step triples (stepk_kernel) where pairs are allowed
     int triple_shape = 2;    // triple region: 2 = 8 waves x 2 rows, 1 = 16 waves x 2 rows
+    int quad_shape = 2;      // four-step region: 2 = 12 waves x 2 rows, 1 = 16 x 2, 3 = 8 x 2
     int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
     int pair_xcd_rows = 1;   // H > 1: blocks of G x H pair tiles per XCD
     int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows

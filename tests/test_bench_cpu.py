@@ -25,7 +25,7 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     monkeypatch.delenv("FFB_TWO_STEP", raising=False)
     monkeypatch.delenv("FFB_STEPS_PER_PASS", raising=False)
     forms = ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev")
-    assert all(bench.steps_per_pass(p) == bench.DEFAULT_STEPS_PER_PASS for p in forms)
+    assert [bench.steps_per_pass(p) for p in forms] == [4, 3, 3, 3, 4, 3]
     with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0" and bench.steps_per_pass("fp32") == 1
     assert "FFB_TWO_STEP" not in os.environ and "FFB_STEPS_PER_PASS" not in os.environ
@@ -33,10 +33,12 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
         assert os.environ["FFB_STEPS_PER_PASS"] == "2" and bench.steps_per_pass("fp64") == 2
     monkeypatch.setenv("FFB_TWO_STEP", "1")
     with bench.pass_mode(None):
-        assert bench.steps_per_pass("e3m12dev") == bench.DEFAULT_STEPS_PER_PASS
+        assert bench.steps_per_pass("e3m12dev") == 3 and bench.steps_per_pass("fp32") == 4
     with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0"
     assert os.environ["FFB_TWO_STEP"] == "1" and "FFB_STEPS_PER_PASS" not in os.environ
+    monkeypatch.setenv("FFB_STEPS_PER_PASS", "4")  # four steps only where a four-step kernel exists
+    assert bench.steps_per_pass("fp64") == 4 and bench.steps_per_pass("bf16dev") == 3
     monkeypatch.setenv("FFB_TWO_STEP", "0")  # one launch per step wins over the steps per pass
     monkeypatch.setenv("FFB_STEPS_PER_PASS", "3")
     assert bench.steps_per_pass("fp32") == 1
@@ -45,7 +47,7 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
 def test_batch_steps_per_pass_selection(monkeypatch):
     monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
     monkeypatch.delenv("FFB_BATCH_STEPS_PER_PASS", raising=False)
-    assert all(bench.batch_steps_per_pass(p) == 3 for p in ("fp32", "fp64", "fp32dev"))
+    assert [bench.batch_steps_per_pass(p) for p in ("fp32", "fp64", "fp32dev")] == [4, 3, 4]
     assert bench.batch_steps_per_pass("e3m12dev") == 1 and bench.batch_steps_per_pass("fp16dev") == 1
     monkeypatch.setenv("FFB_BATCH_STEPS_PER_PASS", "2")
     assert bench.batch_steps_per_pass("fp32") == 2 and bench.batch_steps_per_pass("fp16dev") == 2

@@ -43,13 +43,14 @@ def oracle_of(L, dtype, steps, scale=0):
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-# (mode, tuning): "3" step triples, "2" step pairs, with tile shape / XCD
-# group / persistent grid (forced for every form), or "0" one launch per step
+# (mode, tuning): "4" four steps per launch (4-byte forms; three for the
+# others), "3" step triples, "2" step pairs, with tile shape / XCD group /
+# persistent grid (forced for every form), or "0" one launch per step
 # with workgroup size / persistent grid
 MULTI = [{"pair_shape": 2, "pair_group": 32, "pair_grid": 0},
          {"pair_shape": 1, "pair_group": 3, "pair_grid": 8},
          {"pair_shape": 2, "pair_group": 0, "pair_grid": 40}]
-TUNINGS = [("3", t) for t in MULTI] + [("2", t) for t in MULTI] + [
+TUNINGS = [("4", t) for t in MULTI] + [("3", t) for t in MULTI] + [("2", t) for t in MULTI] + [
     ("0", {"tile_w": 256, "grid_blocks": 0}),
     ("0", {"tile_w": 128, "grid_blocks": 8}),
     ("0", {"tile_w": 512, "grid_blocks": 3}),

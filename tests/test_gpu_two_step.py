@@ -58,7 +58,12 @@ KINDS = {"shape8x2": {**PAIRS, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2},
          "march4x2": {**PAIRS, "FFB_PAIR_KERNEL": 1, "FFB_MARCH_SHAPE": 4, "FFB_MARCH_ROWS": 3},
          # step triples (stepk_kernel): regions 64 x 16 and 64 x 32, outputs 60 x 12 and 60 x 28
          "triple8x2": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 2},
-         "triple16x2": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 1}}
+         "triple16x2": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 1},
+         # four steps per launch (FP32, FP64, FP32 and FP16 deviations; the other forms
+         # run triples under the same setting): regions 64 x 24, 64 x 32, 64 x 16
+         "quad12x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 2},
+         "quad16x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 1},
+         "quad8x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 3}}
 
 
 @pytest.mark.parametrize("kind", list(KINDS))
@@ -70,8 +75,10 @@ def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape, kind):
     nx, ny = shape
     print(f"[gpu] two-step pairs ({kind}) mode {mode} {precision} {nx}x{ny}", flush=True)
     f0 = _random_state(100 * mode + nx + 7 * ny, precision, nx, ny)
-    # triples: 3 (one triple), 8 (two triples and a pair), 7 (two triples and a single step)
-    for steps in ((3, 8, 7) if kind.startswith("triple") else (2, 7)):
+    # triples: 3 (one triple), 8 (two triples and a pair), 7 (two triples and a single step);
+    # four steps: 4, 11 (two and a triple), 9 (two and a single step)
+    steps_of = {"triple": (3, 8, 7), "quad": (4, 11, 9)}
+    for steps in steps_of.get(kind.rstrip("0123456789x"), (2, 7)):
         fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps, KINDS[kind])
         fb, mb, nb = _run(monkeypatch, False, mode, nx, ny, precision, scale, f0, steps)
         assert na == nb == steps
