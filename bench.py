@@ -4,7 +4,7 @@ Workload (default): the plain lid-driven cavity, 4096 x 4096 cells, halfway
 bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
-whole domain; the library runs consecutive steps four per kernel launch
+whole domain; the library runs consecutive steps five per kernel launch
 (FP32; three for FP64; temporal blocking, bitwise the same result; fewer for
 a remainder), and the line also reports step pairs ("pairs") and one launch
 per step ("single_pass").  Inputs are resident in
@@ -97,14 +97,16 @@ class pass_mode:
 
 
 def default_steps_per_pass(precision: str) -> int:
-    """The library's default steps per launch (FFB_STEPS_PER_PASS unset): four
-    for FP32 and FP32 deviations, three for FP64 and the 16-bit forms."""
-    return 4 if precision in ("fp32", "fp32dev") else 3
+    """The library's default steps per launch (FFB_STEPS_PER_PASS unset): five
+    for FP32 and FP32 deviations, two for the 1-3-12 deviations, three for
+    FP64, FP16 and BF16 deviations."""
+    return 5 if precision in ("fp32", "fp32dev") else (2 if precision == "e3m12dev" else 3)
 
 
 def steps_per_pass(precision: str) -> int:
     """Steps the library computes per launch for this form: FFB_STEPS_PER_PASS
-    (1..3), 1 where FFB_TWO_STEP=0, else the library's default."""
+    (1..5; five only for the 4-byte forms, four only for those with a four-step
+    kernel), 1 where FFB_TWO_STEP=0, else the library's default."""
     env = os.environ.get("FFB_TWO_STEP")
     if env is not None:
         try:
@@ -115,31 +117,34 @@ def steps_per_pass(precision: str) -> int:
     env = os.environ.get("FFB_STEPS_PER_PASS")
     if env is not None:
         try:
-            k = max(1, min(4, int(env)))
+            k = max(1, min(5, int(env)))
         except ValueError:
             return 1
         four = precision in ("fp32", "fp64", "fp32dev", "fp16dev")  # forms with a four-step kernel
+        five = precision in ("fp32", "fp32dev")  # forms with a five-step kernel
+        if k == 5 and not five:
+            k = 4
         return k if (k < 4 or four) else 3
     return default_steps_per_pass(precision) if precision in BYTES_PER_POP else 1
 
 
 def batch_steps_per_pass(precision: str) -> int:
     """Steps a batch computes per launch, as ffb_batch_create chooses them:
-    multi-step launches for the 4- and 8-byte forms unless FFB_BATCH_PAIRS
-    says otherwise, four steps each for the 4-byte forms and three for FP64
-    unless FFB_BATCH_STEPS_PER_PASS (1..4; four only for the 4-byte forms)
+    multi-step launches for every form unless FFB_BATCH_PAIRS says
+    otherwise, five steps each for the 4-byte forms and three for the others
+    unless FFB_BATCH_STEPS_PER_PASS (1..5; four and five only for the 4-byte forms)
     says otherwise (which alone, without FFB_BATCH_PAIRS, also selects
     multi-step launches for any form); 1 = one launch per step."""
     pairs_env = os.environ.get("FFB_BATCH_PAIRS")
     k_env = os.environ.get("FFB_BATCH_STEPS_PER_PASS")
-    pairs = BYTES_PER_POP.get(precision, 0) >= 4
+    pairs = True
     if pairs_env:
         try:
             pairs = int(pairs_env) != 0
         except ValueError:
             pairs = False
     four = BYTES_PER_POP.get(precision, 0) == 4
-    k = (4 if four else 3) if pairs else 1
+    k = (5 if four else 3) if pairs else 1
     if k_env:
         try:
             k = int(k_env)
@@ -147,7 +152,7 @@ def batch_steps_per_pass(precision: str) -> int:
             pass
         if not pairs_env:
             pairs = k >= 2
-    return max(2, min(4 if four else 3, k)) if pairs else 1
+    return max(2, min(5 if four else 3, k)) if pairs else 1
 
 
 def hbm_rates(mlups: float, precision: str, per_launch: int) -> dict:
@@ -217,6 +222,12 @@ def roofline(run: dict, workload: str) -> dict:
         out["kernel"] = tr["kernel"]
     else:
         out["traffic_note"] = traffic_status()
+    va = valu_from_profiles(workload)
+    if va:
+        # share of the SIMDs' cycles the launch's VALU instructions take (a
+        # wave64 instruction takes a SIMD 4 cycles): the bound of launches
+        # that trade HBM traffic for arithmetic
+        out["valu"] = {"wave_insts_per_launch": va["wave_insts_per_launch"], "busy": va["busy"]}
     return out
 
 
@@ -522,6 +533,75 @@ def record_traffic(fetch_csv: str, write_csv: str) -> int:
     return 0
 
 
+VALU_FILE = os.path.join(REPO, "profiles", "valu.txt")
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_WAVES", "GRBM_GUI_ACTIVE")
+N_SIMD = 256 * 4  # MI355X: 256 CUs of 4 SIMDs
+VALU_CYCLES = 4   # cycles a SIMD takes per wave64 VALU instruction (16 lanes per cycle, FP32/FP64 unpacked)
+
+
+def _multi_counter_dispatches(path: str):
+    """(kernel, grid, {counter: value}) per dispatch of a multi-counter rocprofv3 csv."""
+    rows = {}
+    for r in csv.DictReader(open(path)):
+        d = rows.setdefault(int(r["Dispatch_Id"]), (r["Kernel_Name"], int(r["Grid_Size"]), {}))
+        d[2][r["Counter_Name"]] = d[2].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [rows[k] for k in sorted(rows)]
+
+
+def record_valu(csv_path: str) -> int:
+    """profiles/valu.txt: VALU instructions per launch (SQ_INSTS_VALU, wave-level,
+    summed over the device) of the kernel that runs each workload of
+    --profile-run, the GPU cycles of its dispatches (GRBM_GUI_ACTIVE, counted
+    per XCD: / 8) and the share of the SIMDs' cycles the instructions take:
+    busy = insts * 4 / (1024 SIMDs * cycles); busy_active, the same from
+    SQ_ACTIVE_INST_VALU (cycles in units of 4), as a cross-check."""
+    disp = _multi_counter_dispatches(csv_path)
+    names = [n for n, _ in _profile_workloads()]
+    per = {c: _segments([(k, g, v.get(c, 0.0)) for k, g, v in disp]) for c in VALU_COUNTERS}
+    if any(len(v) != len(names) for v in per.values()):
+        raise SystemExit(f"record-valu: {len(per['SQ_INSTS_VALU'])} segments for {len(names)} workloads")
+    lines = [f"# kernel sources {kernel_source_hash()}",
+             "# VALU instructions per launch of the kernel that runs each workload (SQ_INSTS_VALU: wave-level",
+             "# instructions summed over the device; mean over the workload's dispatches) from",
+             "#   rocprofv3 --pmc " + " ".join(VALU_COUNTERS) + " --output-format csv -d <d> -o run -- \\",
+             "#       python3 bench.py --profile-run",
+             "#   python3 bench.py --record-valu <d>/run_counter_collection.csv",
+             "# cycles = GRBM_GUI_ACTIVE / 8 XCDs (GPU cycles of a dispatch);",
+             f"# busy = SQ_INSTS_VALU * {VALU_CYCLES} / ({N_SIMD} SIMDs * cycles);",
+             f"# busy_active = SQ_ACTIVE_INST_VALU * 4 / ({N_SIMD} SIMDs * cycles).",
+             "# workload valu_insts waves cycles busy busy_active kernel"]
+    for i, name in enumerate(names):
+        kern, insts, _ = per["SQ_INSTS_VALU"][i]
+        waves = per["SQ_WAVES"][i][1]
+        act = per["SQ_ACTIVE_INST_VALU"][i][1]
+        gui = per["GRBM_GUI_ACTIVE"][i][1] / 8.0
+        busy = insts * VALU_CYCLES / (N_SIMD * gui) if gui > 0 else float("nan")
+        busy2 = act * 4 / (N_SIMD * gui) if gui > 0 else float("nan")
+        lines.append(f"{name} {insts:.0f} {waves:.0f} {gui:.0f} {busy:.3f} {busy2:.3f} {kern}")
+    with open(VALU_FILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
+def valu_from_profiles(workload: str) -> dict | None:
+    """VALU record of `workload` from profiles/valu.txt when it was written from
+    these kernel sources, else None."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    stamp, out = None, None
+    for line in open(VALU_FILE):
+        if line.startswith("# kernel sources "):
+            stamp = line.split()[-1]
+        if line.startswith("#") or not line.strip():
+            continue
+        parts = line.split(None, 6)
+        if parts[0] == workload and len(parts) == 7:
+            out = {"wave_insts_per_launch": float(parts[1]), "busy": float(parts[4]),
+                   "cycles": float(parts[3]), "kernel": parts[6].strip()}
+    return out if stamp == kernel_source_hash() else None
+
+
 def _traffic_table():
     if not os.path.exists(TRAFFIC_FILE):
         return None, {}
@@ -668,9 +748,13 @@ def main() -> int:
     ap.add_argument("--profile-run", action="store_true", help="run the profiled workloads (rocprofv3 --pmc)")
     ap.add_argument("--record-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.txt from the two counter files of --profile-run")
+    ap.add_argument("--record-valu", metavar="SQ_CSV",
+                    help="write profiles/valu.txt from the VALU counter file of --profile-run")
     args = ap.parse_args()
     if args.record_traffic:
         return record_traffic(*args.record_traffic)
+    if args.record_valu:
+        return record_valu(args.record_valu)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the domain is one device-resident "
               "lattice and no multi-GPU path is built (run with --gpus 1)", file=sys.stderr)

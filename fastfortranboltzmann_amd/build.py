@@ -137,14 +137,35 @@ def build_cpp_driver(verbose: bool = False) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-I", CSRC,
-           "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libffb_hip.so")
+    """Compile every translation unit to an object in parallel (each holds its
+    own device code; no symbol crosses units on the device), then link."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
+    inc = ["-I", os.path.join(REPO, "include"), "-I", CSRC]
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory() as tmp:
+        jobs = []
+        for s in SOURCES:
+            obj = os.path.join(tmp, s.replace(".hip", ".o"))
+            jobs.append(([hipcc(), *compile_flags, *inc, "-c", os.path.join(CSRC, s), "-o", obj], obj))
+
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return subprocess.run(cmd, capture_output=True, text=True)
+
+        with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+            results = list(pool.map(run, [c for c, _ in jobs]))
+        for res in results:
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError("hipcc failed building libffb_hip.so")
+        cmd = [hipcc(), *HIPCC_FLAGS, "-o", LIB + ".tmp", *[o for _, o in jobs]]
+        res = run(cmd)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed linking libffb_hip.so")
     os.replace(LIB + ".tmp", LIB)
     build_fortran_host(verbose)
     build_cpp_driver(verbose)

@@ -433,6 +433,36 @@ __device__ __forceinline__ void pull_edge_cell(const StepArgs<C> &a, const typen
     }
 }
 
+// Quotients a / b of FP32 operands sharing one denominator b.  The sequence
+// is the compiler's IEEE-correct division on gfx950 -- an approximate
+// reciprocal, one Newton step, two quotient corrections by FMA -- without the
+// range scaling (v_div_scale) and special-case fixup (v_div_fixup) it wraps
+// around every quotient: both are identities when b and a/b are normal
+// numbers well inside the exponent range and a is 0 or above 2^-100 in
+// magnitude, as the densities and momenta of a step are, so each quotient is
+// bitwise the one a / b gives there, and the reciprocal's refinement is
+// computed once per denominator (9 instructions less for the two velocity
+// quotients of a cell).  FP64 keeps the division operator.
+template <typename C>
+struct Quot {
+    C b;
+    __device__ __forceinline__ explicit Quot(C den) : b(den) {}
+    __device__ __forceinline__ C operator()(C a) const { return a / b; }
+};
+template <>
+struct Quot<float> {
+    float b, y;
+    __device__ __forceinline__ explicit Quot(float den) : b(den) {
+        const float y0 = __builtin_amdgcn_rcpf(den);
+        y = __builtin_fmaf(__builtin_fmaf(-den, y0, 1.0f), y0, y0);
+    }
+    __device__ __forceinline__ float operator()(float a) const {
+        const float q0 = a * y;
+        const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+        return __builtin_fmaf(__builtin_fmaf(-b, q1, a), y, q1);
+    }
+};
+
 // BGK collision of fuzed_pull_streaming_collision_local_unrolled_universal
 // (source/layouts/xyd/simulation.f90:224-293), same operation order.  f[]
 // holds the pulled populations on entry and the post-collision ones on exit.
@@ -443,8 +473,9 @@ __device__ __forceinline__ void collide(const StepArgs<C> &a, C (&f)[9], int x, 
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
     density_check(a, rho, act);
-    ux = ux / rho;
-    uy = uy / rho;
+    const Quot<C> by_rho(rho);
+    ux = by_rho(ux);
+    uy = by_rho(uy);
     const C usq = ux * ux + uy * uy;
     if (a.write_macro && act) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
@@ -485,8 +516,9 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
     C ux = f2 - f4 + f6 - f7 - f8 + f9;
     C uy = f3 - f5 + f6 + f7 - f8 - f9;
     density_check(a, rho, act);
-    ux = (ux + (C)0.5 * F) / rho;
-    uy = uy / rho;
+    const Quot<C> by_rho(rho);
+    ux = by_rho(ux + (C)0.5 * F);
+    uy = by_rho(uy);
     const C usq = ux * ux + uy * uy;
     if (a.write_macro && act) {
         const size_t m = (size_t)y * (size_t)a.nx + (size_t)x;
@@ -534,7 +566,7 @@ __device__ __forceinline__ void collide_dev(const StepArgs<C> &a, C (&D)[9], int
     const C drho = m0 * inv;
     const C rho = (C)1 + drho;
     density_check(a, rho, act);
-    const C r = (C)1 / rho;
+    const C r = Quot<C>(rho)((C)1);
     const C ux = (MODE == BF ? __builtin_fmaf(jx, inv, a.half_force) : jx * inv) * r;
     const C uy = (jy * inv) * r;
     if (a.write_macro && act) {
@@ -1040,11 +1072,58 @@ __device__ __forceinline__ void pair_tile(const StepArgs<C> &a, int tx, int ty,
 // a.step_seed + k - 1; step K writes the macro fields when asked.  Every cell
 // update is the single step's arithmetic, so the result is bitwise that of K
 // step_kernel launches.
+// A coordinate at most RING cells outside [0, n), wrapped into it (n >= 1 may
+// be smaller than RING).
+template <int RING>
+__device__ __forceinline__ int mt_wrap(int u, int n) {
+#pragma unroll
+    for (int k = 0; k < RING; ++k) u = u < 0 ? u + n : (u >= n ? u - n : u);
+    return u;
+}
+
+// Whether region cell (u, v) of a K-step tile takes part in step k: within
+// K - k cells of the domain on a periodic axis, inside it on a wall axis.
+template <int MODE, int K>
+__device__ __forceinline__ bool mt_live(int u, int v, int k, int nx, int ny) {
+    constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
+    const int e = K - k;
+    bool in = PX ? (u >= -e) & (u <= nx - 1 + e) : (u >= 0) & (u < nx);
+    in &= PY ? (v >= -e) & (v <= ny - 1 + e) : (v >= 0) & (v < ny);
+    return in;
+}
+
+// The HBM loads of step 1 of K-step tile (tx, ty): the stored elements each
+// region cell of this thread (region rows w, w + WAVES, ...) pulls, not yet
+// decoded (pull_raw).  Issued one tile ahead by stepk_pipe_kernel.
 template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
+__device__ __forceinline__ void multi_tile_loads(const StepArgs<C> &a, int tx, int ty, RawPop<S> (&raw)[RPT][9]) {
+    using R = typename Store<S>::R;
+    constexpr int ROWS = WAVES * RPT, RING = K - 1, OW = 64 - 2 * RING, OH = ROWS - 2 * RING;
+    constexpr bool PX = MODE != SL, PY = MODE == SW;
+    const int lane = (int)threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int ux = tx * OW - RING + lane, uy0 = ty * OH - RING;
+    const int gx = PX ? mt_wrap<RING>(ux, a.nx) : ux;
+    const R *__restrict__ f = static_cast<const R *>(a.src);
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int uy = uy0 + w + WAVES * j;
+        if (mt_live<MODE, K>(ux, uy, 1, a.nx, a.ny)) {
+            const int gy = PY ? mt_wrap<RING>(uy, a.ny) : uy;
+            pull_raw<S, C, MODE>(a, f, gx, gy, raw[j]);
+        }
+    }
+}
+
+// PIPE: step 1 takes its loaded elements from raw (multi_tile_loads of this
+// tile), and once every row of it has been computed the loads of tile
+// (ntx, nty) go into raw (none for ntx < 0), so that they are in flight
+// while steps 2 .. K run from the LDS.
+template <typename S, typename C, int MODE, int WAVES, int RPT, int K, bool PIPE = false>
 __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
-                                           typename Store<S>::R (&lds)[9][WAVES * RPT][64]) {
+                                           typename Store<S>::R (&lds)[9][WAVES * RPT][64],
+                                           RawPop<S> (&raw)[RPT][9], int ntx = -1, int nty = 0) {
     static_assert(MODE != PF, "Poiseuille flow takes its strip pass between steps");
-    static_assert(K >= 2 && K <= 4, "two to four steps per pass");
+    static_assert(K >= 2 && K <= 5, "two to five steps per pass");
     using R = typename Store<S>::R;
     constexpr int ROWS = WAVES * RPT, RING = K - 1, OW = 64 - 2 * RING, OH = ROWS - 2 * RING;
     constexpr bool PX = MODE != SL, PY = MODE == SW;  // periodic axes
@@ -1054,20 +1133,8 @@ __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
     const int uy0 = ty * OH - RING;        // unwrapped first region row
     const typename Store<S>::R *__restrict__ f = static_cast<const R *>(a.src);
     R *__restrict__ g = static_cast<R *>(a.dst);
-    // wrap a coordinate at most RING cells outside [0, n) (n >= 1 may be smaller than RING)
-    auto wrap = [](int u, int n) {
-#pragma unroll
-        for (int k = 0; k < RING; ++k) u = u < 0 ? u + n : (u >= n ? u - n : u);
-        return u;
-    };
-    // whether region cell (u, v) takes part in step k: within K - k of the tile's
-    // cells in the domain, and inside the domain on a wall axis
-    auto live = [&](int u, int v, int k) {
-        const int e = K - k;
-        bool in = PX ? (u >= -e) & (u <= nx - 1 + e) : (u >= 0) & (u < nx);
-        in &= PY ? (v >= -e) & (v <= ny - 1 + e) : (v >= 0) & (v < ny);
-        return in;
-    };
+    auto wrap = [](int u, int n) { return mt_wrap<RING>(u, n); };
+    auto live = [&](int u, int v, int k) { return mt_live<MODE, K>(u, v, k, nx, ny); };
     const int gx = PX ? wrap(ux, nx) : ux;
     // step 1 on the region
     {
@@ -1079,7 +1146,8 @@ __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
             if (live(ux, uy, 1)) {
                 const int gy = PY ? wrap(uy, ny) : uy;
                 C v[9];
-                pull_pops<S, C, MODE>(a1, f, gx, gy, v);
+                if constexpr (PIPE) raw_pops<S, C, MODE>(a1, raw[j], gy, v);
+                else pull_pops<S, C, MODE>(a1, f, gx, gy, v);
                 collide_mode<S, C, MODE>(a1, v, gx, gy);
                 R o[9];
                 encode_cell<S, C>(a1, v, gx, gy, o);
@@ -1087,6 +1155,9 @@ __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
                 for (int d = 0; d < 9; ++d) lds[d][r][lane] = o[d];
             }
         }
+    }
+    if constexpr (PIPE) {
+        if (ntx >= 0) multi_tile_loads<S, C, MODE, WAVES, RPT, K>(a, ntx, nty, raw);
     }
     __syncthreads();
     // A region inside the domain on every wall axis (most tiles) has no
@@ -1168,9 +1239,44 @@ __device__ __forceinline__ void multi_tile(const StepArgs<C> &a, int tx, int ty,
 template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
 __global__ void __launch_bounds__(64 * WAVES) stepk_kernel(StepArgs<C> a) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
+    RawPop<S> raw[RPT][9];  // unused without PIPE
     const int t = tile_of_block((int)blockIdx.x, a.n_blocks, a.xcd_group);
     const int ty = t / a.n_tiles_x;
-    multi_tile<S, C, MODE, WAVES, RPT, K>(a, t - ty * a.n_tiles_x, ty, lds);
+    multi_tile<S, C, MODE, WAVES, RPT, K>(a, t - ty * a.n_tiles_x, ty, lds, raw);
+}
+
+// K steps per launch over tiles by a persistent grid: workgroup p takes the
+// tiles of blocks p, p + gridDim.x, ... (gridDim.x a multiple of 8, so every
+// block it takes lies on its XCD in the grouped order of tile_of_block) and
+// issues the HBM loads of its next tile's step 1 before steps 2 .. K of the
+// current one (multi_tile, PIPE), so that a workgroup keeps loads in flight
+// through its LDS-only steps.  Results are those of stepk_kernel bit for bit
+// (same cell updates; only the time of the loads differs).
+template <typename S, typename C, int MODE, int WAVES, int RPT, int K>
+__global__ void __launch_bounds__(64 * WAVES) stepk_pipe_kernel(StepArgs<C> a) {
+    __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
+    RawPop<S> raw[RPT][9];
+    int p = (int)blockIdx.x;
+    if (p >= a.n_blocks) return;  // whole workgroup
+    auto coords = [&](int q, int &tx, int &ty) {
+        const int t = tile_of_block(q, a.n_blocks, a.xcd_group);
+        ty = t / a.n_tiles_x;
+        tx = t - ty * a.n_tiles_x;
+    };
+    int tx, ty;
+    coords(p, tx, ty);
+    multi_tile_loads<S, C, MODE, WAVES, RPT, K>(a, tx, ty, raw);
+    for (;;) {
+        const int q = p + (int)gridDim.x;
+        int ntx = -1, nty = 0;
+        if (q < a.n_blocks) coords(q, ntx, nty);
+        multi_tile<S, C, MODE, WAVES, RPT, K, true>(a, tx, ty, lds, raw, ntx, nty);
+        if (q >= a.n_blocks) break;
+        __syncthreads();  // step K has read the LDS before the next step 1 writes it
+        p = q;
+        tx = ntx;
+        ty = nty;
+    }
 }
 
 // a.n_blocks tiles; a grid smaller than that walks them (persistent

@@ -206,6 +206,53 @@ int launch_between(ffb_lattice *L, const void *src, void *dst, int write_macro) 
     }
 }
 
+// One K-step launch over a.n_blocks tiles: one workgroup per tile
+// (stepk_kernel), or with L->multi_pipe a persistent grid that loads each
+// workgroup's next tile during the LDS-only steps of the current one
+// (stepk_pipe_kernel): L->multi_grid workgroups, or as many as fit on the
+// device at once (occupancy x compute units, rounded up to a multiple of 8).
+template <typename S, typename C, int MODE, int WAVES, int RPT, int K, bool PIPE_OK = true>
+int launch_multi(ffb_lattice *L, const StepArgs<C> &a) {
+    const dim3 block(64 * WAVES);
+    if (PIPE_OK && L->multi_pipe) {
+        static int per_cu = -1;  // resident workgroups per CU, one query per kernel
+        if (per_cu < 0) {
+            int n = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&stepk_pipe_kernel<S, C, MODE, WAVES, RPT, K>), 64 * WAVES, 0));
+            per_cu = std::max(1, n);
+        }
+        // an explicit L->multi_grid is taken as given (any size is correct; a
+        // multiple of 8 keeps each workgroup's tiles on its XCD's group)
+        const int g = std::min(a.n_blocks, L->multi_grid > 0 ? L->multi_grid : ((per_cu * L->n_cu + 7) / 8) * 8);
+        hipLaunchKernelGGL((stepk_pipe_kernel<S, C, MODE, WAVES, RPT, K>), dim3((unsigned)g), block, L->lds_pad,
+                           L->stream, a);
+    } else {
+        hipLaunchKernelGGL((stepk_kernel<S, C, MODE, WAVES, RPT, K>), dim3((unsigned)a.n_blocks), block, L->lds_pad,
+                           L->stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    return FFB_OK;
+}
+
+// K steps in one launch over (64 - 2(K-1)) x (WAVES RPT - 2(K-1)) tiles
+// (stepk_kernel); PIPE_OK: the persistent variant can be selected.
+template <typename S, typename C, int WAVES, int RPT, int K, bool PIPE_OK = true>
+int launch_multik_shape(ffb_lattice *L, StepArgs<C> a) {
+    constexpr int OW = 64 - 2 * (K - 1), OH = WAVES * RPT - 2 * (K - 1);
+    a.n_tiles_x = (a.nx + OW - 1) / OW;
+    a.n_tiles_y = (a.ny + OH - 1) / OH;
+    a.n_blocks = a.n_tiles_x * a.n_tiles_y;
+    a.xcd_group = L->pair_xcd_group;
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: return launch_multi<S, C, SW, WAVES, RPT, K, PIPE_OK>(L, a);
+    case FFB_COUETTE_FLOW: return launch_multi<S, C, CF, WAVES, RPT, K, PIPE_OK>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_multi<S, C, BF, WAVES, RPT, K, PIPE_OK>(L, a);
+    case FFB_SLIDING_LID: return launch_multi<S, C, SL, WAVES, RPT, K, PIPE_OK>(L, a);
+    default: return fail(FFB_E_INVALID, "multi-step pass: Poiseuille flow is not supported");
+    }
+}
+
 // Two steps in one launch (step2_kernel): tiles of 62 x (WAVES RPT - 2) cells.
 template <typename S, typename C, int WAVES, int RPT, bool NT> int launch_step2_shape(ffb_lattice *L, StepArgs<C> a) {
     constexpr int ROWS = WAVES * RPT;
@@ -276,6 +323,8 @@ template <typename S, typename C> int launch_step2(ffb_lattice *L, int write_mac
     if (L->pair_kernel == 1 && L->layout == 1) return launch_march<S, C>(L, a);  // row-blocked layout only
     const bool nt = L->pair_nt != 0;
     if constexpr (sizeof(typename Store<S>::R) <= 4) {
+        // 8 waves x 4 rows (64 x 32, 62 x 30 tiles) by stepk_kernel
+        if (L->pair_shape == 3) return launch_multik_shape<S, C, 8, 4, 2, false>(L, a);
         if (L->pair_shape == 2)
             return nt ? launch_step2_shape<S, C, 8, 2, true>(L, a) : launch_step2_shape<S, C, 8, 2, false>(L, a);
         return nt ? launch_step2_shape<S, C, 16, 2, true>(L, a) : launch_step2_shape<S, C, 16, 2, false>(L, a);
@@ -291,16 +340,13 @@ template <typename S, typename C, int WAVES, int RPT> int launch_step3_shape(ffb
     a.n_tiles_y = (a.ny + ROWS - 5) / (ROWS - 4);
     a.n_blocks = a.n_tiles_x * a.n_tiles_y;
     a.xcd_group = L->pair_xcd_group;
-    const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
     switch (L->cfg.sim_mode) {
-    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((stepk_kernel<S, C, SW, WAVES, RPT, 3>), grid, block, 0, L->stream, a); break;
-    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((stepk_kernel<S, C, CF, WAVES, RPT, 3>), grid, block, 0, L->stream, a); break;
-    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((stepk_kernel<S, C, BF, WAVES, RPT, 3>), grid, block, 0, L->stream, a); break;
-    case FFB_SLIDING_LID: hipLaunchKernelGGL((stepk_kernel<S, C, SL, WAVES, RPT, 3>), grid, block, 0, L->stream, a); break;
+    case FFB_SHEAR_WAVE: return launch_multi<S, C, SW, WAVES, RPT, 3>(L, a);
+    case FFB_COUETTE_FLOW: return launch_multi<S, C, CF, WAVES, RPT, 3>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_multi<S, C, BF, WAVES, RPT, 3>(L, a);
+    case FFB_SLIDING_LID: return launch_multi<S, C, SL, WAVES, RPT, 3>(L, a);
     default: return fail(FFB_E_INVALID, "three-step pass: Poiseuille flow is not supported");
     }
-    HIPCHK(hipGetLastError());
-    return FFB_OK;
 }
 
 // Region shapes of the triples: 2 = 8 waves x 2 rows (64 x 16 cells, 60 x 12
@@ -308,6 +354,9 @@ template <typename S, typename C, int WAVES, int RPT> int launch_step3_shape(ffb
 template <typename S, typename C> int launch_step3(ffb_lattice *L, int write_macro) {
     const StepArgs<C> a = make_args<C>(L, write_macro);
     if (L->triple_shape == 1) return launch_step3_shape<S, C, 16, 2>(L, a);
+    if constexpr (sizeof(typename Store<S>::R) <= 4) {
+        if (L->triple_shape == 3) return launch_multik_shape<S, C, 8, 4, 3, false>(L, a);  // 64 x 32, 60 x 28 tiles
+    }
     return launch_step3_shape<S, C, 8, 2>(L, a);
 }
 
@@ -320,23 +369,52 @@ template <typename S, typename C, int WAVES, int RPT> int launch_step4_shape(ffb
     a.n_tiles_y = (a.ny + ROWS - 7) / (ROWS - 6);
     a.n_blocks = a.n_tiles_x * a.n_tiles_y;
     a.xcd_group = L->pair_xcd_group;
-    const dim3 grid((unsigned)a.n_blocks), block(64 * WAVES);
     switch (L->cfg.sim_mode) {
-    case FFB_SHEAR_WAVE: hipLaunchKernelGGL((stepk_kernel<S, C, SW, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
-    case FFB_COUETTE_FLOW: hipLaunchKernelGGL((stepk_kernel<S, C, CF, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
-    case FFB_BODY_FORCE_CHANNEL: hipLaunchKernelGGL((stepk_kernel<S, C, BF, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
-    case FFB_SLIDING_LID: hipLaunchKernelGGL((stepk_kernel<S, C, SL, WAVES, RPT, 4>), grid, block, 0, L->stream, a); break;
+    case FFB_SHEAR_WAVE: return launch_multi<S, C, SW, WAVES, RPT, 4>(L, a);
+    case FFB_COUETTE_FLOW: return launch_multi<S, C, CF, WAVES, RPT, 4>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_multi<S, C, BF, WAVES, RPT, 4>(L, a);
+    case FFB_SLIDING_LID: return launch_multi<S, C, SL, WAVES, RPT, 4>(L, a);
     default: return fail(FFB_E_INVALID, "four-step pass: Poiseuille flow is not supported");
     }
-    HIPCHK(hipGetLastError());
-    return FFB_OK;
 }
 
 template <typename S, typename C> int launch_step4(ffb_lattice *L, int write_macro) {
     const StepArgs<C> a = make_args<C>(L, write_macro);
     if (L->quad_shape == 1) return launch_step4_shape<S, C, 16, 2>(L, a);
     if (L->quad_shape == 3) return launch_step4_shape<S, C, 8, 2>(L, a);
+    if constexpr (sizeof(typename Store<S>::R) <= 4) {
+        if (L->quad_shape == 4) return launch_multik_shape<S, C, 8, 4, 4, false>(L, a);  // 64 x 32, 58 x 26 tiles
+    }
     return launch_step4_shape<S, C, 12, 2>(L, a);
+}
+
+// Five steps in one launch (4-byte elements): tiles of 56 x (WAVES RPT - 8)
+// cells; regions 1 = 8 waves x 4 rows (64 x 32, 56 x 24 tiles, the default),
+// 2 = 12 waves x 2 rows (64 x 24, 56 x 16).
+template <typename S, typename C, int WAVES, int RPT> int launch_step5_shape(ffb_lattice *L, StepArgs<C> a) {
+    constexpr int ROWS = WAVES * RPT;
+    a.n_tiles_x = (a.nx + 55) / 56;
+    a.n_tiles_y = (a.ny + ROWS - 9) / (ROWS - 8);
+    a.n_blocks = a.n_tiles_x * a.n_tiles_y;
+    a.xcd_group = L->pair_xcd_group;
+    switch (L->cfg.sim_mode) {
+    case FFB_SHEAR_WAVE: return launch_multi<S, C, SW, WAVES, RPT, 5>(L, a);
+    case FFB_COUETTE_FLOW: return launch_multi<S, C, CF, WAVES, RPT, 5>(L, a);
+    case FFB_BODY_FORCE_CHANNEL: return launch_multi<S, C, BF, WAVES, RPT, 5>(L, a);
+    case FFB_SLIDING_LID: return launch_multi<S, C, SL, WAVES, RPT, 5>(L, a);
+    default: return fail(FFB_E_INVALID, "five-step pass: Poiseuille flow is not supported");
+    }
+}
+
+int dispatch_step5(ffb_lattice *L, int write_macro) {
+    if (L->cfg.precision == FFB_PREC_FP32_DEV) {
+        const StepArgs<float> a = make_args<float>(L, write_macro);
+        return L->penta_shape == 2 ? launch_step5_shape<f32dev, float, 12, 2>(L, a)
+                                   : launch_step5_shape<f32dev, float, 8, 4>(L, a);
+    }
+    const StepArgs<float> a = make_args<float>(L, write_macro);
+    return L->penta_shape == 2 ? launch_step5_shape<float, float, 12, 2>(L, a)
+                               : launch_step5_shape<float, float, 8, 4>(L, a);
 }
 
 int dispatch_step4(ffb_lattice *L, int write_macro) {
@@ -705,14 +783,37 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // deviations: FP32 4096^2 205,200 against 191,900 MLUPS in triples, FP32
     // deviations 8192^2 223,900 against 199,100; FP64 110,600 against 107,900
     // with 64 x 16 regions (its 64 x 24 regions hold one workgroup per CU:
-    // 92,200), so FP64 and the 16-bit forms keep triples.
+    // 92,200), so FP64 and the 16-bit forms keep triples.  Five steps (8 waves
+    // x 4 rows, 64 x 32 regions, 56 x 24 tiles; 4-byte elements only), the
+    // default for FP32 and FP32 deviations: FP32 4096^2 215,400 against 205,600
+    // in fours, FP32 deviations 8192^2 249,900 against 224,300 (one box,
+    // alternating; 12 x 2 regions 205,300 / 242,700, 16 x 2 157,000 / 174,700).
+    // The 16-bit forms run 64 x 32 regions of 8 waves x 4 rows (below):
+    // FP16 deviations 8192^2 230,400 MLUPS in triples against 196,400 with 64 x
+    // 16 regions (223,300 in pairs, 224,000 in fours); the 1-3-12 form 206,200
+    // in pairs against 200,500 in triples and 174,400 with 64 x 16 triples.
     {
         const bool four = cfg->precision == FFB_PREC_FP32 || cfg->precision == FFB_PREC_FP32_DEV;
-        L->steps_per_pass = (int)std::max(1LL, std::min(4LL, env_ll("FFB_STEPS_PER_PASS", four ? 4 : 3)));
+        const int dflt = four ? 5 : (cfg->precision == FFB_PREC_E3M12_DEV ? 2 : 3);
+        L->steps_per_pass = (int)std::max(1LL, std::min(four ? 5LL : 4LL, env_ll("FFB_STEPS_PER_PASS", dflt)));
     }
     L->quad_shape = (int)env_ll("FFB_QUAD_SHAPE", 2);
+    L->penta_shape = (int)env_ll("FFB_PENTA_SHAPE", 1);
+    // LDS bytes added to each multi-step workgroup (caps the workgroups per CU)
+    L->lds_pad = (unsigned)std::max(0LL, std::min(65536LL, env_ll("FFB_MULTI_LDS_PAD", 0)));
+    // multi-step launches (three or four steps) by a persistent grid that
+    // loads each workgroup's next tile while it runs the current one's LDS
+    // steps (stepk_pipe_kernel), FFB_MULTI_PIPE; FFB_MULTI_GRID workgroups
+    // (0: as many as fit at once)
+    L->multi_pipe = (int)env_ll("FFB_MULTI_PIPE", 0) ? 1 : 0;
+    L->multi_grid = (int)std::max(0LL, env_ll("FFB_MULTI_GRID", 0));
+    {
+        int n_cu = 0;  // a failed query leaves a grid of occupancy x 8 workgroups: slower, still correct
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, cfg->device) != hipSuccess) n_cu = 8;
+        L->n_cu = std::max(1, n_cu);
+    }
     if (L->steps_per_pass == 1) L->two_step = 0;
-    L->triple_shape = (int)env_ll("FFB_TRIPLE_SHAPE", 2);
+    L->triple_shape = (int)env_ll("FFB_TRIPLE_SHAPE", is_16bit_form(cfg->precision) ? 3 : 2);
     // pair tiles per XCD group, by the width of the domain in 62-cell tiles:
     // 8 up to 96 tiles per row, else 32.  Measured on one box, us per pair
     // launch: FP32 4096^2 (67 tiles per row) 241 at 8, 243 at 12, 247-249 at
@@ -730,7 +831,9 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // for FP32 (4096^2: 248-254 against 259-268 us per pair; 8192^2 995 against
     // 1033) and the 16-bit forms (FP16 8192^2 683 against 771), shape 1 for FP32
     // deviations (8192^2 866 against 943).
-    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", cfg->precision == FFB_PREC_FP32_DEV ? 1 : 2);
+    L->pair_shape = (int)env_ll("FFB_PAIR_SHAPE", is_16bit_form(cfg->precision)
+                                                      ? 3
+                                                      : (cfg->precision == FFB_PREC_FP32_DEV ? 1 : 2));
     L->pair_nt = (int)env_ll("FFB_PAIR_NT", 0);
     // pair workgroups: 0 = one per tile, else a persistent grid of this size
     L->pair_grid = (int)std::max(0LL, env_ll("FFB_PAIR_GRID", 0));
@@ -827,7 +930,18 @@ int ffb_step(ffb_lattice_t *L, int32_t n_steps, int32_t write_macro_last) {
     const bool quads = pairs && L->steps_per_pass >= 4 &&
                        (L->cfg.precision == FFB_PREC_FP32 || L->cfg.precision == FFB_PREC_FP64 ||
                         L->cfg.precision == FFB_PREC_FP32_DEV || L->cfg.precision == FFB_PREC_FP16_DEV);
+    const bool pentas = quads && L->steps_per_pass >= 5 &&
+                        (L->cfg.precision == FFB_PREC_FP32 || L->cfg.precision == FFB_PREC_FP32_DEV);
     for (int32_t k = 0; k < n_steps;) {
+        if (pentas && n_steps - k >= 5) {  // steps k+1 .. k+5 in one pass
+            const int wm = (write_macro_last && k + 5 == n_steps) ? 1 : 0;
+            int rc = dispatch_step5(L, wm);
+            if (rc) return rc;
+            L->cur = 1 - L->cur;
+            L->steps += 5;
+            k += 5;
+            continue;
+        }
         if (quads && n_steps - k >= 4) {  // steps k+1 .. k+4 in one pass
             const int wm = (write_macro_last && k + 4 == n_steps) ? 1 : 0;
             int rc = dispatch_step4(L, wm);

@@ -188,6 +188,7 @@ __global__ void __launch_bounds__(TW)
 template <typename S, typename C, int WAVES, int RPT, bool NT, int K>
 __global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, const PTile *tiles, int n, int G) {
     __shared__ typename Store<S>::R lds[9][WAVES * RPT][64];
+    RawPop<S> raw[RPT][9];  // multi_tile's pipelined form only
     for (int p = (int)blockIdx.x; p < n; p += (int)gridDim.x) {
         const int t = tile_of_block(p, n, G);
         const PTile pt = load_uniform(tiles + t);
@@ -202,10 +203,10 @@ __global__ void __launch_bounds__(64 * WAVES) batch_pair_kernel(BatchArgs<C> b, 
             }
         } else {
             switch (L.kind) {
-            case SW: multi_tile<S, C, SW, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
-            case CF: multi_tile<S, C, CF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
-            case SL: multi_tile<S, C, SL, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
-            default: multi_tile<S, C, BF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds); break;
+            case SW: multi_tile<S, C, SW, WAVES, RPT, K>(a, pt.tx, pt.ty, lds, raw); break;
+            case CF: multi_tile<S, C, CF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds, raw); break;
+            case SL: multi_tile<S, C, SL, WAVES, RPT, K>(a, pt.tx, pt.ty, lds, raw); break;
+            default: multi_tile<S, C, BF, WAVES, RPT, K>(a, pt.tx, pt.ty, lds, raw); break;
             }
         }
         if (p + (int)gridDim.x < n) __syncthreads();  // the next tile rewrites the LDS
@@ -270,7 +271,7 @@ struct ffb_batch {
     void *segs = nullptr;     // Seg per row segment: (lattice, row)
     int n_tiles = 0;
     bool pairs = false;       // two or three steps per launch (batch_pair_kernel)
-    int per_pass = 1;         // steps per launch: 1, 2 (pairs) or 3 (triples)
+    int per_pass = 1;         // steps per launch: 1, or 2 .. 5 (multi-step launches)
     void *ptiles = nullptr;   // PTile per multi-step tile (capacity ptile_cap)
     int n_ptiles = 0, ptile_cap = 0;
     int ptile_shape = 0;      // 4 pair_shape + steps per launch of the tile table
@@ -296,13 +297,15 @@ int grid_for(const ffb_batch *B) {
 }
 
 // region rows of a multi-step tile: 16 (8 waves x 2 rows), 32 (16 waves x 2
-// rows, shape 1); four steps 24 (12 waves x 2 rows) or 32; FP64 pairs 8 (8
-// waves x 1 row), FP64 triples 16.  Output tiles of K steps are
-// (64 - 2(K-1)) x (rows - 2(K-1)): pairs 62 x 14 / 62 x 30 (FP64 62 x 6),
-// triples 60 x 12 / 60 x 28 (FP64 60 x 12), four steps 58 x 18 / 58 x 26.
+// rows, shape 1); four steps 24 (12 waves x 2 rows) or 32; five steps 24 (12
+// waves x 2 rows) or 32 (8 waves x 4 rows, shape 1); FP64 pairs 8 (8 waves x 1
+// row), FP64 triples 16.  Output tiles of K steps are (64 - 2(K-1)) x
+// (rows - 2(K-1)): pairs 62 x 14 / 62 x 30 (FP64 62 x 6), triples 60 x 12 /
+// 60 x 28 (FP64 60 x 12), four steps 58 x 18 / 58 x 26, five steps 56 x 16 /
+// 56 x 24.
 int region_rows(const ffb_batch *B, int shape, int k) {
     if (B->elem == 8) return k >= 3 ? 16 : 8;
-    if (k == 4) return shape == 1 ? 32 : 24;
+    if (k >= 4) return shape == 1 ? 32 : 24;
     return shape == 1 ? 32 : 16;
 }
 
@@ -376,7 +379,7 @@ template <typename C> int upload_tables(ffb_batch *B) {
         L.half_force = (C)0.5 * L.force;
     }
     size_t cap = 0;
-    for (int k = 2; k <= 4; ++k)
+    for (int k = 2; k <= 5; ++k)
         for (int sh = 1; sh <= 2; ++sh) cap = std::max(cap, pair_tiles(B, sh, k).size());
     B->ptile_cap = (int)cap;
     BHIPCHK(hipMalloc(&B->ptiles, (size_t)B->ptile_cap * sizeof(PTile)));
@@ -475,15 +478,22 @@ void launch_pair_c(const ffb_batch *B, int parity, int write_macro) {
 }
 
 template <typename S, typename C, int K> void launch_pair_shape(const ffb_batch *B, int parity, int write_macro) {
-    if (B->tuning.pair_shape == 1) launch_pair_c<S, C, 16, 2, false, K>(B, parity, write_macro);
-    else if constexpr (K == 4) launch_pair_c<S, C, 12, 2, false, K>(B, parity, write_macro);
-    else launch_pair_c<S, C, 8, 2, false, K>(B, parity, write_macro);
+    if (B->tuning.pair_shape == 1) {  // 64 x 32 regions
+        if constexpr (K >= 3) launch_pair_c<S, C, 8, 4, false, K>(B, parity, write_macro);
+        else launch_pair_c<S, C, 16, 2, false, K>(B, parity, write_macro);
+    } else if constexpr (K == 5) {
+        launch_pair_c<S, C, 12, 2, false, K>(B, parity, write_macro);
+    } else if constexpr (K == 4) {
+        launch_pair_c<S, C, 12, 2, false, K>(B, parity, write_macro);
+    } else {
+        launch_pair_c<S, C, 8, 2, false, K>(B, parity, write_macro);
+    }
 }
 
 template <int K> void launch_pair_k(const ffb_batch *B, int parity, int write_macro) {
-    if constexpr (K == 4) {  // 4-byte forms only (per_pass is at most 3 for the others)
-        if (B->precision == FFB_PREC_FP32_DEV) launch_pair_shape<f32dev, float, 4>(B, parity, write_macro);
-        else launch_pair_shape<float, float, 4>(B, parity, write_macro);
+    if constexpr (K >= 4) {  // 4-byte forms only (per_pass is at most 3 for the others)
+        if (B->precision == FFB_PREC_FP32_DEV) launch_pair_shape<f32dev, float, K>(B, parity, write_macro);
+        else launch_pair_shape<float, float, K>(B, parity, write_macro);
         return;
     }
     switch (B->precision) {
@@ -503,7 +513,8 @@ template <int K> void launch_pair_k(const ffb_batch *B, int parity, int write_ma
 // single domain's pairs (FP64 pairs: nontemporal); regions 8 waves x 2 rows or
 // 16 waves x 2 rows, FP64 pairs 8 waves x 1 row, FP64 triples 8 waves x 2 rows.
 void launch_pair(const ffb_batch *B, int parity, int write_macro) {
-    if (B->per_pass >= 4) launch_pair_k<4>(B, parity, write_macro);
+    if (B->per_pass >= 5) launch_pair_k<5>(B, parity, write_macro);
+    else if (B->per_pass == 4) launch_pair_k<4>(B, parity, write_macro);
     else if (B->per_pass == 3) launch_pair_k<3>(B, parity, write_macro);
     else launch_pair_k<2>(B, parity, write_macro);
     count_steps(B, (unsigned)B->per_pass);
@@ -581,7 +592,9 @@ ffb_batch_tuning_t ffb_batch_default_tuning(const ffb_batch *B) {
     ffb_batch_tuning_t t{};
     t.tile_w = 256;
     t.grid_blocks = 0;
-    t.pair_shape = env("FFB_BATCH_PAIR_SHAPE", 2) == 1 ? 1 : 2;
+    // region shape: three or more steps 64 x 32 regions of 8 waves x 4 rows
+    // (shape 1), pairs 64 x 16 of 8 waves x 2 rows (shape 2)
+    t.pair_shape = env("FFB_BATCH_PAIR_SHAPE", B->per_pass >= 3 ? 1 : 2) == 1 ? 1 : 2;
     t.pair_group = std::max(0, env("FFB_BATCH_PAIR_GROUP", B->elem >= 8 ? 4 : 32));
     t.pair_grid = std::max(0, env("FFB_BATCH_PAIR_GRID", 0));
     return t;
@@ -638,16 +651,19 @@ int ffb_batch_create(const ffb_batch_lattice_t *lattices, int32_t count, int32_t
     // in pairs against 83,200; FP32 deviations 127,000 / 83,700; FP64 63,000 /
     // 39,100; FP16 127,000 in pairs against 144,100, BF16 129,000 / 141,900,
     // 1-3-12 120,500 / 138,200); FFB_BATCH_PAIRS=0/1 at creation overrides
-    // and multi-step launches of 2..4 steps (FFB_BATCH_STEPS_PER_PASS; default
-    // 4 for the 4-byte forms, 3 for FP64 when multi-step launches are on; four
-    // steps only for the 4-byte forms)
+    // and multi-step launches of 2..5 steps (FFB_BATCH_STEPS_PER_PASS; default
+    // 5 for the 4-byte forms, 3 for FP64 and the 16-bit forms; four and five
+    // steps only for the 4-byte forms).  512 mixed lattices, autotuned, one
+    // box: FP32 five steps 169,700-178,000 MLUPS against 150,700-168,200 with
+    // four; FP16 deviations three steps in 64 x 32 regions (8 waves x 4 rows)
+    // 177,500 against 144,100 with one launch per step (129,800 in pairs).
     {
         const char *e = std::getenv("FFB_BATCH_PAIRS");
-        B->pairs = e && *e ? std::atoi(e) != 0 : B->elem >= 4;
+        B->pairs = e && *e ? std::atoi(e) != 0 : true;
         const char *k = std::getenv("FFB_BATCH_STEPS_PER_PASS");
-        int kk = k && *k ? std::atoi(k) : (B->pairs ? (B->elem == 4 ? 4 : 3) : 1);
+        int kk = k && *k ? std::atoi(k) : (B->pairs ? (B->elem == 4 ? 5 : 3) : 1);
         if (!(e && *e) && k && *k) B->pairs = kk >= 2;  // the steps per pass alone selects
-        kk = std::max(1, std::min(B->elem == 4 ? 4 : 3, kk));
+        kk = std::max(1, std::min(B->elem == 4 ? 5 : 3, kk));
         B->per_pass = B->pairs ? std::max(2, kk) : 1;
     }
     B->desc.assign(lattices, lattices + count);

@@ -48,11 +48,17 @@ struct ffb_lattice {
     int xcd_group = 0;       // G consecutive tiles per XCD, 0: plain order (tile_of_block)
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
     int steps_per_pass = 2;  // 3: step triples (stepk_kernel) where pairs are allowed
-    int triple_shape = 2;    // triple region: 2 = 8 waves x 2 rows, 1 = 16 waves x 2 rows
-    int quad_shape = 2;      // four-step region: 2 = 12 waves x 2 rows, 1 = 16 x 2, 3 = 8 x 2
+    int triple_shape = 2;    // triple region: 2 = 8 waves x 2 rows, 1 = 16 waves x 2 rows, 3 = 8 x 4 (<= 4-byte elements)
+    int quad_shape = 2;      // four-step region: 2 = 12 waves x 2 rows, 1 = 16 x 2, 3 = 8 x 2, 4 = 8 x 4 (<= 4-byte elements)
+    int penta_shape = 1;     // five-step region (4-byte elements): 1 = 8 waves x 4 rows, 2 = 12 x 2
+    unsigned lds_pad = 0;    // dynamic LDS bytes added to multi-step launches (fewer workgroups per CU)
+    int multi_pipe = 0;      // 1: three/four-step launches by stepk_pipe_kernel (persistent, loads one tile ahead)
+    int multi_grid = 0;      // stepk_pipe_kernel workgroups, 0: occupancy x compute units
+    int n_cu = 1;            // compute units of the device
     int pair_xcd_group = 0;  // G consecutive pair tiles per XCD, 0: plain order
     int pair_xcd_rows = 1;   // H > 1: blocks of G x H pair tiles per XCD
-    int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows
+    int pair_shape = 1;      // pair region (4-byte and narrower): 1 = 16 waves x 2 rows, 2 = 8 waves x 2 rows,
+                             // 3 = 8 waves x 4 rows (stepk_kernel)
     int pair_nt = 1;         // pair stores: 1 nontemporal, 0 default cache policy
     int pair_grid = 0;       // pair workgroups: 0 = one per tile, else a persistent grid
     int pair_kernel = 0;     // step pairs: 0 = tiles (step2_kernel), 1 = column march (march2_kernel)

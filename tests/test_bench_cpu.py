@@ -25,7 +25,7 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     monkeypatch.delenv("FFB_TWO_STEP", raising=False)
     monkeypatch.delenv("FFB_STEPS_PER_PASS", raising=False)
     forms = ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev")
-    assert [bench.steps_per_pass(p) for p in forms] == [4, 3, 3, 3, 4, 3]
+    assert [bench.steps_per_pass(p) for p in forms] == [5, 3, 3, 3, 5, 2]
     with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0" and bench.steps_per_pass("fp32") == 1
     assert "FFB_TWO_STEP" not in os.environ and "FFB_STEPS_PER_PASS" not in os.environ
@@ -33,12 +33,15 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
         assert os.environ["FFB_STEPS_PER_PASS"] == "2" and bench.steps_per_pass("fp64") == 2
     monkeypatch.setenv("FFB_TWO_STEP", "1")
     with bench.pass_mode(None):
-        assert bench.steps_per_pass("e3m12dev") == 3 and bench.steps_per_pass("fp32") == 4
+        assert bench.steps_per_pass("e3m12dev") == 2 and bench.steps_per_pass("fp32") == 5
     with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0"
     assert os.environ["FFB_TWO_STEP"] == "1" and "FFB_STEPS_PER_PASS" not in os.environ
     monkeypatch.setenv("FFB_STEPS_PER_PASS", "4")  # four steps only where a four-step kernel exists
     assert bench.steps_per_pass("fp64") == 4 and bench.steps_per_pass("bf16dev") == 3
+    monkeypatch.setenv("FFB_STEPS_PER_PASS", "5")  # five steps only for the 4-byte forms
+    assert bench.steps_per_pass("fp32dev") == 5 and bench.steps_per_pass("fp64") == 4
+    assert bench.steps_per_pass("e3m12dev") == 3
     monkeypatch.setenv("FFB_TWO_STEP", "0")  # one launch per step wins over the steps per pass
     monkeypatch.setenv("FFB_STEPS_PER_PASS", "3")
     assert bench.steps_per_pass("fp32") == 1
@@ -47,8 +50,8 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
 def test_batch_steps_per_pass_selection(monkeypatch):
     monkeypatch.delenv("FFB_BATCH_PAIRS", raising=False)
     monkeypatch.delenv("FFB_BATCH_STEPS_PER_PASS", raising=False)
-    assert [bench.batch_steps_per_pass(p) for p in ("fp32", "fp64", "fp32dev")] == [4, 3, 4]
-    assert bench.batch_steps_per_pass("e3m12dev") == 1 and bench.batch_steps_per_pass("fp16dev") == 1
+    assert [bench.batch_steps_per_pass(p) for p in ("fp32", "fp64", "fp32dev")] == [5, 3, 5]
+    assert bench.batch_steps_per_pass("e3m12dev") == 3 and bench.batch_steps_per_pass("fp16dev") == 3
     monkeypatch.setenv("FFB_BATCH_STEPS_PER_PASS", "2")
     assert bench.batch_steps_per_pass("fp32") == 2 and bench.batch_steps_per_pass("fp16dev") == 2
     monkeypatch.setenv("FFB_BATCH_PAIRS", "0")
@@ -102,3 +105,41 @@ def test_profile_segments_follow_the_markers():
             ("copy_kernel<1>", m1, 1), ("batch_count_kernel", 1, 0.1), ("batch_pair_kernel", 9, 7.0),
             ("copy_kernel<2>", m2, 1)]
     assert bench._segments(disp) == [("step2_kernel<a>", 101.0, 2), ("batch_pair_kernel", 7.0, 1)]
+
+
+def test_valu_record_from_counters(monkeypatch, tmp_path):
+    """--record-valu: per workload the mean SQ_INSTS_VALU of its kernel's
+    dispatches and busy = insts * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8)."""
+    m1, m2 = bench.MARK_START // 16, bench.MARK_END // 16
+    monkeypatch.setattr(bench, "_profile_workloads", lambda: [("w1", None), ("w2", None)])
+    rows, did = [], [0]
+
+    def dispatch(kernel, grid, vals):
+        did[0] += 1
+        for c in bench.VALU_COUNTERS:
+            rows.append(f"{did[0]},{kernel},{grid},{c},{vals.get(c, 0.0)}")
+
+    cnt = {"SQ_INSTS_VALU": 1.0e8, "SQ_ACTIVE_INST_VALU": 2.0e7, "SQ_WAVES": 1000.0,
+           "GRBM_GUI_ACTIVE": 8 * 500000.0, "SQ_BUSY_CYCLES": 1.0}
+    dispatch("copy_kernel<1>", m1, {})
+    dispatch("stepk_kernel<a>", 10, cnt)
+    dispatch("stepk_kernel<a>", 10, {**cnt, "SQ_INSTS_VALU": 1.2e8})
+    dispatch("copy_kernel<1>", m2, {})
+    dispatch("copy_kernel<1>", m1, {})
+    dispatch("batch_pair_kernel", 9, cnt)
+    dispatch("copy_kernel<1>", m2, {})
+    path = tmp_path / "sq.csv"
+    path.write_text("Dispatch_Id,Kernel_Name,Grid_Size,Counter_Name,Counter_Value\n" + "\n".join(rows) + "\n")
+    monkeypatch.setattr(bench, "VALU_FILE", str(tmp_path / "valu.txt"))
+    assert bench.record_valu(str(path)) == 0
+    v = bench.valu_from_profiles("w1")
+    assert v["wave_insts_per_launch"] == 1.1e8 and v["kernel"] == "stepk_kernel<a>"
+    assert abs(v["busy"] - round(1.1e8 * 4 / (1024 * 500000.0), 3)) < 1e-9
+    text = (tmp_path / "valu.txt").read_text().replace(bench.kernel_source_hash(), "0123456789abcdef")
+    (tmp_path / "valu.txt").write_text(text)
+    assert bench.valu_from_profiles("w1") is None  # stamped with other sources: refused
+
+
+def test_committed_valu_file_matches_the_sources():
+    """profiles/valu.txt is re-taken with profiles/traffic.txt."""
+    assert bench.valu_from_profiles("sl4096_fp32") is not None

@@ -43,14 +43,14 @@ def oracle_of(L, dtype, steps, scale=0):
     return OracleLattice(L.kind, L.n_x, L.n_y, p, dtype).init().run(steps)
 
 
-# (mode, tuning): "4" four steps per launch (4-byte forms; three for the
-# others), "3" step triples, "2" step pairs, with tile shape / XCD group /
+# (mode, tuning): "5" / "4" five / four steps per launch (4-byte forms; three
+# for the others), "3" step triples, "2" step pairs, with tile shape / XCD group /
 # persistent grid (forced for every form), or "0" one launch per step
 # with workgroup size / persistent grid
 MULTI = [{"pair_shape": 2, "pair_group": 32, "pair_grid": 0},
          {"pair_shape": 1, "pair_group": 3, "pair_grid": 8},
          {"pair_shape": 2, "pair_group": 0, "pair_grid": 40}]
-TUNINGS = [("4", t) for t in MULTI] + [("3", t) for t in MULTI] + [("2", t) for t in MULTI] + [
+TUNINGS = [("5", t) for t in MULTI] + [("4", t) for t in MULTI] + [("3", t) for t in MULTI] + [("2", t) for t in MULTI] + [
     ("0", {"tile_w": 256, "grid_blocks": 0}),
     ("0", {"tile_w": 128, "grid_blocks": 8}),
     ("0", {"tile_w": 512, "grid_blocks": 3}),
@@ -62,7 +62,7 @@ TUNINGS = [("4", t) for t in MULTI] + [("3", t) for t in MULTI] + [("2", t) for 
                          ids=[f"k{p}-" + "-".join(f"{k}{v}" for k, v in t.items()) for p, t in TUNINGS])
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32dev", "fp16dev", "bf16dev", "e3m12dev"])
 def test_batch_matches_oracle_per_lattice(monkeypatch, precision, mode, tuning):
-    """mode "3"/"2": three or two steps per launch (the tuning applies to the
+    """mode "5"/"4"/"3"/"2": five to two steps per launch (the tuning applies to the
     multi-step kernel; a remainder takes single steps); "0": one launch per
     step everywhere."""
     from fastfortranboltzmann_amd.batch import Batch

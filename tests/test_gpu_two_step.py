@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 MODES = {1: "shear wave", 2: "Couette", 4: "sliding lid", 5: "body-force channel"}
 FORMS = [("fp64", 0), ("fp32", 0), ("fp32dev", 0), ("fp16dev", 10), ("bf16dev", 2), ("e3m12dev", 3)]
-SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15), (70, 67)]
+SHAPES = [(1, 1), (2, 3), (5, 8), (63, 17), (130, 31), (125, 15), (70, 67), (300, 101)]
 
 
 def _run(monkeypatch, pairs, mode, nx, ny, precision, scale, f0, steps, env=None):
@@ -63,7 +63,25 @@ KINDS = {"shape8x2": {**PAIRS, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 2},
          # run triples under the same setting): regions 64 x 24, 64 x 32, 64 x 16
          "quad12x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 2},
          "quad16x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 1},
-         "quad8x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 3}}
+         "quad8x2": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 3},
+         # 8 waves x 4 rows (64 x 32 regions) by stepk_kernel for two, three and
+         # four steps (4-byte and narrower elements; FP64 takes its default shape)
+         "pair8x4": {"FFB_STEPS_PER_PASS": 2, "FFB_PAIR_KERNEL": 0, "FFB_PAIR_SHAPE": 3},
+         "triple8x4": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 3},
+         "quad8x4": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 4},
+         # five steps per launch (FP32 and FP32 deviations; the other forms run
+         # four): regions 64 x 32 (8 waves x 4 rows) and 64 x 24 (12 waves x 2 rows)
+         "penta8x4": {"FFB_STEPS_PER_PASS": 5, "FFB_PENTA_SHAPE": 1},
+         "penta12x2": {"FFB_STEPS_PER_PASS": 5, "FFB_PENTA_SHAPE": 2},
+         # persistent grids that load the next tile during the LDS steps
+         # (stepk_pipe_kernel): the default grid, and 1 or 3 workgroups, so
+         # that each workgroup walks several tiles
+         "triple_pipe": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 2, "FFB_MULTI_PIPE": 1},
+         "triple_pipe_g3": {"FFB_STEPS_PER_PASS": 3, "FFB_TRIPLE_SHAPE": 2, "FFB_MULTI_PIPE": 1, "FFB_MULTI_GRID": 3},
+         "quad_pipe": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 2, "FFB_MULTI_PIPE": 1},
+         "penta_pipe_g3": {"FFB_STEPS_PER_PASS": 5, "FFB_PENTA_SHAPE": 1, "FFB_MULTI_PIPE": 1, "FFB_MULTI_GRID": 3},
+         "quad_pipe_g1": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 2, "FFB_MULTI_PIPE": 1, "FFB_MULTI_GRID": 1},
+         "quad_pipe_g3": {"FFB_STEPS_PER_PASS": 4, "FFB_QUAD_SHAPE": 2, "FFB_MULTI_PIPE": 1, "FFB_MULTI_GRID": 3}}
 
 
 @pytest.mark.parametrize("kind", list(KINDS))
@@ -77,8 +95,9 @@ def test_pairs_equal_single_steps_bitwise(monkeypatch, mode, form, shape, kind):
     f0 = _random_state(100 * mode + nx + 7 * ny, precision, nx, ny)
     # triples: 3 (one triple), 8 (two triples and a pair), 7 (two triples and a single step);
     # four steps: 4, 11 (two and a triple), 9 (two and a single step)
-    steps_of = {"triple": (3, 8, 7), "quad": (4, 11, 9)}
-    for steps in steps_of.get(kind.rstrip("0123456789x"), (2, 7)):
+    # five steps: 5, 14 (two and a four), 12 (two and a pair)
+    steps_of = {"triple": (3, 8, 7), "quad": (4, 11, 9), "penta": (5, 14, 12)}
+    for steps in steps_of.get(next((k for k in steps_of if kind.startswith(k)), None), (2, 7)):
         fa, ma, na = _run(monkeypatch, True, mode, nx, ny, precision, scale, f0, steps, KINDS[kind])
         fb, mb, nb = _run(monkeypatch, False, mode, nx, ny, precision, scale, f0, steps)
         assert na == nb == steps
