@@ -874,10 +874,10 @@ int ffb_batch_autotune(ffb_batch_t *B, int32_t trial_steps, ffb_batch_tuning_t *
     BHIPCHK(hipSetDevice(B->device));
     // Candidates of the kernel the batch runs for consecutive steps, each timed
     // by direct launches (no graph is captured until the winner is known):
-    //   step pairs: region shape (62 x 14 or 62 x 30 tiles; FP64 one shape) x
-    //     XCD group (4, 6, 8, 12, 16, 32, 64 tiles) x grid (one workgroup per tile, or a
-    //     persistent grid of 1, 2 or 4 rounds of the workgroups a CU holds:
-    //     4 of the 8-wave shape, 2 of the 16-wave one)
+    //   multi-step launches: region shape (1: 64 x 32, 2: 64 x 16 or 64 x 24;
+    //     FP64 one shape) x XCD group (4, 6, 8, 12, 16, 32, 64 tiles) x grid (one
+    //     workgroup per tile, or a persistent grid of 1, 2 or 4 rounds of 2 or 4
+    //     workgroups per CU)
     //   one launch per step: workgroup size (64 .. 1024 threads) x persistent
     //     grid (0: one workgroup per tile; 2 .. 384 workgroups per CU)
     std::vector<ffb_batch_tuning_t> cands;
