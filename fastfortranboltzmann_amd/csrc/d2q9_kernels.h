@@ -529,12 +529,30 @@ __device__ __forceinline__ void collide_guo(const StepArgs<C> &a, C (&f)[9], int
     const C om = a.omega;
     const C one = 1, three = 3, c45 = (C)4.5, c15 = (C)1.5, nine = 9;
     const C pre = (one - (C)0.5 * om) * F;  // (1 - omega/2) F
+    // Per direction, with c = c_d:
+    //   cu  = c_x u_x + c_y u_y,
+    //   feq = w rho (((1 + 3 cu) + 4.5 cu cu) - 1.5 u^2),
+    //   src = w pre (3 (c_x - u_x) + 9 cu c_x),
+    //   f'  = f + omega (feq - f) + src.
+    // Products by a zero velocity component are written out of these
+    // expressions (x * 0 may not be dropped by the compiler): c_x = 0 gives
+    // 3 (0 - u_x) + 9 cu 0 = 3 (-u_x) up to the sign of a zero, and a zero
+    // component adds a signed zero to cu; products by +-1 the compiler folds
+    // itself.  Every rounded operation that remains is the one of the full
+    // expressions, so the values are theirs (a zero's sign can differ only in
+    // a term whose sum with a non-zero population is unchanged by it).
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
-        const C cx = (C)cx_of(d), cy = (C)cy_of(d);
-        const C cu = cx * ux + cy * uy;
-        const C feq = a.w[d] * rho * (((one + three * cu) + c45 * cu * cu) - c15 * usq);
-        const C src = a.w[d] * pre * (three * (cx - ux) + nine * cu * cx);
+        const int ix = cx_of(d), iy = cy_of(d);
+        C cu;
+        if (ix == 0 && iy == 0) cu = (C)0;
+        else if (ix == 0) cu = (C)iy * uy;
+        else if (iy == 0) cu = (C)ix * ux;
+        else cu = (C)ix * ux + (C)iy * uy;
+        const C w = a.w[d == 0 ? 0 : (d < 5 ? 1 : 5)];  // the weight of the class (equal values)
+        const C feq = w * rho * (((one + three * cu) + c45 * cu * cu) - c15 * usq);
+        const C g = ix == 0 ? three * -ux : three * ((C)ix - ux) + nine * cu * (C)ix;
+        const C src = w * pre * g;
         f[d] = f[d] + om * (feq - f[d]) + src;
     }
 }

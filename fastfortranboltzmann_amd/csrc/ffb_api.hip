@@ -350,7 +350,8 @@ template <typename S, typename C, int WAVES, int RPT> int launch_step3_shape(ffb
 }
 
 // Region shapes of the triples: 2 = 8 waves x 2 rows (64 x 16 cells, 60 x 12
-// outputs), 1 = 16 waves x 2 rows (64 x 32, 60 x 28; FP64 73.7 KB of LDS).
+// outputs), 1 = 16 waves x 2 rows (64 x 32, 60 x 28; FP64 73.7 KB of LDS),
+// 3 = 8 waves x 4 rows (64 x 32, 60 x 28; 4-byte and narrower elements).
 template <typename S, typename C> int launch_step3(ffb_lattice *L, int write_macro) {
     const StepArgs<C> a = make_args<C>(L, write_macro);
     if (L->triple_shape == 1) return launch_step3_shape<S, C, 16, 2>(L, a);
@@ -801,10 +802,13 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     L->penta_shape = (int)env_ll("FFB_PENTA_SHAPE", 1);
     // LDS bytes added to each multi-step workgroup (caps the workgroups per CU)
     L->lds_pad = (unsigned)std::max(0LL, std::min(65536LL, env_ll("FFB_MULTI_LDS_PAD", 0)));
-    // multi-step launches (three or four steps) by a persistent grid that
-    // loads each workgroup's next tile while it runs the current one's LDS
-    // steps (stepk_pipe_kernel), FFB_MULTI_PIPE; FFB_MULTI_GRID workgroups
-    // (0: as many as fit at once)
+    // multi-step launches (three to five steps, the default region shapes) by
+    // a persistent grid that loads each workgroup's next tile while it runs the
+    // current one's LDS steps (stepk_pipe_kernel), FFB_MULTI_PIPE; FFB_MULTI_GRID
+    // workgroups (0: as many as fit at once).  Bitwise equal and measured
+    // slower (FP32 4096^2 four steps 365.8 against 325.3 us per launch): the
+    // multi-step launches keep the vector ALUs busy, so there is no idle time
+    // for the early loads to fill.
     L->multi_pipe = (int)env_ll("FFB_MULTI_PIPE", 0) ? 1 : 0;
     L->multi_grid = (int)std::max(0LL, env_ll("FFB_MULTI_GRID", 0));
     {

@@ -136,11 +136,12 @@ int ffb_init(ffb_lattice_t *lat);
  * exchange_halos and the Poiseuille pre-pass, then execute_local_sim_step,
  * simulation.f90:18-73).  When write_macro_last != 0 the last step also
  * writes rho, u_x, u_y (the reference's write_macro_fields).  In every form
- * (not in Poiseuille flow) consecutive steps run in triples, three updates per
- * kernel launch, with a pair or a single step for a remainder; the results are
- * bitwise those of one launch per step (FFB_STEPS_PER_PASS=1/2/3 at
- * ffb_create selects one launch per step, pairs or triples; FFB_TWO_STEP=0 one
- * launch per step). */
+ * (not in Poiseuille flow) consecutive steps run several per kernel launch
+ * (five for FP32 and FP32 deviations, three for FP64, FP16 and BF16
+ * deviations, two for the 1-3-12 form), with fewer for a remainder; the
+ * results are bitwise those of one launch per step (FFB_STEPS_PER_PASS=1..5 at
+ * ffb_create selects the steps per launch; FFB_TWO_STEP=0 one launch per
+ * step). */
 int ffb_step(ffb_lattice_t *lat, int32_t n_steps, int32_t write_macro_last);
 
 /* Changes the physical parameters of an existing handle (the next steps use
@@ -241,13 +242,13 @@ int ffb_bench_copy(int32_t device, int64_t bytes, int32_t iters, double *gbps);
 /* ------------------------------------------------------------------------
  * Batched form (no reference counterpart: the reference runs one domain per
  * process).  Many independent lattices of uneven size, each with its own
- * omega and boundary kind, advance together: in the 4- and 8-byte forms
- * consecutive steps run as step triples, three steps per launch of one kernel
- * over the tiles of all lattices (FFB_BATCH_STEPS_PER_PASS=2 at creation: step
- * pairs), in the 16-bit forms one step per launch of one kernel over the row
- * segments of all lattices (all bitwise the same; FFB_BATCH_PAIRS=0/1 at
- * creation selects one launch per step or multi-step launches; either kernel
- * runs one workgroup per tile or a persistent grid), and the launches of a
+ * omega and boundary kind, advance together: consecutive steps run several
+ * per launch of one kernel over the tiles of all lattices (five for the 4-byte
+ * forms, three for FP64 and the 16-bit forms; FFB_BATCH_STEPS_PER_PASS=2..5 at
+ * creation selects another count), or with FFB_BATCH_PAIRS=0 one step per
+ * launch of one kernel over the row segments of all lattices (all bitwise the
+ * same; either kernel runs one workgroup per tile or a persistent grid), and
+ * the launches of a
  * ping-pong cycle are captured once in a hipGraph and replayed.  ffb_batch_autotune()
  * picks the launch shape of the kernel the batch runs by timing the
  * candidates on the batch.
@@ -269,9 +270,10 @@ typedef struct ffb_batch_tuning {
     /* one launch per step (also a remainder of a multi-step batch) */
     int32_t tile_w;       /* threads per workgroup (64-cell row segments x 64): 64 .. 1024 */
     int32_t grid_blocks;  /* persistent grid size; 0 = one workgroup per tile */
-    /* step pairs and triples */
-    int32_t pair_shape;   /* 2: 8-wave regions (pairs 62 x 14, triples 60 x 12 output tiles), 1: 16 waves
-                             (62 x 30, 60 x 28); FP64 one shape (62 x 6, 60 x 12) */
+    /* multi-step launches (K = 2 .. 5 steps each) */
+    int32_t pair_shape;   /* 1: 64 x 32 regions (output tiles (64 - 2(K-1)) x (32 - 2(K-1)); 8 waves x 4
+                             rows for three or more steps, 16 x 2 for pairs), 2: 64 x 16 regions (8 waves x
+                             2 rows; four and five steps 64 x 24, 12 x 2); FP64 one shape (62 x 6, 60 x 12) */
     int32_t pair_group;   /* consecutive tiles per XCD (0: dispatch order) */
     int32_t pair_grid;    /* persistent grid size (rounded up to a multiple of 8); 0 = one workgroup per tile */
     int32_t reserved;
@@ -300,7 +302,7 @@ int ffb_batch_reset_timer(ffb_batch_t *batch);
 int ffb_batch_info(ffb_batch_t *batch, int64_t *cells, int32_t *tiles, ffb_batch_tuning_t *tuning);
 int ffb_batch_set_tuning(ffb_batch_t *batch, const ffb_batch_tuning_t *tuning);
 /* Times every candidate launch shape of the kernel the batch runs (step
- * pairs or triples: pair_shape x pair_group x pair_grid; one launch per step: tile_w x
+ * multi-step launches: pair_shape x pair_group x pair_grid; one launch per step: tile_w x
  * grid_blocks) for trial_steps steps, keeps the fastest, and re-initialises
  * the batch (the trials advance it).  *best_ms_per_step is per step. */
 int ffb_batch_autotune(ffb_batch_t *batch, int32_t trial_steps, ffb_batch_tuning_t *best, double *best_ms_per_step);

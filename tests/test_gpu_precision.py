@@ -79,3 +79,19 @@ def test_16bit_storage_error_against_fp64_cavity():
     for form, bound in (("fp32", 1e-4), ("fp32dev", 1e-4), ("fp16dev", 2e-3), ("bf16dev", 5e-3),
                         ("e3m12dev", 2e-3)):
         assert f[form]["shear_wave"]["nu_rel_err"] < bound, (form, f[form]["shear_wave"])
+
+
+def test_error_target_met_on_1024_cavity():
+    """The precision target, relative L2 velocity error <= 1e-4 against FP64,
+    on the cavity 1024^2, Re 1000, 2000 steps (the bench's error case): met by
+    the 1-3-12 deviations (16 bits, stochastic rounding; measured 8.16e-5) and
+    the FP32 deviations (measured 1.81e-6).  FP16 deviations (2.85e-4) do not
+    meet it and are not asserted here."""
+    from fastfortranboltzmann_amd.precision import precision_study
+
+    print("[gpu] precision target 1024^2", flush=True)
+    res = precision_study(1024, 1000.0, 2000, forms=("e3m12dev", "fp32dev"), log=lambda m: None)
+    f = res["forms"]
+    print({k: v.get("rel_l2_u") for k, v in f.items()}, flush=True)
+    assert f["e3m12dev"]["rel_l2_u"] <= 1e-4
+    assert f["fp32dev"]["rel_l2_u"] <= 1e-4
