@@ -5,7 +5,7 @@ bounce-back on three walls and a moving-wall lid at u_lid = 0.1, omega from
 Re = 1000 (omega = 1/(3*0.1*4096/1000 + 1/2)), rho_0 = 1, fluid at rest,
 FP32 populations.  One "step" is one fused pull-stream-collide update of the
 whole domain; the library runs consecutive steps five per kernel launch
-(FP32; three for FP64; temporal blocking, bitwise the same result; fewer for
+(FP32; four for FP64; temporal blocking, bitwise the same result; fewer for
 a remainder), and the line also reports step pairs ("pairs") and one launch
 per step ("single_pass").  Inputs are resident in
 HBM before timing.
@@ -98,9 +98,9 @@ class pass_mode:
 
 def default_steps_per_pass(precision: str) -> int:
     """The library's default steps per launch (FFB_STEPS_PER_PASS unset): five
-    for FP32 and FP32 deviations, two for the 1-3-12 deviations, three for
-    FP64, FP16 and BF16 deviations."""
-    return 5 if precision in ("fp32", "fp32dev") else (2 if precision == "e3m12dev" else 3)
+    for FP32 and FP32 deviations, four for FP64, two for the 1-3-12
+    deviations, three for FP16 and BF16 deviations."""
+    return {"fp32": 5, "fp32dev": 5, "fp64": 4, "e3m12dev": 2}.get(precision, 3)
 
 
 def steps_per_pass(precision: str) -> int:

@@ -442,12 +442,22 @@ __device__ __forceinline__ void pull_edge_cell(const StepArgs<C> &a, const typen
 // magnitude, as the densities and momenta of a step are, so each quotient is
 // bitwise the one a / b gives there, and the reciprocal's refinement is
 // computed once per denominator (9 instructions less for the two velocity
-// quotients of a cell).  FP64 keeps the division operator.
+// quotients of a cell).  FP64: the compiler's sequence is a reciprocal, two
+// Newton steps and one quotient correction (11 instructions less per cell).
 template <typename C>
-struct Quot {
-    C b;
-    __device__ __forceinline__ explicit Quot(C den) : b(den) {}
-    __device__ __forceinline__ C operator()(C a) const { return a / b; }
+struct Quot;
+template <>
+struct Quot<double> {
+    double b, y;
+    __device__ __forceinline__ explicit Quot(double den) : b(den) {
+        const double y0 = __builtin_amdgcn_rcp(den);
+        const double y1 = __builtin_fma(y0, __builtin_fma(-den, y0, 1.0), y0);
+        y = __builtin_fma(y1, __builtin_fma(-den, y1, 1.0), y1);
+    }
+    __device__ __forceinline__ double operator()(double a) const {
+        const double q = a * y;
+        return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+    }
 };
 template <>
 struct Quot<float> {

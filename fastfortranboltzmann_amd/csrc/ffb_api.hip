@@ -793,12 +793,16 @@ int ffb_create(const ffb_config_t *cfg, ffb_lattice_t **out) {
     // FP16 deviations 8192^2 230,400 MLUPS in triples against 196,400 with 64 x
     // 16 regions (223,300 in pairs, 224,000 in fours); the 1-3-12 form 206,200
     // in pairs against 200,500 in triples and 174,400 with 64 x 16 triples.
+    // FP64: four steps in 64 x 16 regions (8 waves x 2 rows, 58 x 10 tiles)
+    // 115,200 MLUPS at 4096^2 against 107,600 in triples (with the shared
+    // reciprocal of the velocity quotients; 110,600 against 107,900 before).
     {
         const bool four = cfg->precision == FFB_PREC_FP32 || cfg->precision == FFB_PREC_FP32_DEV;
-        const int dflt = four ? 5 : (cfg->precision == FFB_PREC_E3M12_DEV ? 2 : 3);
+        const bool fp64 = cfg->precision == FFB_PREC_FP64;
+        const int dflt = four ? 5 : (fp64 ? 4 : (cfg->precision == FFB_PREC_E3M12_DEV ? 2 : 3));
         L->steps_per_pass = (int)std::max(1LL, std::min(four ? 5LL : 4LL, env_ll("FFB_STEPS_PER_PASS", dflt)));
     }
-    L->quad_shape = (int)env_ll("FFB_QUAD_SHAPE", 2);
+    L->quad_shape = (int)env_ll("FFB_QUAD_SHAPE", cfg->precision == FFB_PREC_FP64 ? 3 : 2);
     L->penta_shape = (int)env_ll("FFB_PENTA_SHAPE", 1);
     // LDS bytes added to each multi-step workgroup (caps the workgroups per CU)
     L->lds_pad = (unsigned)std::max(0LL, std::min(65536LL, env_ll("FFB_MULTI_LDS_PAD", 0)));

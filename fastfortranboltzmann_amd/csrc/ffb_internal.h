@@ -49,7 +49,8 @@ struct ffb_lattice {
     int two_step = 0;        // 1: step pairs in one pass (step2_kernel) where the mode allows
     int steps_per_pass = 2;  // 3: step triples (stepk_kernel) where pairs are allowed
     int triple_shape = 2;    // triple region: 2 = 8 waves x 2 rows, 1 = 16 waves x 2 rows, 3 = 8 x 4 (<= 4-byte elements)
-    int quad_shape = 2;      // four-step region: 2 = 12 waves x 2 rows, 1 = 16 x 2, 3 = 8 x 2, 4 = 8 x 4 (<= 4-byte elements)
+    int quad_shape = 2;      // four-step region: 2 = 12 waves x 2 rows (default), 1 = 16 x 2, 3 = 8 x 2 (FP64 default),
+                             // 4 = 8 x 4 (<= 4-byte elements)
     int penta_shape = 1;     // five-step region (4-byte elements): 1 = 8 waves x 4 rows, 2 = 12 x 2
     unsigned lds_pad = 0;    // dynamic LDS bytes added to multi-step launches (fewer workgroups per CU)
     int multi_pipe = 0;      // 1: three/four-step launches by stepk_pipe_kernel (persistent, loads one tile ahead)

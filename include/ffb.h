@@ -137,7 +137,7 @@ int ffb_init(ffb_lattice_t *lat);
  * simulation.f90:18-73).  When write_macro_last != 0 the last step also
  * writes rho, u_x, u_y (the reference's write_macro_fields).  In every form
  * (not in Poiseuille flow) consecutive steps run several per kernel launch
- * (five for FP32 and FP32 deviations, three for FP64, FP16 and BF16
+ * (five for FP32 and FP32 deviations, four for FP64, three for FP16 and BF16
  * deviations, two for the 1-3-12 form), with fewer for a remainder; the
  * results are bitwise those of one launch per step (FFB_STEPS_PER_PASS=1..5 at
  * ffb_create selects the steps per launch; FFB_TWO_STEP=0 one launch per

@@ -25,7 +25,7 @@ def test_pass_mode_sets_and_restores_the_switch(monkeypatch):
     monkeypatch.delenv("FFB_TWO_STEP", raising=False)
     monkeypatch.delenv("FFB_STEPS_PER_PASS", raising=False)
     forms = ("fp32", "fp64", "fp16dev", "bf16dev", "fp32dev", "e3m12dev")
-    assert [bench.steps_per_pass(p) for p in forms] == [5, 3, 3, 3, 5, 2]
+    assert [bench.steps_per_pass(p) for p in forms] == [5, 4, 3, 3, 5, 2]
     with bench.pass_mode(1):
         assert os.environ["FFB_TWO_STEP"] == "0" and bench.steps_per_pass("fp32") == 1
     assert "FFB_TWO_STEP" not in os.environ and "FFB_STEPS_PER_PASS" not in os.environ

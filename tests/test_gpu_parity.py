@@ -228,32 +228,35 @@ def test_gpu_cavity_4096_fp64_step_1000_matches_oracle():
     np.testing.assert_array_equal(gux, o.u_x)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("steps,decades", [(1, 3.0), (5, 0.3)], ids=["1step-6decades", "5steps-0.6decades"])
 @pytest.mark.parametrize("mode", [1, 4, 5])
-def test_fp32_quotients_over_wide_density_range(mode, steps, decades):
-    """FP32 velocity quotients u = j / rho over a wide range of densities and
+def test_quotients_over_wide_density_range(mode, steps, decades, precision):
+    """FP32 and FP64 velocity quotients u = j / rho over a wide range of densities and
     momenta: a random state whose cells carry scales 10^U(-decades, decades),
     so that a cell's pulled populations differ by up to 2 * decades decades
     (rho from about 1e-3 to 1e3 and |u| up to about 1 for one step).  The
     kernels divide with a shared refined reciprocal (the compiler's division
     without its range scaling); the oracle divides with IEEE division.  One
     step (every pulled value positive, so rho > 0) in a single-step launch,
-    and five steps in one five-step launch from a milder state; bitwise equal
-    in the shear wave, the sliding lid and the Guo-forced channel."""
+    and five steps from a milder state (FP32: one five-step launch; FP64: a
+    triple and a pair); bitwise equal in the shear wave, the sliding lid and
+    the Guo-forced channel."""
     from fastfortranboltzmann_amd import Lattice, Settings
 
     nx, ny = 150, 61
-    print(f"[gpu] wide-range quotients mode {mode} steps {steps} decades {decades}", flush=True)
+    print(f"[gpu] wide-range quotients mode {mode} steps {steps} decades {decades} {precision}", flush=True)
+    dt = np.float64 if precision == "fp64" else np.float32
     rng = np.random.default_rng(4321 + mode)
     w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
     scale = 10.0 ** rng.uniform(-decades, decades, (1, ny + 2, nx + 2))
-    f0 = (w[:, None, None] * scale * (1 + 0.3 * rng.uniform(-1, 1, (9, ny + 2, nx + 2)))).astype(np.float32)
+    f0 = (w[:, None, None] * scale * (1 + 0.3 * rng.uniform(-1, 1, (9, ny + 2, nx + 2)))).astype(dt)
     p = Params(omega=0.83, u_lid=0.12, force=3e-4)
-    o = OracleLattice(mode, nx, ny, p, np.float32).init()
+    o = OracleLattice(mode, nx, ny, p, dt).init()
     o.f[:] = f0
     o.run(steps)
     s = Settings(N_X=nx, N_Y=ny, SIM_MODE=mode, OMEGA=p.omega, U_LID=p.u_lid, BODY_FORCE=p.force,
-                 precision="fp32")
+                 precision=precision)
     with Lattice(s) as lat:
         lat.init()
         lat.set_populations(f0)
