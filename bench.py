@@ -18,7 +18,7 @@ is the bytes one launch must move -- one read and one write of the stored
 state, 18*s bytes per cell, whatever the number of updates the launch
 performs -- over the launch's average duration (HIP events on the library's
 stream, over whole launches); `effective_gbs` is the per-update rate
-18*s*cells*steps/time, above the peak for step pairs and triples, and not a
+18*s*cells*steps/time, above the peak for multi-step launches, and not a
 roofline figure.  `roofline.traffic` is the
 HBM traffic per launch measured by rocprofv3 (profiles/traffic.txt), reported
 only when that file was written from the kernel sources being benched.  The
@@ -73,8 +73,8 @@ def kernel_source_hash() -> str:
 
 
 class pass_mode:
-    """Lattices created inside run k steps per launch: 3 (step triples), 2
-    (step pairs), 1 (one launch per step), or the library's default (None)."""
+    """Lattices created inside run k steps per launch (1 = one launch per
+    step, 2 = step pairs, up to 5), or the library's default (None)."""
 
     KEYS = ("FFB_STEPS_PER_PASS", "FFB_TWO_STEP")
 
@@ -312,8 +312,8 @@ def shear_wave_256(steps: int = 2000, chunk: int = 200) -> dict:
 
 def batched(single_fp32_mlups: float, single_pass_fp32_mlups: float, steps: int = 202) -> dict:
     """512 mixed lattices (sides 64..1024, omega 0.6..1.9, shear wave /
-    Couette / body-force channel), autotuned: FP32 (step triples), and the
-    1-3-12 and FP16 deviation forms (one launch per step), each with the
+    Couette / body-force channel), autotuned: FP32 (five steps per launch),
+    and the 1-3-12 and FP16 deviation forms (three per launch), each with the
     kernel and launch shape that ran, and every lattice's relative L2
     velocity error against an FP64 batch of the same lattices after the same
     steps (median / p90 / max)."""
