@@ -228,6 +228,15 @@ def roofline(run: dict, workload: str) -> dict:
         # wave64 instruction takes a SIMD 4 cycles): the bound of launches
         # that trade HBM traffic for arithmetic
         out["valu"] = {"wave_insts_per_launch": va["wave_insts_per_launch"], "busy": va["busy"]}
+        # what holds the launch back: the vector ALUs when their instructions fill
+        # the SIMDs' cycles, HBM when the measured traffic runs at 0.7 of the
+        # peak or more (the copy ceiling is about 0.84), else neither
+        if va["busy"] >= 0.9:
+            out["limiter"] = "valu"
+        elif tr and tr["bytes"] / (run["kernel_ms"] * 1e-3) / 1e9 >= 0.7 * HBM_PEAK_GBS:
+            out["limiter"] = "hbm"
+        else:
+            out["limiter"] = "neither"
     return out
 
 

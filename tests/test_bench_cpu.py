@@ -135,6 +135,9 @@ def test_valu_record_from_counters(monkeypatch, tmp_path):
     v = bench.valu_from_profiles("w1")
     assert v["wave_insts_per_launch"] == 1.1e8 and v["kernel"] == "stepk_kernel<a>"
     assert abs(v["busy"] - round(1.1e8 * 4 / (1024 * 500000.0), 3)) < 1e-9
+    run = {"achieved_gbs": 3000.0, "bytes_per_launch": 1, "steps_per_launch": 5, "kernel_ms": 0.4}
+    assert bench.roofline(run, "w1")["limiter"] == "neither"  # busy 0.859, no traffic record
+    assert bench.roofline(run, "no_such_workload").get("limiter") is None
     text = (tmp_path / "valu.txt").read_text().replace(bench.kernel_source_hash(), "0123456789abcdef")
     (tmp_path / "valu.txt").write_text(text)
     assert bench.valu_from_profiles("w1") is None  # stamped with other sources: refused
