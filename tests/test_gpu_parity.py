@@ -202,25 +202,28 @@ def test_gpu_mass_conservation_and_moments():
 
 
 @pytest.mark.slow
-def test_gpu_cavity_4096_fp64_step_1000_matches_oracle():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gpu_cavity_4096_step_1000_matches_oracle(precision):
     """The cavity workload at full size: 4096^2, Re 1000 (omega from
-    calculate_reynolds, settings.f90:166-172), u_lid 0.1, FP64, 1000 steps.
-    Bar: max|df| <= 1e-12 against the reference restated by the oracle; the
-    kernels are bitwise equal, so the test asserts 0."""
+    calculate_reynolds, settings.f90:166-172), u_lid 0.1, 1000 steps, through
+    the default multi-step launches (FP64 four steps, FP32 five).  Bar (FP64):
+    max|df| <= 1e-12 against the reference restated by the oracle; the
+    kernels are bitwise equal, so the test asserts 0 in both precisions."""
     import os
 
     from oracle.oracle import set_threads
     from fastfortranboltzmann_amd import omega_for_reynolds
 
     n, steps = 4096, 1000
+    dt = np.float64 if precision == "fp64" else np.float32
     p = Params(omega=omega_for_reynolds(1000.0, 0.1, n), u_lid=0.1)
-    print(f"[gpu] cavity {n}^2 fp64 {steps} steps vs oracle", flush=True)
-    with _gpu_lattice(4, n, n, p, "fp64") as lat:
+    print(f"[gpu] cavity {n}^2 {precision} {steps} steps vs oracle", flush=True)
+    with _gpu_lattice(4, n, n, p, precision) as lat:
         lat.init().step(steps, write_macro_last=True)
         g = lat.interior()
         _, gux, _ = lat.macros()
     set_threads(min(16, os.cpu_count() or 1))
-    o = OracleLattice(4, n, n, p, np.float64).init().run(steps)
+    o = OracleLattice(4, n, n, p, dt).init().run(steps)
     dmax = float(np.abs(g - o.interior).max())
     print(f"[gpu] cavity {n}^2 max|df| = {dmax:.3e}, max|u_x| = {float(np.abs(gux).max()):.4f}", flush=True)
     assert dmax <= 1e-12
